@@ -1,0 +1,258 @@
+/*
+ * msv.h -- C ABI of the MI355X (gfx950) vector and segment primitive engine.
+ *
+ * Drop-in boundary for the `massively` crate's device-resident primitives.
+ * Every function returns an msv_status; details of the last failure on the
+ * calling thread are available from msv_last_error(). No C++ exception and
+ * no GPU fault crosses this boundary for a rejected argument.
+ *
+ * Reference interfaces replaced (paths relative to the massively repository):
+ *   Executor / DeviceVec / DeviceSlice   massively/src/core/runtime.rs:21-545
+ *   Error                                massively/src/core/error.rs:5-13
+ *   vector::*                            massively/src/api/algorithm (cited per function)
+ *   seg::ForEachSegment / Segmentation   massively/src/seg/mod.rs, seg/segmentation.rs
+ *
+ * Lengths are MIndex (= uint32_t, massively/src/lib.rs:79-82). Flags (MFlag)
+ * are uint32_t and true iff nonzero (massively/src/flag.rs:9-19).
+ *
+ * Threading: one executor owns one HIP stream; calls on one executor must be
+ * externally serialised. Work is enqueued asynchronously; functions that
+ * return a host value (msv_reduce, msv_to_host, msv_vec_len of a vector with
+ * a device-resident length, segmentation constructors) synchronise.
+ */
+#ifndef MSV_H
+#define MSV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSV_ABI_VERSION 1u
+
+/* Status codes; 1..7 map one-to-one onto massively::Error
+ * (massively/src/core/error.rs:5-13). */
+typedef enum msv_status {
+  MSV_OK = 0,
+  MSV_ERR_LENGTH_MISMATCH = 1,      /* LengthMismatch { left, right } */
+  MSV_ERR_OUTPUT_TOO_SHORT = 2,     /* OutputTooShort { input, output } */
+  MSV_ERR_LENGTH_TOO_LARGE = 3,     /* LengthTooLarge { len } */
+  MSV_ERR_INVALID_SEGMENTATION = 4, /* InvalidSegmentation */
+  MSV_ERR_UNBOUND_COLUMN = 5,       /* UnboundColumn */
+  MSV_ERR_FOREIGN_EXECUTOR = 6,     /* ForeignExecutor */
+  MSV_ERR_LAUNCH = 7,               /* Launch { message } (HIP runtime failure) */
+  MSV_ERR_INVALID_ARGUMENT = 8,     /* null handle, bad dtype/op combination, bad range */
+  MSV_ERR_UNSUPPORTED = 9,          /* valid request this build does not serve */
+  MSV_ERR_OUT_OF_MEMORY = 10
+} msv_status;
+
+typedef struct msv_error {
+  int32_t code;       /* msv_status */
+  uint64_t left;      /* LengthMismatch.left / OutputTooShort.input / LengthTooLarge.len */
+  uint64_t right;     /* LengthMismatch.right / OutputTooShort.output */
+  char message[256];
+} msv_error;
+
+typedef enum msv_dtype {
+  MSV_U32 = 0,
+  MSV_I32 = 1,
+  MSV_F32 = 2,
+  MSV_U64 = 3,
+  MSV_I64 = 4,
+  MSV_F64 = 5
+} msv_dtype;
+
+/* Built-in ReductionOp set (massively/src/core/reduce.rs:35-66). XOR is
+ * integer-only. Integer ADD wraps modulo 2^bits. MIN(a,b) = b<a ? b : a. */
+typedef enum msv_op { MSV_OP_ADD = 0, MSV_OP_MIN = 1, MSV_OP_MAX = 2, MSV_OP_XOR = 3 } msv_op;
+
+/* Built-in BinaryPredicateOp key comparisons (massively/src/core/ordering/mod.rs:29-65). */
+typedef enum msv_cmp { MSV_CMP_LESS = 0, MSV_CMP_GREATER = 1 } msv_cmp;
+
+/* Built-in PredicateOp set (massively/src/core/predicate.rs:40-52), evaluated
+ * on an element x of the stencil column. Thresholds `a`/`b` hold the bit
+ * pattern of a value of the stencil's dtype (zero-extended to 64 bits).
+ *   NONZERO      x != 0          ZERO       x == 0
+ *   LT/LE/GT/GE  x < a, x <= a, x > a, x >= a (numeric compare in the dtype)
+ *   BITS_ANY     (bits(x) & a) != 0
+ *   BITS_EQ      (bits(x) & a) == b                                       */
+typedef enum msv_pred_kind {
+  MSV_PRED_NONZERO = 0,
+  MSV_PRED_ZERO = 1,
+  MSV_PRED_LT = 2,
+  MSV_PRED_LE = 3,
+  MSV_PRED_GT = 4,
+  MSV_PRED_GE = 5,
+  MSV_PRED_BITS_ANY = 6,
+  MSV_PRED_BITS_EQ = 7
+} msv_pred_kind;
+
+typedef struct msv_predicate {
+  uint32_t kind; /* msv_pred_kind */
+  uint32_t reserved;
+  uint64_t a;
+  uint64_t b;
+} msv_predicate;
+
+typedef struct msv_executor msv_executor; /* Executor<R> */
+typedef struct msv_vec msv_vec;           /* DeviceVec<R,T> / MVec (refcounted) */
+typedef struct msv_segmentation msv_segmentation; /* seg::Segmentation<R> */
+
+/* DeviceSlice / DeviceSliceMut: a by-value view {vec, offset, len}. `len`
+ * MSV_WHOLE means "to the end of the vector's logical length". */
+#define MSV_WHOLE 0xffffffffu
+typedef struct msv_slice {
+  msv_vec* vec;
+  uint32_t offset;
+  uint32_t len;
+} msv_slice;
+
+/* ---- errors and versions --------------------------------------------- */
+uint32_t msv_abi_version(void);
+/* Copies the calling thread's last error into *out (code MSV_OK if none). */
+void msv_last_error(msv_error* out);
+
+/* ---- executor (massively/src/core/runtime.rs:21-197) ------------------- */
+/* Executor::new: binds `device`, creates one stream and a caching allocator. */
+msv_status msv_executor_create(int32_t device, msv_executor** out);
+msv_status msv_executor_destroy(msv_executor* ex);
+/* Executor::sync (runtime.rs:192-196). */
+msv_status msv_executor_sync(msv_executor* ex);
+/* Monotonic executor id used for ForeignExecutor checks (runtime.rs:12,49-54). */
+uint64_t msv_executor_id(const msv_executor* ex);
+/* The executor's HIP stream (hipStream_t), for callers that interleave their own work. */
+void* msv_executor_stream(msv_executor* ex);
+/* Returns cached device memory to the driver (synchronises). */
+msv_status msv_executor_trim(msv_executor* ex);
+
+/* ---- device vectors (runtime.rs:79-177, allocation.rs:852-861) -------- */
+/* Executor::alloc: uninitialised vector of `len` elements. */
+msv_status msv_alloc(msv_executor* ex, msv_dtype dtype, uint32_t len, msv_vec** out);
+/* Executor::to_device: copies `len` host elements into a new vector. */
+msv_status msv_to_device(msv_executor* ex, msv_dtype dtype, const void* host, uint32_t len,
+                         msv_vec** out);
+/* Executor::to_host: copies the slice (clipped to a device-resident logical
+ * length, core/extent.rs:17-36) to `host` (capacity `host_cap` elements) and
+ * writes the number of elements copied to *out_len. Synchronises. */
+msv_status msv_to_host(msv_executor* ex, msv_slice src, void* host, uint32_t host_cap,
+                       uint32_t* out_len);
+/* Wraps caller-owned device memory (not freed by the library). */
+msv_status msv_wrap_device(msv_executor* ex, msv_dtype dtype, void* device_ptr, uint32_t len,
+                           msv_vec** out);
+/* Shallow Clone (runtime.rs:275-285): shares the allocation. */
+msv_status msv_vec_retain(msv_vec* v);
+msv_status msv_vec_release(msv_vec* v);
+/* Logical length; synchronises only if the length is device-resident. */
+msv_status msv_vec_len(msv_executor* ex, msv_vec* v, uint32_t* out_len);
+uint32_t msv_vec_capacity(const msv_vec* v);
+int32_t msv_vec_dtype(const msv_vec* v);
+void* msv_vec_device_ptr(const msv_vec* v);
+/* Device address of the u32 logical length, or NULL when the length is host-known. */
+void* msv_vec_device_len_ptr(const msv_vec* v);
+
+/* ---- vector:: scans and reductions ----------------------------------- */
+/* vector::inclusive_scan (api/algorithm/scan.rs:96-113; core/scan.rs:1125-1156). */
+msv_status msv_inclusive_scan(msv_executor* ex, msv_slice input, msv_op op, msv_vec** out);
+/* vector::exclusive_scan (api/algorithm/scan.rs:216-257; core/scan.rs:1209-1233).
+ * `init` points to one host element of the input dtype. */
+msv_status msv_exclusive_scan(msv_executor* ex, msv_slice input, const void* init, msv_op op,
+                              msv_vec** out);
+/* vector::reduce (api/algorithm/reduce.rs:32-50): result = op(init, fold(x)),
+ * written to the host element `out_host`. Synchronises. */
+msv_status msv_reduce(msv_executor* ex, msv_slice input, const void* init, msv_op op,
+                      void* out_host);
+
+/* ---- vector:: selection and movement -------------------------------- */
+/* vector::copy_where / remove_where (api/algorithm/selection.rs:146-260): stable
+ * compaction by a u32 stencil. The output has capacity len(input) and a
+ * device-resident logical length (no host sync). */
+msv_status msv_copy_where(msv_executor* ex, msv_slice input, msv_slice stencil, msv_vec** out);
+msv_status msv_remove_where(msv_executor* ex, msv_slice input, msv_slice stencil, msv_vec** out);
+/* copy_where with a built-in predicate evaluated on `stencil` (any dtype;
+ * may be the input itself). `invert` != 0 gives remove_where semantics. */
+msv_status msv_copy_where_pred(msv_executor* ex, msv_slice input, msv_slice stencil,
+                               const msv_predicate* pred, int32_t invert, msv_vec** out);
+/* vector::gather (api/algorithm/indexed.rs:157-208): out[i] = values[indices[i]]. */
+msv_status msv_gather(msv_executor* ex, msv_slice values, msv_slice indices, msv_vec** out);
+/* vector::scatter (api/algorithm/scatter.rs:225-265): output[indices[i]] = values[i].
+ * Indices are unchecked by the reference; this build rejects none but never
+ * writes outside `output` (out-of-range indices are dropped). */
+msv_status msv_scatter(msv_executor* ex, msv_slice values, msv_slice indices, msv_slice output);
+
+/* ---- vector:: ordering ---------------------------------------------- */
+/* vector::radix_sort_by_key (api/algorithm/by_key.rs:252-325; core/radix.rs):
+ * stable ascending sort in integer / IEEE total order. Returns the permuted
+ * values; `out_keys` (nullable) also receives the sorted keys. */
+msv_status msv_radix_sort_by_key(msv_executor* ex, msv_slice keys, msv_slice values,
+                                 msv_vec** out_values, msv_vec** out_keys);
+/* vector::sort_by_key (api/algorithm/by_key.rs:193-250): stable sort by the
+ * built-in comparison; float keys compare with IEEE `<` (-0 == +0). */
+msv_status msv_sort_by_key(msv_executor* ex, msv_slice keys, msv_slice values, msv_cmp cmp,
+                           msv_vec** out_values, msv_vec** out_keys);
+/* vector::sort (api/algorithm/ordering.rs:64-76). */
+msv_status msv_sort(msv_executor* ex, msv_slice input, msv_cmp cmp, msv_vec** out);
+
+/* ---- vector:: merge and ordered set operations ------------------------ */
+/* vector::merge (api/algorithm/merge.rs:109-190): stable, ties take left. */
+msv_status msv_merge(msv_executor* ex, msv_slice left, msv_slice right, msv_cmp cmp,
+                     msv_vec** out);
+/* vector::merge_by_key (api/algorithm/merge.rs:195-282). */
+msv_status msv_merge_by_key(msv_executor* ex, msv_slice left_keys, msv_slice left_values,
+                            msv_slice right_keys, msv_slice right_values, msv_cmp cmp,
+                            msv_vec** out_keys, msv_vec** out_values);
+/* vector::set_union / set_intersection / set_difference
+ * (api/algorithm/set.rs:44-195; multiset rules core/set.rs:18-46). Output
+ * capacity: union L+R, others L; the logical length is device-resident. */
+typedef enum msv_set_kind { MSV_SET_UNION = 0, MSV_SET_INTERSECTION = 1, MSV_SET_DIFFERENCE = 2 } msv_set_kind;
+msv_status msv_set_op(msv_executor* ex, msv_set_kind kind, msv_slice left, msv_slice right,
+                      msv_cmp cmp, msv_vec** out);
+
+/* ---- seg:: ---------------------------------------------------------- */
+/* Segmentation::from_offsets (seg/segmentation.rs:187-218): offsets must be
+ * nonempty, start at 0 and be nondecreasing; the last offset is the value
+ * count. Synchronises (validation result). */
+msv_status msv_segmentation_from_offsets(msv_executor* ex, msv_slice offsets,
+                                         msv_segmentation** out);
+/* Segmentation::from_lengths (seg/segmentation.rs:224-264). */
+msv_status msv_segmentation_from_lengths(msv_executor* ex, msv_slice lengths,
+                                         msv_segmentation** out);
+msv_status msv_segmentation_destroy(msv_segmentation* s);
+/* Segmentation::offsets (shared, retained for the caller) / segment_count / value count. */
+msv_status msv_segmentation_offsets(msv_segmentation* s, msv_vec** out);
+uint32_t msv_segmentation_segment_count(const msv_segmentation* s);
+uint32_t msv_segmentation_value_count(const msv_segmentation* s);
+
+/* ForEachSegment(InclusiveScan(op)) (seg/mod.rs:458-572). */
+msv_status msv_seg_inclusive_scan(msv_executor* ex, msv_slice values,
+                                  const msv_segmentation* seg, msv_op op, msv_vec** out);
+/* ForEachSegment(ExclusiveScan(op, init)): each segment starts from init. */
+msv_status msv_seg_exclusive_scan(msv_executor* ex, msv_slice values,
+                                  const msv_segmentation* seg, const void* init, msv_op op,
+                                  msv_vec** out);
+/* ForEachSegment(Reduce(op, init)) (seg/mod.rs:676-806,1314-1336): one value
+ * per segment, init (+) fold(segment); empty segments yield init. */
+msv_status msv_seg_reduce(msv_executor* ex, msv_slice values, const msv_segmentation* seg,
+                          const void* init, msv_op op, msv_vec** out);
+/* ForEachSegment(Sort(cmp)) (seg/mod.rs:379-425): stable sort inside every
+ * segment by `keys`; `values` (nullable .vec) are permuted alongside. */
+msv_status msv_seg_sort(msv_executor* ex, msv_slice keys, msv_slice values,
+                        const msv_segmentation* seg, msv_cmp cmp, msv_vec** out_keys,
+                        msv_vec** out_values);
+
+/* ---- seeded synthetic inputs (massively/src/api/util/random.rs:12-22,331-396) */
+/* uniform_u32(min, max, seed): pcg_hash32(i ^ seed_key(seed) ^ stream) mapped
+ * to [min, max]; bit-identical to the reference generator. */
+msv_status msv_random_uniform_u32(msv_executor* ex, uint32_t min, uint32_t max, uint64_t seed,
+                                  msv_slice out);
+msv_status msv_random_uniform_u64(msv_executor* ex, uint64_t min, uint64_t max, uint64_t seed,
+                                  msv_slice out);
+/* out[i] = start + i (u32). */
+msv_status msv_iota_u32(msv_executor* ex, uint32_t start, msv_slice out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSV_H */

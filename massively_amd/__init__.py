@@ -1,0 +1,35 @@
+"""massively_amd: MI355X (gfx950) engine for massively's vector and segment primitives.
+
+Host-side mirror of the reference's public API (``Executor``, ``DeviceVec``,
+``DeviceSlice``, ``vector::*``, ``seg::*``, ``Error``) over the C ABI of
+``libmsv.so`` (include/msv.h). All computation runs in hand-written HIP
+kernels; importing this package does not load the library, but every
+operation raises if it is missing.
+"""
+from . import op, seg, vector
+from .error import (
+    Error,
+    ForeignExecutor,
+    InvalidSegmentation,
+    LengthMismatch,
+    LengthTooLarge,
+    OutputTooShort,
+    UnboundColumn,
+)
+from .runtime import DeviceSlice, DeviceVec, Executor
+
+__all__ = [
+    "Executor",
+    "DeviceVec",
+    "DeviceSlice",
+    "Error",
+    "LengthMismatch",
+    "OutputTooShort",
+    "LengthTooLarge",
+    "InvalidSegmentation",
+    "UnboundColumn",
+    "ForeignExecutor",
+    "op",
+    "seg",
+    "vector",
+]

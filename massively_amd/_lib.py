@@ -1,0 +1,160 @@
+"""ctypes binding of libmsv.so (the C ABI declared in include/msv.h).
+
+The library is loaded lazily and the product path fails loudly when it is
+missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("MSV_LIBRARY", PKG / "libmsv.so"))
+
+# msv_status
+OK = 0
+ERR_LENGTH_MISMATCH = 1
+ERR_OUTPUT_TOO_SHORT = 2
+ERR_LENGTH_TOO_LARGE = 3
+ERR_INVALID_SEGMENTATION = 4
+ERR_UNBOUND_COLUMN = 5
+ERR_FOREIGN_EXECUTOR = 6
+ERR_LAUNCH = 7
+ERR_INVALID_ARGUMENT = 8
+ERR_UNSUPPORTED = 9
+ERR_OUT_OF_MEMORY = 10
+
+# msv_dtype
+U32, I32, F32, U64, I64, F64 = range(6)
+NP_DTYPES = {
+    U32: np.dtype(np.uint32),
+    I32: np.dtype(np.int32),
+    F32: np.dtype(np.float32),
+    U64: np.dtype(np.uint64),
+    I64: np.dtype(np.int64),
+    F64: np.dtype(np.float64),
+}
+DTYPE_OF_NP = {v: k for k, v in NP_DTYPES.items()}
+
+WHOLE = 0xFFFFFFFF
+
+
+class MsvError(C.Structure):
+    _fields_ = [
+        ("code", C.c_int32),
+        ("left", C.c_uint64),
+        ("right", C.c_uint64),
+        ("message", C.c_char * 256),
+    ]
+
+
+class MsvSlice(C.Structure):
+    _fields_ = [("vec", C.c_void_p), ("offset", C.c_uint32), ("len", C.c_uint32)]
+
+
+class MsvPredicate(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("a", C.c_uint64),
+        ("b", C.c_uint64),
+    ]
+
+
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+_S = MsvSlice
+_I = C.c_int32
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+
+# name -> (restype, argtypes)
+PROTOTYPES = {
+    "msv_abi_version": (_U32, []),
+    "msv_last_error": (None, [C.POINTER(MsvError)]),
+    "msv_executor_create": (_I, [_I, _PP]),
+    "msv_executor_destroy": (_I, [_P]),
+    "msv_executor_sync": (_I, [_P]),
+    "msv_executor_id": (_U64, [_P]),
+    "msv_executor_stream": (_P, [_P]),
+    "msv_executor_trim": (_I, [_P]),
+    "msv_alloc": (_I, [_P, _I, _U32, _PP]),
+    "msv_to_device": (_I, [_P, _I, _P, _U32, _PP]),
+    "msv_to_host": (_I, [_P, _S, _P, _U32, C.POINTER(_U32)]),
+    "msv_wrap_device": (_I, [_P, _I, _P, _U32, _PP]),
+    "msv_vec_retain": (_I, [_P]),
+    "msv_vec_release": (_I, [_P]),
+    "msv_vec_len": (_I, [_P, _P, C.POINTER(_U32)]),
+    "msv_vec_capacity": (_U32, [_P]),
+    "msv_vec_dtype": (_I, [_P]),
+    "msv_vec_device_ptr": (_P, [_P]),
+    "msv_vec_device_len_ptr": (_P, [_P]),
+    "msv_inclusive_scan": (_I, [_P, _S, _I, _PP]),
+    "msv_exclusive_scan": (_I, [_P, _S, _P, _I, _PP]),
+    "msv_reduce": (_I, [_P, _S, _P, _I, _P]),
+    "msv_copy_where": (_I, [_P, _S, _S, _PP]),
+    "msv_remove_where": (_I, [_P, _S, _S, _PP]),
+    "msv_copy_where_pred": (_I, [_P, _S, _S, C.POINTER(MsvPredicate), _I, _PP]),
+    "msv_gather": (_I, [_P, _S, _S, _PP]),
+    "msv_scatter": (_I, [_P, _S, _S, _S]),
+    "msv_radix_sort_by_key": (_I, [_P, _S, _S, _PP, _PP]),
+    "msv_sort_by_key": (_I, [_P, _S, _S, _I, _PP, _PP]),
+    "msv_sort": (_I, [_P, _S, _I, _PP]),
+    "msv_merge": (_I, [_P, _S, _S, _I, _PP]),
+    "msv_merge_by_key": (_I, [_P, _S, _S, _S, _S, _I, _PP, _PP]),
+    "msv_set_op": (_I, [_P, _I, _S, _S, _I, _PP]),
+    "msv_segmentation_from_offsets": (_I, [_P, _S, _PP]),
+    "msv_segmentation_from_lengths": (_I, [_P, _S, _PP]),
+    "msv_segmentation_destroy": (_I, [_P]),
+    "msv_segmentation_offsets": (_I, [_P, _PP]),
+    "msv_segmentation_segment_count": (_U32, [_P]),
+    "msv_segmentation_value_count": (_U32, [_P]),
+    "msv_seg_inclusive_scan": (_I, [_P, _S, _P, _I, _PP]),
+    "msv_seg_exclusive_scan": (_I, [_P, _S, _P, _P, _I, _PP]),
+    "msv_seg_reduce": (_I, [_P, _S, _P, _P, _I, _PP]),
+    "msv_seg_sort": (_I, [_P, _S, _S, _P, _I, _PP, _PP]),
+    "msv_random_uniform_u32": (_I, [_P, _U32, _U32, _U64, _S]),
+    "msv_random_uniform_u64": (_I, [_P, _U64, _U64, _U64, _S]),
+    "msv_iota_u32": (_I, [_P, _U32, _S]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load(path: Path | str | None = None):
+    """Loads libmsv.so and binds every prototype; raises if it is missing."""
+    global _lib
+    with _lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = Path(path) if path else LIB_PATH
+        if not p.exists():
+            raise RuntimeError(
+                f"libmsv.so not found at {p}; build it with `python -m massively_amd.build` "
+                "(the HIP extension is required, there is no CPU fallback)"
+            )
+        lib = C.CDLL(str(p))
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(lib, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def lib():
+    return _lib if _lib is not None else load()
+
+
+def last_error() -> MsvError:
+    e = MsvError()
+    lib().msv_last_error(C.byref(e))
+    return e

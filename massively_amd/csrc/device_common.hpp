@@ -1,0 +1,420 @@
+// Device-side building blocks shared by every gfx950 kernel of the library:
+// element traits, the closed operator set, wave64 DPP scans, and the
+// decoupled-lookback tile-status protocol.
+//
+// Wave64 scans use DPP row shifts inside each 16-lane row followed by the two
+// row broadcasts (lane 15 -> rows 1/3, lane 31 -> rows 2/3). Lanes whose DPP
+// source is outside the row are not written, so they keep the operator's
+// identity passed as the `old` operand; this makes the same code correct for
+// add, min, max and xor.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace msv {
+
+constexpr int kWave = 64;
+
+// Closed operator set (mirrors the built-in ReductionOp implementations a
+// massively user writes: Add, Min, Max, Xor).
+enum OpKind : int { OP_ADD = 0, OP_MIN = 1, OP_MAX = 2, OP_XOR = 3 };
+
+template <typename T>
+struct Bits;
+template <>
+struct Bits<uint32_t> { using U = uint32_t; };
+template <>
+struct Bits<int32_t> { using U = uint32_t; };
+template <>
+struct Bits<float> { using U = uint32_t; };
+template <>
+struct Bits<uint64_t> { using U = uint64_t; };
+template <>
+struct Bits<int64_t> { using U = uint64_t; };
+template <>
+struct Bits<double> { using U = uint64_t; };
+
+template <typename To, typename From>
+__device__ __host__ inline To bit_cast(From v) {
+  static_assert(sizeof(To) == sizeof(From), "size");
+  To out;
+  __builtin_memcpy(&out, &v, sizeof(To));
+  return out;
+}
+
+template <typename T>
+struct IsFloat { static constexpr bool value = false; };
+template <>
+struct IsFloat<float> { static constexpr bool value = true; };
+template <>
+struct IsFloat<double> { static constexpr bool value = true; };
+
+template <typename T>
+struct Limits;
+template <>
+struct Limits<uint32_t> {
+  __host__ __device__ static uint32_t lowest() { return 0u; }
+  __host__ __device__ static uint32_t highest() { return 0xffffffffu; }
+};
+template <>
+struct Limits<int32_t> {
+  __host__ __device__ static int32_t lowest() { return INT32_MIN; }
+  __host__ __device__ static int32_t highest() { return INT32_MAX; }
+};
+template <>
+struct Limits<uint64_t> {
+  __host__ __device__ static uint64_t lowest() { return 0ull; }
+  __host__ __device__ static uint64_t highest() { return ~0ull; }
+};
+template <>
+struct Limits<int64_t> {
+  __host__ __device__ static int64_t lowest() { return INT64_MIN; }
+  __host__ __device__ static int64_t highest() { return INT64_MAX; }
+};
+template <>
+struct Limits<float> {
+  __host__ __device__ static float lowest() { return -__builtin_huge_valf(); }
+  __host__ __device__ static float highest() { return __builtin_huge_valf(); }
+};
+template <>
+struct Limits<double> {
+  __host__ __device__ static double lowest() { return -__builtin_huge_val(); }
+  __host__ __device__ static double highest() { return __builtin_huge_val(); }
+};
+
+// Operator application. `a` is the earlier (left) operand. Integer addition
+// wraps modulo 2^bits (signed types add through their unsigned twin). Float
+// addition's identity is -0.0 so that padding lanes never change the sign of
+// a zero result. Min/max keep the left operand on ties: min(a,b) = b<a ? b : a.
+template <typename T, int OP>
+struct Combine {
+  __host__ __device__ static inline T apply(T a, T b) {
+    if constexpr (OP == OP_ADD) {
+      if constexpr (IsFloat<T>::value) {
+        return a + b;
+      } else {
+        using U = typename Bits<T>::U;
+        return (T)((U)a + (U)b);
+      }
+    } else if constexpr (OP == OP_MIN) {
+      return (b < a) ? b : a;
+    } else if constexpr (OP == OP_MAX) {
+      return (a < b) ? b : a;
+    } else {
+      static_assert(!IsFloat<T>::value, "xor is integer-only");
+      return a ^ b;
+    }
+  }
+  __host__ __device__ static inline T identity() {
+    if constexpr (OP == OP_ADD) {
+      if constexpr (IsFloat<T>::value) {
+        return (T)-0.0;
+      } else {
+        return (T)0;
+      }
+    } else if constexpr (OP == OP_MIN) {
+      return Limits<T>::highest();
+    } else if constexpr (OP == OP_MAX) {
+      return Limits<T>::lowest();
+    } else {
+      return (T)0;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// DPP moves for 32- and 64-bit values.
+// ---------------------------------------------------------------------------
+enum DppCtrl : int {
+  DPP_ROW_SHR1 = 0x111,
+  DPP_ROW_SHR2 = 0x112,
+  DPP_ROW_SHR4 = 0x114,
+  DPP_ROW_SHR8 = 0x118,
+  DPP_ROW_BCAST15 = 0x142,
+  DPP_ROW_BCAST31 = 0x143,
+};
+
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ inline uint32_t dpp_u32(uint32_t old, uint32_t src) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, CTRL, ROW_MASK, BANK_MASK,
+                                               false);
+}
+
+template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf, typename T>
+__device__ inline T dpp_move(T old, T src) {
+  if constexpr (sizeof(T) == 4) {
+    return bit_cast<T>(dpp_u32<CTRL, ROW_MASK, BANK_MASK>(bit_cast<uint32_t>(old),
+                                                          bit_cast<uint32_t>(src)));
+  } else {
+    static_assert(sizeof(T) == 8, "dpp_move: 4 or 8 byte values");
+    uint64_t o = bit_cast<uint64_t>(old), s = bit_cast<uint64_t>(src);
+    uint32_t lo = dpp_u32<CTRL, ROW_MASK, BANK_MASK>((uint32_t)o, (uint32_t)s);
+    uint32_t hi = dpp_u32<CTRL, ROW_MASK, BANK_MASK>((uint32_t)(o >> 32), (uint32_t)(s >> 32));
+    return bit_cast<T>(((uint64_t)hi << 32) | lo);
+  }
+}
+
+template <typename T>
+__device__ inline T readlane(T v, int lane) {
+  if constexpr (sizeof(T) == 4) {
+    return bit_cast<T>((uint32_t)__builtin_amdgcn_readlane((int)bit_cast<uint32_t>(v), lane));
+  } else {
+    uint64_t u = bit_cast<uint64_t>(v);
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return bit_cast<T>(((uint64_t)hi << 32) | lo);
+  }
+}
+
+// Lane l receives the value of lane l-1; lane 0 receives `fill`.
+template <typename T>
+__device__ inline T shift_up1(T v, T fill) {
+  const int lane = (int)__lane_id();
+  const int src = (lane == 0 ? 0 : lane - 1) << 2;
+  T out;
+  if constexpr (sizeof(T) == 4) {
+    out = bit_cast<T>((uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)bit_cast<uint32_t>(v)));
+  } else {
+    uint64_t u = bit_cast<uint64_t>(v);
+    uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)u);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)(u >> 32));
+    out = bit_cast<T>(((uint64_t)hi << 32) | lo);
+  }
+  return lane == 0 ? fill : out;
+}
+
+// Inclusive wave64 scan (Hillis-Steele over DPP). Result lane l holds
+// x_0 (+) ... (+) x_l.
+template <typename T, int OP>
+__device__ inline T wave_inclusive_scan(T x) {
+  using C = Combine<T, OP>;
+  const T id = C::identity();
+  x = C::apply(dpp_move<DPP_ROW_SHR1>(id, x), x);
+  x = C::apply(dpp_move<DPP_ROW_SHR2>(id, x), x);
+  x = C::apply(dpp_move<DPP_ROW_SHR4>(id, x), x);
+  x = C::apply(dpp_move<DPP_ROW_SHR8>(id, x), x);
+  x = C::apply(dpp_move<DPP_ROW_BCAST15, 0xa>(id, x), x);
+  x = C::apply(dpp_move<DPP_ROW_BCAST31, 0xc>(id, x), x);
+  return x;
+}
+
+// Wave total, uniform across the wave.
+template <typename T, int OP>
+__device__ inline T wave_reduce(T x) {
+  return readlane(wave_inclusive_scan<T, OP>(x), kWave - 1);
+}
+
+// Segmented (head-flag) wave scan: pair (flag, value), combine
+// (f1,v1).(f2,v2) = (f1|f2, f2 ? v2 : v1(+)v2).
+template <typename T, int OP>
+__device__ inline void wave_inclusive_seg_scan(uint32_t& f, T& x) {
+  using C = Combine<T, OP>;
+  const T id = C::identity();
+#define MSV_SEG_STEP(CTRL, RM)                                  \
+  {                                                             \
+    T pv = dpp_move<CTRL, RM>(id, x);                           \
+    uint32_t pf = dpp_move<CTRL, RM>(0u, f);                    \
+    x = f ? x : C::apply(pv, x);                                \
+    f = f | pf;                                                 \
+  }
+  MSV_SEG_STEP(DPP_ROW_SHR1, 0xf)
+  MSV_SEG_STEP(DPP_ROW_SHR2, 0xf)
+  MSV_SEG_STEP(DPP_ROW_SHR4, 0xf)
+  MSV_SEG_STEP(DPP_ROW_SHR8, 0xf)
+  MSV_SEG_STEP(DPP_ROW_BCAST15, 0xa)
+  MSV_SEG_STEP(DPP_ROW_BCAST31, 0xc)
+#undef MSV_SEG_STEP
+}
+
+__device__ inline uint64_t lanemask_lt() {
+  const uint32_t lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+
+// ---------------------------------------------------------------------------
+// Scoped atomics. Agent scope makes the accesses coherent across the eight
+// XCDs (each with its own L2) of one GPU.
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t ld_agent(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_agent(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t add_agent(uint64_t* p, uint64_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Decoupled lookback.
+//
+// A launch takes tile ids from a 64-bit counter whose high half is the launch
+// generation. The block that draws the last id (ntiles-1) has seen every other
+// block draw first, so it rewinds the low half and bumps the generation with a
+// single atomic add; the counter is never cleared by the host and stays valid
+// under stream-ordered replay.
+//
+// Status word (64 bit): [63:35] generation (29 bits) | [34] head bit |
+// [33:32] flag (1 = aggregate, 2 = inclusive prefix) | [31:0] payload.
+// A word is valid for a launch iff its flag is nonzero and its generation
+// matches; stale words of earlier launches therefore never need clearing.
+// 64-bit payloads use two words per tile (low, high half) that each carry the
+// flag; a reader accepts them only when both are valid with equal flags.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kFlagAgg = 1;
+constexpr uint32_t kFlagInc = 2;
+constexpr uint32_t kGenMask = (1u << 29) - 1;
+
+struct TileTicket {
+  uint32_t tile;
+  uint32_t gen;
+};
+
+__device__ inline TileTicket draw_tile(uint64_t* counter, uint32_t ntiles) {
+  const uint64_t old = add_agent(counter, 1ull);
+  TileTicket t;
+  t.tile = (uint32_t)old;
+  t.gen = (uint32_t)(old >> 32) & kGenMask;
+  if (t.tile == ntiles - 1) add_agent(counter, (1ull << 32) - (uint64_t)ntiles);
+  return t;
+}
+
+__device__ inline uint64_t pack_status(uint32_t gen, uint32_t flag, uint32_t head,
+                                       uint32_t payload) {
+  return ((uint64_t)((gen << 3) | (head << 2) | flag) << 32) | payload;
+}
+__device__ inline uint32_t status_flag(uint64_t w, uint32_t gen) {
+  const uint32_t hi = (uint32_t)(w >> 32);
+  return ((hi >> 3) == gen) ? (hi & 3u) : 0u;
+}
+
+// Per-tile status for a payload of type T (4 or 8 bytes) plus an optional
+// head bit used by segmented scans.
+template <typename T>
+struct TileStatus {
+  static constexpr int kWords = sizeof(T) / 4;
+  uint64_t* words;
+  uint32_t gen;
+
+  __device__ inline void publish(uint32_t tile, uint32_t flag, uint32_t head, T v) const {
+    if constexpr (kWords == 1) {
+      st_agent(words + tile, pack_status(gen, flag, head, bit_cast<uint32_t>(v)));
+    } else {
+      const uint64_t u = bit_cast<uint64_t>(v);
+      st_agent(words + 2 * (size_t)tile, pack_status(gen, flag, head, (uint32_t)u));
+      st_agent(words + 2 * (size_t)tile + 1, pack_status(gen, flag, head, (uint32_t)(u >> 32)));
+    }
+  }
+  // Spins until the status of `tile` is valid; returns flag and payload.
+  __device__ inline uint32_t wait(uint32_t tile, uint32_t& head, T& v) const {
+    if constexpr (kWords == 1) {
+      uint64_t w;
+      uint32_t f;
+      while ((f = status_flag(w = ld_agent(words + tile), gen)) == 0) __builtin_amdgcn_s_sleep(1);
+      head = (uint32_t)(w >> 34) & 1u;
+      v = bit_cast<T>((uint32_t)w);
+      return f;
+    } else {
+      uint64_t lo, hi;
+      uint32_t fl, fh;
+      for (;;) {
+        lo = ld_agent(words + 2 * (size_t)tile);
+        hi = ld_agent(words + 2 * (size_t)tile + 1);
+        fl = status_flag(lo, gen);
+        fh = status_flag(hi, gen);
+        if (fl != 0 && fl == fh) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      head = (uint32_t)(lo >> 34) & 1u;
+      v = bit_cast<T>(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+      return fl;
+    }
+  }
+};
+
+// Wave-cooperative lookback over predecessors of `tile` (called by one whole
+// wave). Returns the exclusive prefix of the tile. Lane k inspects tile
+// (base - k); the nearest inclusive-prefix tile ends the walk.
+template <typename T, int OP>
+__device__ inline T wave_lookback(const TileStatus<T>& st, uint32_t tile) {
+  using C = Combine<T, OP>;
+  const uint32_t lane = __lane_id();
+  T prefix = C::identity();
+  int64_t base = (int64_t)tile - 1;
+  for (;;) {
+    const int64_t p = base - (int64_t)lane;
+    T v = C::identity();
+    uint32_t flag = kFlagInc, head = 0;
+    if (p >= 0) flag = st.wait((uint32_t)p, head, v);
+    const uint64_t inc_mask = ballot(flag == kFlagInc);
+    if (inc_mask) {
+      const uint32_t first = (uint32_t)__builtin_ctzll(inc_mask);
+      const T contrib = lane <= first ? v : C::identity();
+      // Lanes hold tiles in reverse order, but all supported operators are
+      // commutative, so the order of combination inside the window is free.
+      prefix = C::apply(wave_reduce<T, OP>(contrib), prefix);
+      return prefix;
+    }
+    prefix = C::apply(wave_reduce<T, OP>(v), prefix);
+    base -= kWave;
+  }
+}
+
+// Segmented variant: walk stops at the nearest tile that is inclusive or that
+// contains a segment head (its aggregate then already is the carry).
+template <typename T, int OP>
+__device__ inline void wave_lookback_seg(const TileStatus<T>& st, uint32_t tile, uint32_t& head_out,
+                                         T& carry) {
+  using C = Combine<T, OP>;
+  const uint32_t lane = __lane_id();
+  T prefix = C::identity();
+  uint32_t any_head = 0;
+  int64_t base = (int64_t)tile - 1;
+  for (;;) {
+    const int64_t p = base - (int64_t)lane;
+    T v = C::identity();
+    uint32_t flag = kFlagInc, head = 0;
+    if (p >= 0) flag = st.wait((uint32_t)p, head, v);
+    const uint64_t stop_mask = ballot(flag == kFlagInc || head != 0);
+    const uint64_t head_mask = ballot(head != 0);
+    if (stop_mask) {
+      const uint32_t first = (uint32_t)__builtin_ctzll(stop_mask);
+      // Tiles nearer than `first` have no head; combine them with the stop
+      // tile. Segmented combination is not commutative, so fold in order from
+      // the farthest (lane `first`) to the nearest (lane 0).
+      T acc = C::identity();
+      for (int k = (int)first; k >= 0; --k) acc = C::apply(acc, readlane(v, k));
+      any_head |= (head_mask & ((first == 63) ? ~0ull : ((2ull << first) - 1))) ? 1u : 0u;
+      prefix = C::apply(acc, prefix);
+      break;
+    }
+    T acc = C::identity();
+    for (int k = kWave - 1; k >= 0; --k) acc = C::apply(acc, readlane(v, k));
+    prefix = C::apply(acc, prefix);
+    base -= kWave;
+  }
+  head_out = any_head;
+  carry = prefix;
+}
+
+// Order-preserving unsigned encodings for radix ordering (integer order and
+// IEEE total order), see massively core/radix.rs:13-118.
+template <typename K>
+__device__ inline typename Bits<K>::U radix_encode(K k) {
+  using U = typename Bits<K>::U;
+  if constexpr (IsFloat<K>::value) {
+    const U raw = bit_cast<U>(k);
+    const U sign = (U)1 << (sizeof(U) * 8 - 1);
+    return (raw & sign) ? (U)~raw : (U)(raw ^ sign);
+  } else if constexpr ((K)-1 < (K)0) {
+    const U sign = (U)1 << (sizeof(U) * 8 - 1);
+    return (U)k ^ sign;
+  } else {
+    return (U)k;
+  }
+}
+
+}  // namespace msv

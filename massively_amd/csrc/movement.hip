@@ -1,0 +1,255 @@
+// vector::gather / scatter and seeded synthetic inputs for gfx950.
+//
+// gather: out[i] = values[idx[i]]   (massively/src/core/indexed.rs:16-196)
+// scatter: out[idx[i]] = values[i]  (massively/src/core/scatter.rs:8-21)
+// Each thread owns ROWS 16-byte index vectors laid out row-striped like the
+// scan tiles, so index loads and the sequential-side stores are full-wave
+// dwordx4 accesses and each thread keeps 16 independent random accesses in
+// flight. The reference leaves out-of-range indices undefined; here a gather
+// reads zero for them and a scatter drops them, so no index can fault.
+#include "device_common.hpp"
+#include "msv_internal.hpp"
+
+namespace msv {
+
+constexpr int kMoveBlock = 256;
+constexpr int kMoveRows = 4;  // 4 x uint4 of indices = 16 indices per thread
+constexpr uint32_t kMoveTile = kMoveBlock * kMoveRows * 4;
+
+template <typename T>
+__global__ __launch_bounds__(kMoveBlock) void gather_kernel(const T* __restrict__ values,
+                                                            uint32_t vlen,
+                                                            const uint32_t* __restrict__ idx,
+                                                            T* __restrict__ out, uint32_t n,
+                                                            bool aligned) {
+  const uint32_t tid = threadIdx.x;
+  const size_t base = (size_t)blockIdx.x * kMoveTile;
+  const bool full = aligned && base + kMoveTile <= n;
+  uint32_t ix[kMoveRows][4];
+  if (full) {
+#pragma unroll
+    for (int r = 0; r < kMoveRows; ++r) {
+      const uint4 q = *reinterpret_cast<const uint4*>(idx + base + ((size_t)r * kMoveBlock + tid) * 4);
+      ix[r][0] = q.x; ix[r][1] = q.y; ix[r][2] = q.z; ix[r][3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < kMoveRows; ++r)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const size_t i = base + ((size_t)r * kMoveBlock + tid) * 4 + k;
+        ix[r][k] = i < n ? idx[i] : 0xffffffffu;
+      }
+  }
+  T v[kMoveRows][4];
+#pragma unroll
+  for (int r = 0; r < kMoveRows; ++r)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[r][k] = ix[r][k] < vlen ? values[ix[r][k]] : (T)0;
+#pragma unroll
+  for (int r = 0; r < kMoveRows; ++r) {
+    const size_t o = base + ((size_t)r * kMoveBlock + tid) * 4;
+    if (full) {
+      if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<uint4*>(out + o) =
+            make_uint4(bit_cast<uint32_t>(v[r][0]), bit_cast<uint32_t>(v[r][1]),
+                       bit_cast<uint32_t>(v[r][2]), bit_cast<uint32_t>(v[r][3]));
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[o + k] = v[r][k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (o + k < n) out[o + k] = v[r][k];
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMoveBlock) void scatter_kernel(const T* __restrict__ values,
+                                                             const uint32_t* __restrict__ idx,
+                                                             uint32_t n, T* __restrict__ out,
+                                                             uint32_t olen) {
+  const uint32_t tid = threadIdx.x;
+  const size_t base = (size_t)blockIdx.x * kMoveTile;
+#pragma unroll
+  for (int r = 0; r < kMoveRows; ++r)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const size_t i = base + ((size_t)r * kMoveBlock + tid) * 4 + k;
+      if (i < n) {
+        const uint32_t d = idx[i];
+        if (d < olen) out[d] = values[i];
+      }
+    }
+}
+
+// ---- seeded streams (massively/src/api/util/random.rs:12-22,331-396) ------
+__host__ __device__ inline uint32_t pcg_hash32(uint32_t input) {
+  const uint32_t state = input * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+
+static uint32_t seed_key(uint64_t seed) {
+  return pcg_hash32((uint32_t)seed ^ pcg_hash32((uint32_t)(seed >> 32)));
+}
+
+__global__ void uniform_u32_kernel(uint32_t* __restrict__ out, uint32_t n, uint32_t start,
+                                   uint32_t key, uint32_t lo, uint32_t span) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t v = pcg_hash32((start + i) ^ key);
+  out[i] = (span == 0xffffffffu) ? v : lo + v % (span + 1u);
+}
+
+__global__ void uniform_u64_kernel(uint64_t* __restrict__ out, uint32_t n, uint32_t start,
+                                   uint32_t key, uint64_t lo, uint64_t span) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = start + i;
+  const uint64_t v = (uint64_t)pcg_hash32(p ^ key) | ((uint64_t)pcg_hash32(p ^ key ^ 0x9e3779b9u) << 32);
+  out[i] = (span == ~0ull) ? v : lo + v % (span + 1ull);
+}
+
+__global__ void iota_u32_kernel(uint32_t* __restrict__ out, uint32_t n, uint32_t start) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = start + i;
+}
+
+// ---- host ------------------------------------------------------------------
+msv_status gather_run(Ctx& c, msv_dtype dt, const void* values, uint32_t vlen,
+                      const uint32_t* idx, void* out, uint32_t n) {
+  if (n == 0) return MSV_OK;
+  const uint32_t blocks = (n + kMoveTile - 1) / kMoveTile;
+  const bool aligned = (((uintptr_t)idx | (uintptr_t)out) & 15u) == 0;
+  if (dtype_size(dt) == 4)
+    hipLaunchKernelGGL(gather_kernel<uint32_t>, dim3(blocks), dim3(kMoveBlock), 0, c.stream(),
+                       (const uint32_t*)values, vlen, idx, (uint32_t*)out, n, aligned);
+  else
+    hipLaunchKernelGGL(gather_kernel<uint64_t>, dim3(blocks), dim3(kMoveBlock), 0, c.stream(),
+                       (const uint64_t*)values, vlen, idx, (uint64_t*)out, n, aligned);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+msv_status scatter_run(Ctx& c, msv_dtype dt, const void* values, const uint32_t* idx,
+                       uint32_t n, void* out, uint32_t olen) {
+  if (n == 0) return MSV_OK;
+  const uint32_t blocks = (n + kMoveTile - 1) / kMoveTile;
+  if (dtype_size(dt) == 4)
+    hipLaunchKernelGGL(scatter_kernel<uint32_t>, dim3(blocks), dim3(kMoveBlock), 0, c.stream(),
+                       (const uint32_t*)values, idx, n, (uint32_t*)out, olen);
+  else
+    hipLaunchKernelGGL(scatter_kernel<uint64_t>, dim3(blocks), dim3(kMoveBlock), 0, c.stream(),
+                       (const uint64_t*)values, idx, n, (uint64_t*)out, olen);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+msv_status iota_run(Ctx& c, uint32_t* out, uint32_t n, uint32_t start) {
+  if (n == 0) return MSV_OK;
+  hipLaunchKernelGGL(iota_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream(), out, n,
+                     start);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+}  // namespace msv
+
+using namespace msv;
+
+static msv_status check_index_slice(const View& v) {
+  if (v.dtype != MSV_U32) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "indices must be u32 (MIndex)");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  return MSV_OK;
+}
+
+extern "C" {
+
+msv_status msv_gather(msv_executor* ex, msv_slice values, msv_slice indices, msv_vec** out) {
+  clear_error();
+  View v, ix;
+  MSV_TRY(resolve(ex, values, &v));
+  MSV_TRY(resolve(ex, indices, &ix));
+  MSV_TRY(check_index_slice(ix));
+  MSV_TRY(new_vec(ex, v.dtype, ix.len, false, out));
+  Ctx c(ex);
+  msv_status s = gather_run(c, v.dtype, v.ptr, v.len, (const uint32_t*)ix.ptr, (*out)->ptr, ix.len);
+  if (s != MSV_OK) {
+    msv_vec_release(*out);
+    *out = nullptr;
+  }
+  return s;
+}
+
+msv_status msv_scatter(msv_executor* ex, msv_slice values, msv_slice indices, msv_slice output) {
+  clear_error();
+  View v, ix, o;
+  MSV_TRY(resolve(ex, values, &v));
+  MSV_TRY(resolve(ex, indices, &ix));
+  MSV_TRY(resolve(ex, output, &o));
+  MSV_TRY(check_index_slice(ix));
+  if (v.len != ix.len) {
+    set_error(MSV_ERR_LENGTH_MISMATCH, v.len, ix.len, "input lengths differ: %u != %u", v.len,
+              ix.len);
+    return MSV_ERR_LENGTH_MISMATCH;
+  }
+  if (o.dtype != v.dtype) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "output dtype differs from values");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  Ctx c(ex);
+  return scatter_run(c, v.dtype, v.ptr, (const uint32_t*)ix.ptr, v.len, o.ptr, o.len);
+}
+
+msv_status msv_random_uniform_u32(msv_executor* ex, uint32_t min, uint32_t max, uint64_t seed,
+                                  msv_slice out) {
+  clear_error();
+  View o;
+  MSV_TRY(resolve(ex, out, &o));
+  if (o.dtype != MSV_U32 || min > max) {
+    set_error(MSV_ERR_LAUNCH, 0, 0, "random distribution range must satisfy min <= max (u32 out)");
+    return MSV_ERR_LAUNCH;
+  }
+  if (o.len == 0) return MSV_OK;
+  Ctx c(ex);
+  hipLaunchKernelGGL(uniform_u32_kernel, dim3((o.len + 255) / 256), dim3(256), 0, c.stream(),
+                     (uint32_t*)o.ptr, o.len, 0u, seed_key(seed), min, max - min);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+msv_status msv_random_uniform_u64(msv_executor* ex, uint64_t min, uint64_t max, uint64_t seed,
+                                  msv_slice out) {
+  clear_error();
+  View o;
+  MSV_TRY(resolve(ex, out, &o));
+  if (o.dtype != MSV_U64 || min > max) {
+    set_error(MSV_ERR_LAUNCH, 0, 0, "random distribution range must satisfy min <= max (u64 out)");
+    return MSV_ERR_LAUNCH;
+  }
+  if (o.len == 0) return MSV_OK;
+  Ctx c(ex);
+  hipLaunchKernelGGL(uniform_u64_kernel, dim3((o.len + 255) / 256), dim3(256), 0, c.stream(),
+                     (uint64_t*)o.ptr, o.len, 0u, seed_key(seed), min, max - min);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+msv_status msv_iota_u32(msv_executor* ex, uint32_t start, msv_slice out) {
+  clear_error();
+  View o;
+  MSV_TRY(resolve(ex, out, &o));
+  if (o.dtype != MSV_U32) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "iota output must be u32");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  Ctx c(ex);
+  return iota_run(c, (uint32_t*)o.ptr, o.len, start);
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+// Host-side internals of libmsv: executor, vectors, caching allocator, error
+// reporting, and the per-call launch context handed to kernel modules.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/msv.h"
+
+namespace msv {
+
+// ---- errors ---------------------------------------------------------------
+void set_error(msv_status code, uint64_t left, uint64_t right, const char* fmt, ...);
+void clear_error();
+
+#define MSV_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      ::msv::set_error(MSV_ERR_LAUNCH, 0, 0, "%s: %s (%s:%d)", #expr,                   \
+                       hipGetErrorString(_e), __FILE__, __LINE__);                     \
+      return MSV_ERR_LAUNCH;                                                           \
+    }                                                                                  \
+  } while (0)
+
+#define MSV_TRY(expr)                     \
+  do {                                    \
+    msv_status _s = (expr);               \
+    if (_s != MSV_OK) return _s;          \
+  } while (0)
+
+inline size_t dtype_size(msv_dtype d) {
+  switch (d) {
+    case MSV_U32:
+    case MSV_I32:
+    case MSV_F32:
+      return 4;
+    case MSV_U64:
+    case MSV_I64:
+    case MSV_F64:
+      return 8;
+  }
+  return 0;
+}
+inline bool dtype_valid(int d) { return d >= MSV_U32 && d <= MSV_F64; }
+inline bool dtype_is_float(msv_dtype d) { return d == MSV_F32 || d == MSV_F64; }
+
+// ---- caching allocator (one per executor) ---------------------------------
+class Allocator {
+ public:
+  explicit Allocator(int device) : device_(device) {}
+  // Returns nullptr and sets the error on failure.
+  void* alloc(size_t bytes);
+  void free(void* p);
+  void trim();  // caller synchronises first
+  size_t cached_bytes() const { return cached_; }
+
+ private:
+  static size_t round(size_t bytes);
+  int device_;
+  std::mutex mu_;
+  std::multimap<size_t, void*> free_;
+  std::unordered_map<void*, size_t> sizes_;
+  size_t cached_ = 0;
+};
+
+}  // namespace msv
+
+struct msv_executor {
+  uint64_t id;
+  int device;
+  int num_cus;
+  hipStream_t stream;
+  std::atomic<int> refs;
+  bool destroyed;
+  msv::Allocator allocator;
+  // Decoupled-lookback state: one tile counter (tile id | generation << 32)
+  // and a growable status-word array, shared by every lookback kernel on this
+  // executor's stream.
+  uint64_t* d_counter;
+  uint64_t* d_status;
+  size_t status_words;
+  uint64_t lookback_launches;
+  explicit msv_executor(int dev) : allocator(dev) {}
+};
+
+struct msv_vec {
+  std::atomic<int> refs;
+  msv_executor* ex;
+  uint64_t owner_id;
+  msv_dtype dtype;
+  uint32_t capacity;
+  void* ptr;
+  bool owned;
+  uint32_t host_len;   // logical length when d_len == nullptr
+  uint32_t* d_len;     // device-resident logical length (owned), or nullptr
+};
+
+struct msv_segmentation {
+  msv_vec* offsets;     // S+1 offsets, retained
+  uint32_t segment_count;
+  uint32_t value_count;
+};
+
+namespace msv {
+
+void executor_retain(msv_executor* ex);
+void executor_release(msv_executor* ex);
+
+// Resolved view of a msv_slice.
+struct View {
+  msv_vec* vec = nullptr;
+  void* ptr = nullptr;  // element pointer at slice offset
+  uint32_t len = 0;
+  msv_dtype dtype = MSV_U32;
+};
+
+// Validates ownership/range and resolves the length (synchronising only for a
+// MSV_WHOLE slice of a vector whose length is device-resident).
+msv_status resolve(msv_executor* ex, const msv_slice& s, View* out);
+
+// Creates a new owned output vector.
+msv_status new_vec(msv_executor* ex, msv_dtype dtype, uint32_t capacity, bool device_len,
+                   msv_vec** out);
+
+// Per-call launch context: scratch memory returned to the executor's cache at
+// the end of the call (stream order makes reuse by later work safe).
+class Ctx {
+ public:
+  explicit Ctx(msv_executor* ex) : ex_(ex) {}
+  ~Ctx();
+  hipStream_t stream() const { return ex_->stream; }
+  int num_cus() const { return ex_->num_cus; }
+  msv_executor* ex() const { return ex_; }
+  // nullptr + error on failure.
+  void* scratch(size_t bytes);
+  // Lookback state sized for `words` status words; counts one launch.
+  msv_status lookback(size_t words, uint64_t** counter, uint64_t** status);
+
+ private:
+  msv_executor* ex_;
+  std::vector<void*> scratch_;
+};
+
+// ---- kernel module entry points (implemented in the .hip modules) ----------
+// All take device pointers and lengths already validated by the API layer.
+msv_status scan_run(Ctx& c, msv_dtype dt, msv_op op, bool exclusive, const void* in, void* out,
+                    uint32_t n, const void* init_host);
+msv_status reduce_run(Ctx& c, msv_dtype dt, msv_op op, const void* in, uint32_t n,
+                      const void* init_host, void* out_host);
+
+}  // namespace msv

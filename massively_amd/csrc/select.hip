@@ -1,0 +1,281 @@
+// vector::copy_where / remove_where (stencil and built-in predicate forms) for
+// gfx950, in one pass.
+//
+// Reference: massively/src/api/algorithm/selection.rs:131-260 and
+// core/selection.rs:151-411 (scan of flags, count, selected-index scatter,
+// gather: four launches and ~30 B/elem). Here one kernel reads each value and
+// its flag once: a 256-thread block owns 4096 elements laid out in 4-element
+// groups, row-striped (group (r, t) starts at (r*256 + t)*4), counts the
+// selected elements per (row, thread), turns the counts into in-tile ranks
+// with DPP wave scans, resolves the tile's output offset by decoupled
+// lookback on counts, stages the selected elements in LDS in input order and
+// writes them out contiguously. The block that owns the last tile writes the
+// total count into the output's device-resident length (no host sync).
+#include "device_common.hpp"
+#include "msv_internal.hpp"
+
+namespace msv {
+
+constexpr int kSelBlock = 256;
+constexpr int kSelRows = 4;
+constexpr uint32_t kSelTile = kSelBlock * kSelRows * 4;
+
+// Built-in predicates (massively/src/core/predicate.rs:40-52 built-ins).
+enum PredKind : uint32_t {
+  PRED_NONZERO = 0, PRED_ZERO, PRED_LT, PRED_LE, PRED_GT, PRED_GE, PRED_BITS_ANY, PRED_BITS_EQ
+};
+
+template <typename S>
+struct Pred {
+  uint32_t kind;
+  S a;
+  typename Bits<S>::U ua, ub;
+  __device__ inline bool operator()(S x) const {
+    using U = typename Bits<S>::U;
+    switch (kind) {
+      case PRED_NONZERO: return x != (S)0;
+      case PRED_ZERO: return x == (S)0;
+      case PRED_LT: return x < a;
+      case PRED_LE: return x <= a;
+      case PRED_GT: return x > a;
+      case PRED_GE: return x >= a;
+      case PRED_BITS_ANY: return (bit_cast<U>(x) & ua) != 0;
+      default: return (bit_cast<U>(x) & ua) == ub;
+    }
+  }
+};
+
+template <typename T>
+__device__ inline void load4(const T* __restrict__ p, size_t i, uint32_t n, bool full, T (&v)[4]) {
+  if (full) {
+    if constexpr (sizeof(T) == 4) {
+      const uint4 q = *reinterpret_cast<const uint4*>(p + i);
+      v[0] = bit_cast<T>(q.x); v[1] = bit_cast<T>(q.y); v[2] = bit_cast<T>(q.z); v[3] = bit_cast<T>(q.w);
+    } else {
+      const uint4 q0 = *reinterpret_cast<const uint4*>(p + i);
+      const uint4 q1 = *reinterpret_cast<const uint4*>(p + i + 2);
+      v[0] = bit_cast<T>(((uint64_t)q0.y << 32) | q0.x);
+      v[1] = bit_cast<T>(((uint64_t)q0.w << 32) | q0.z);
+      v[2] = bit_cast<T>(((uint64_t)q1.y << 32) | q1.x);
+      v[3] = bit_cast<T>(((uint64_t)q1.w << 32) | q1.z);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = (i + k < n) ? p[i + k] : (T)0;
+  }
+}
+
+// FORM 0: u32 stencil array (flag = stencil != 0).
+// FORM 1: predicate on a separate stencil column of type S.
+// FORM 2: predicate on the input itself (S == T, one read).
+template <typename T, typename S, int FORM>
+__global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__ in,
+                                                           const S* __restrict__ stencil,
+                                                           Pred<S> pred, bool invert,
+                                                           uint32_t n, bool aligned,
+                                                           T* __restrict__ out,
+                                                           uint32_t* __restrict__ out_len,
+                                                           uint64_t* counter, uint64_t* status,
+                                                           uint32_t ntiles) {
+  constexpr int NW = kSelBlock / kWave;
+  __shared__ T s_stage[kSelTile];
+  __shared__ uint32_t s_wave[kSelRows][NW];
+  __shared__ uint32_t s_prefix;
+  __shared__ TileTicket s_ticket;
+
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
+  if (tid == 0) s_ticket = draw_tile(counter, ntiles);
+  __syncthreads();
+  const uint32_t tile = s_ticket.tile, gen = s_ticket.gen;
+  const size_t base = (size_t)tile * kSelTile;
+  const bool full = aligned && base + kSelTile <= n;
+
+  T x[kSelRows][4];
+  uint32_t f[kSelRows][4];
+#pragma unroll
+  for (int r = 0; r < kSelRows; ++r) {
+    const size_t i = base + ((size_t)r * kSelBlock + tid) * 4;
+    load4<T>(in, i, n, full, x[r]);
+    if constexpr (FORM == 0) {
+      S s[4];
+      load4<S>(stencil, i, n, full, s);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) f[r][k] = ((s[k] != 0) != invert) && (i + k < n);
+    } else if constexpr (FORM == 1) {
+      S s[4];
+      load4<S>(stencil, i, n, full, s);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) f[r][k] = (pred(s[k]) != invert) && (i + k < n);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) f[r][k] = (pred(bit_cast<S>(x[r][k])) != invert) && (i + k < n);
+    }
+  }
+
+  uint32_t ws[kSelRows];
+#pragma unroll
+  for (int r = 0; r < kSelRows; ++r) {
+    const uint32_t c = f[r][0] + f[r][1] + f[r][2] + f[r][3];
+    ws[r] = wave_inclusive_scan<uint32_t, OP_ADD>(c);
+    if (lane == 63) s_wave[r][wid] = ws[r];
+  }
+  __syncthreads();
+  uint32_t pre[kSelRows];
+  uint32_t agg = 0;
+#pragma unroll
+  for (int r = 0; r < kSelRows; ++r)
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      if (w == (int)wid) pre[r] = agg;
+      agg += s_wave[r][w];
+    }
+
+  if (wid == 0) {
+    TileStatus<uint32_t> st{status, gen};
+    uint32_t excl = 0;
+    if (tile == 0) {
+      if (lane == 0) st.publish(0, kFlagInc, 0, agg);
+    } else {
+      if (lane == 0) st.publish(tile, kFlagAgg, 0, agg);
+      excl = wave_lookback<uint32_t, OP_ADD>(st, tile);
+      if (lane == 0) st.publish(tile, kFlagInc, 0, excl + agg);
+    }
+    if (lane == 0) {
+      s_prefix = excl;
+      if (tile == ntiles - 1) *out_len = excl + agg;
+    }
+  }
+
+  // Stage selected elements at their in-tile rank (input order).
+#pragma unroll
+  for (int r = 0; r < kSelRows; ++r) {
+    uint32_t pos = pre[r] + shift_up1<uint32_t>(ws[r], 0u);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (f[r][k]) s_stage[pos] = x[r][k];
+      pos += f[r][k];
+    }
+  }
+  __syncthreads();
+  const size_t o = s_prefix;
+  for (uint32_t i = tid; i < agg; i += kSelBlock) out[o + i] = s_stage[i];
+}
+
+__global__ void store_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
+
+template <typename T, typename S, int FORM>
+static msv_status select_launch(Ctx& c, const void* in, const void* st, const msv_predicate* p,
+                                bool invert, uint32_t n, void* out, uint32_t* out_len) {
+  if (n == 0) {
+    hipLaunchKernelGGL(store_u32_kernel, dim3(1), dim3(1), 0, c.stream(), out_len, 0u);
+    MSV_HIP(hipGetLastError());
+    return MSV_OK;
+  }
+  Pred<S> pred{};
+  if (p) {
+    using U = typename Bits<S>::U;
+    pred.kind = p->kind;
+    pred.ua = (U)p->a;
+    pred.ub = (U)p->b;
+    pred.a = bit_cast<S>((U)p->a);
+  } else {
+    pred.kind = PRED_NONZERO;
+  }
+  const uint32_t ntiles = (n + kSelTile - 1) / kSelTile;
+  uint64_t *counter, *status;
+  MSV_TRY(c.lookback(ntiles, &counter, &status));
+  const bool aligned = (((uintptr_t)in | (uintptr_t)st) & 15u) == 0;
+  hipLaunchKernelGGL((select_kernel<T, S, FORM>), dim3(ntiles), dim3(kSelBlock), 0, c.stream(),
+                     (const T*)in, (const S*)st, pred, invert, n, aligned, (T*)out, out_len,
+                     counter, status, ntiles);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+template <typename S>
+static msv_status select_with_stencil_type(Ctx& c, size_t vsize, bool same, const void* in,
+                                           const void* st, const msv_predicate* p, bool invert,
+                                           uint32_t n, void* out, uint32_t* out_len) {
+  if (same) {
+    if constexpr (sizeof(S) == 4)
+      return select_launch<uint32_t, S, 2>(c, in, st, p, invert, n, out, out_len);
+    else
+      return select_launch<uint64_t, S, 2>(c, in, st, p, invert, n, out, out_len);
+  }
+  if (vsize == 4) return select_launch<uint32_t, S, 1>(c, in, st, p, invert, n, out, out_len);
+  return select_launch<uint64_t, S, 1>(c, in, st, p, invert, n, out, out_len);
+}
+
+}  // namespace msv
+
+using namespace msv;
+
+static msv_status select_api(msv_executor* ex, msv_slice input, msv_slice stencil,
+                             const msv_predicate* pred, bool invert, msv_vec** out) {
+  clear_error();
+  View in, st;
+  MSV_TRY(resolve(ex, input, &in));
+  MSV_TRY(resolve(ex, stencil, &st));
+  if (in.len != st.len) {
+    set_error(MSV_ERR_LENGTH_MISMATCH, in.len, st.len, "input lengths differ: %u != %u", in.len,
+              st.len);
+    return MSV_ERR_LENGTH_MISMATCH;
+  }
+  if (!pred && st.dtype != MSV_U32) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "stencil must be u32 flags (MFlag)");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  if (pred && pred->kind > PRED_BITS_EQ) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, pred->kind, 0, "unknown predicate kind %u", pred->kind);
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  MSV_TRY(new_vec(ex, in.dtype, in.len, true, out));
+  Ctx c(ex);
+  const size_t vs = dtype_size(in.dtype);
+  msv_status s;
+  if (!pred) {
+    if (vs == 4)
+      s = select_launch<uint32_t, uint32_t, 0>(c, in.ptr, st.ptr, nullptr, invert, in.len,
+                                              (*out)->ptr, (*out)->d_len);
+    else
+      s = select_launch<uint64_t, uint32_t, 0>(c, in.ptr, st.ptr, nullptr, invert, in.len,
+                                              (*out)->ptr, (*out)->d_len);
+  } else {
+    const bool same = in.ptr == st.ptr && in.dtype == st.dtype;
+    switch (st.dtype) {
+      case MSV_U32: s = select_with_stencil_type<uint32_t>(c, vs, same, in.ptr, st.ptr, pred, invert, in.len, (*out)->ptr, (*out)->d_len); break;
+      case MSV_I32: s = select_with_stencil_type<int32_t>(c, vs, same, in.ptr, st.ptr, pred, invert, in.len, (*out)->ptr, (*out)->d_len); break;
+      case MSV_F32: s = select_with_stencil_type<float>(c, vs, same, in.ptr, st.ptr, pred, invert, in.len, (*out)->ptr, (*out)->d_len); break;
+      case MSV_U64: s = select_with_stencil_type<uint64_t>(c, vs, same, in.ptr, st.ptr, pred, invert, in.len, (*out)->ptr, (*out)->d_len); break;
+      case MSV_I64: s = select_with_stencil_type<int64_t>(c, vs, same, in.ptr, st.ptr, pred, invert, in.len, (*out)->ptr, (*out)->d_len); break;
+      default: s = select_with_stencil_type<double>(c, vs, same, in.ptr, st.ptr, pred, invert, in.len, (*out)->ptr, (*out)->d_len); break;
+    }
+  }
+  if (s != MSV_OK) {
+    msv_vec_release(*out);
+    *out = nullptr;
+  }
+  return s;
+}
+
+extern "C" {
+
+msv_status msv_copy_where(msv_executor* ex, msv_slice input, msv_slice stencil, msv_vec** out) {
+  return select_api(ex, input, stencil, nullptr, false, out);
+}
+
+msv_status msv_remove_where(msv_executor* ex, msv_slice input, msv_slice stencil, msv_vec** out) {
+  return select_api(ex, input, stencil, nullptr, true, out);
+}
+
+msv_status msv_copy_where_pred(msv_executor* ex, msv_slice input, msv_slice stencil,
+                               const msv_predicate* pred, int32_t invert, msv_vec** out) {
+  if (!pred) {
+    clear_error();
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null predicate");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  return select_api(ex, input, stencil, pred, invert != 0, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+"""The closed set of built-in operators.
+
+The reference compiles user-written CubeCL traits into each kernel:
+``ReductionOp`` (massively/src/core/reduce.rs:35-66), ``BinaryPredicateOp``
+(core/ordering/mod.rs:29-65) and ``PredicateOp`` (core/predicate.rs:40-52).
+This engine serves the built-in members of those families:
+
+* reductions ``Add`` (integer add wraps modulo 2^bits), ``Min``, ``Max``,
+  ``Xor`` (integers only); ``Min(a, b) = b if b < a else a``;
+* key comparisons ``Less`` and ``Greater``;
+* predicates ``NonZero``, ``Zero``, thresholds ``Lt/Le/Gt/Ge(t)`` and bitmask
+  tests ``BitsAny(mask)`` / ``BitsEq(mask, value)`` on the raw element bits.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+class ReductionOp:
+    code: int = -1
+    name: str = ""
+
+    def __repr__(self):
+        return self.name
+
+
+class _Add(ReductionOp):
+    code, name = 0, "Add"
+
+
+class _Min(ReductionOp):
+    code, name = 1, "Min"
+
+
+class _Max(ReductionOp):
+    code, name = 2, "Max"
+
+
+class _Xor(ReductionOp):
+    code, name = 3, "Xor"
+
+
+Add, Min, Max, Xor = _Add(), _Min(), _Max(), _Xor()
+
+
+class BinaryPredicateOp:
+    code: int = -1
+    name: str = ""
+
+    def __repr__(self):
+        return self.name
+
+
+class _Less(BinaryPredicateOp):
+    code, name = 0, "Less"
+
+
+class _Greater(BinaryPredicateOp):
+    code, name = 1, "Greater"
+
+
+Less, Greater = _Less(), _Greater()
+
+
+@dataclass(frozen=True)
+class PredicateOp:
+    kind: int
+    a: object = 0
+    b: object = 0
+
+    def c_struct(self, dtype: np.dtype) -> L.MsvPredicate:
+        return L.MsvPredicate(self.kind, 0, _bits(self.a, dtype), _bits(self.b, dtype))
+
+
+def _bits(v, dtype: np.dtype) -> int:
+    """Bit pattern of `v` as an element of `dtype`, zero-extended to 64 bits."""
+    dtype = np.dtype(dtype)
+    arr = np.array([v], dtype=dtype)
+    return int(arr.view(np.uint32 if dtype.itemsize == 4 else np.uint64)[0])
+
+
+NonZero = PredicateOp(0)
+Zero = PredicateOp(1)
+
+
+def Lt(t):
+    return PredicateOp(2, t)
+
+
+def Le(t):
+    return PredicateOp(3, t)
+
+
+def Gt(t):
+    return PredicateOp(4, t)
+
+
+def Ge(t):
+    return PredicateOp(5, t)
+
+
+def BitsAny(mask):
+    return PredicateOp(6, mask)
+
+
+def BitsEq(mask, value):
+    return PredicateOp(7, mask, value)
+
+
+def host_scalar(value, dtype) -> C.Array:
+    """A one-element host buffer holding `value` as `dtype` (init arguments)."""
+    arr = np.array([value], dtype=np.dtype(dtype))
+    return arr

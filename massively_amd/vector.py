@@ -1,0 +1,174 @@
+"""``massively::vector`` algorithms on the gfx950 kernels of libmsv.
+
+Signatures follow the reference's free functions: the executor first, then
+device slices (or vectors), then the operator. Owned results are returned as
+new :class:`DeviceVec` objects; ``reduce`` returns a host scalar.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .error import LengthMismatch, check
+from .op import BinaryPredicateOp, PredicateOp, ReductionOp
+from .runtime import DeviceSlice, DeviceVec, Executor, as_slice
+
+_lib = L.lib
+
+
+def _out(exec: Executor, h: C.c_void_p) -> DeviceVec:
+    return DeviceVec(exec, h)
+
+
+def _scalar(value, dtype) -> np.ndarray:
+    return np.array([value], dtype=np.dtype(dtype))
+
+
+def inclusive_scan(exec: Executor, input, op: ReductionOp) -> DeviceVec:
+    """massively/src/api/algorithm/scan.rs:96-113."""
+    s = as_slice(input)
+    h = C.c_void_p()
+    check(_lib().msv_inclusive_scan(exec.handle, s._c(), op.code, C.byref(h)))
+    return _out(exec, h)
+
+
+def exclusive_scan(exec: Executor, input, init, op: ReductionOp) -> DeviceVec:
+    """massively/src/api/algorithm/scan.rs:216-257: out[0] = init."""
+    s = as_slice(input)
+    init_buf = _scalar(init, s.dtype)
+    h = C.c_void_p()
+    check(
+        _lib().msv_exclusive_scan(
+            exec.handle, s._c(), init_buf.ctypes.data_as(C.c_void_p), op.code, C.byref(h)
+        )
+    )
+    return _out(exec, h)
+
+
+def reduce(exec: Executor, input, init, op: ReductionOp):
+    """massively/src/api/algorithm/reduce.rs:32-50: op(init, fold(input))."""
+    s = as_slice(input)
+    init_buf = _scalar(init, s.dtype)
+    out = np.zeros(1, dtype=s.dtype)
+    check(
+        _lib().msv_reduce(
+            exec.handle, s._c(), init_buf.ctypes.data_as(C.c_void_p), op.code,
+            out.ctypes.data_as(C.c_void_p),
+        )
+    )
+    return out[0]
+
+
+def copy_where(exec: Executor, input, stencil) -> DeviceVec:
+    """massively/src/api/algorithm/selection.rs:146-165 (device-resident length)."""
+    h = C.c_void_p()
+    check(_lib().msv_copy_where(exec.handle, as_slice(input)._c(), as_slice(stencil)._c(),
+                                C.byref(h)))
+    return _out(exec, h)
+
+
+def remove_where(exec: Executor, input, stencil) -> DeviceVec:
+    """massively/src/api/algorithm/selection.rs:203-222."""
+    h = C.c_void_p()
+    check(_lib().msv_remove_where(exec.handle, as_slice(input)._c(), as_slice(stencil)._c(),
+                                  C.byref(h)))
+    return _out(exec, h)
+
+
+def copy_where_if(exec: Executor, input, stencil, pred: PredicateOp, invert: bool = False):
+    """copy_where with ``lazy::map(stencil, pred)`` as the stencil, fused."""
+    st = as_slice(stencil)
+    p = pred.c_struct(st.dtype)
+    h = C.c_void_p()
+    check(_lib().msv_copy_where_pred(exec.handle, as_slice(input)._c(), st._c(), C.byref(p),
+                                     1 if invert else 0, C.byref(h)))
+    return _out(exec, h)
+
+
+def remove_where_if(exec: Executor, input, stencil, pred: PredicateOp):
+    return copy_where_if(exec, input, stencil, pred, invert=True)
+
+
+def gather(exec: Executor, values, indices) -> DeviceVec:
+    """massively/src/api/algorithm/indexed.rs:157-208: out[i] = values[indices[i]]."""
+    h = C.c_void_p()
+    check(_lib().msv_gather(exec.handle, as_slice(values)._c(), as_slice(indices)._c(),
+                            C.byref(h)))
+    return _out(exec, h)
+
+
+def scatter(exec: Executor, values, indices, output) -> None:
+    """massively/src/api/algorithm/scatter.rs:248-265: output[indices[i]] = values[i]."""
+    check(_lib().msv_scatter(exec.handle, as_slice(values)._c(), as_slice(indices)._c(),
+                             as_slice(output)._c()))
+
+
+def radix_sort_by_key(exec: Executor, keys, values, return_keys: bool = False):
+    """massively/src/api/algorithm/by_key.rs:252-297: stable, integer/IEEE total order.
+
+    Returns the permuted values (and the sorted keys when ``return_keys``).
+    """
+    hv, hk = C.c_void_p(), C.c_void_p()
+    check(_lib().msv_radix_sort_by_key(exec.handle, as_slice(keys)._c(), as_slice(values)._c(),
+                                       C.byref(hv), C.byref(hk) if return_keys else None))
+    v = _out(exec, hv)
+    return (v, _out(exec, hk)) if return_keys else v
+
+
+def sort_by_key(exec: Executor, keys, values, less: BinaryPredicateOp, return_keys=False):
+    """massively/src/api/algorithm/by_key.rs:193-250: stable comparison sort."""
+    hv, hk = C.c_void_p(), C.c_void_p()
+    check(_lib().msv_sort_by_key(exec.handle, as_slice(keys)._c(), as_slice(values)._c(),
+                                 less.code, C.byref(hv), C.byref(hk) if return_keys else None))
+    v = _out(exec, hv)
+    return (v, _out(exec, hk)) if return_keys else v
+
+
+def sort(exec: Executor, input, less: BinaryPredicateOp) -> DeviceVec:
+    """massively/src/api/algorithm/ordering.rs:64-76."""
+    h = C.c_void_p()
+    check(_lib().msv_sort(exec.handle, as_slice(input)._c(), less.code, C.byref(h)))
+    return _out(exec, h)
+
+
+def merge(exec: Executor, left, right, less: BinaryPredicateOp) -> DeviceVec:
+    """massively/src/api/algorithm/merge.rs:109-190: stable, ties take left."""
+    h = C.c_void_p()
+    check(_lib().msv_merge(exec.handle, as_slice(left)._c(), as_slice(right)._c(), less.code,
+                           C.byref(h)))
+    return _out(exec, h)
+
+
+def merge_by_key(exec: Executor, left_keys, left_values, right_keys, right_values,
+                 less: BinaryPredicateOp):
+    """massively/src/api/algorithm/merge.rs:195-282. Returns (keys, values)."""
+    hk, hv = C.c_void_p(), C.c_void_p()
+    check(_lib().msv_merge_by_key(
+        exec.handle, as_slice(left_keys)._c(), as_slice(left_values)._c(),
+        as_slice(right_keys)._c(), as_slice(right_values)._c(), less.code,
+        C.byref(hk), C.byref(hv)))
+    return _out(exec, hk), _out(exec, hv)
+
+
+def _set(kind: int, exec: Executor, left, right, less: BinaryPredicateOp) -> DeviceVec:
+    h = C.c_void_p()
+    check(_lib().msv_set_op(exec.handle, kind, as_slice(left)._c(), as_slice(right)._c(),
+                            less.code, C.byref(h)))
+    return _out(exec, h)
+
+
+def set_union(exec: Executor, left, right, less: BinaryPredicateOp) -> DeviceVec:
+    """massively/src/api/algorithm/set.rs:44-110 (multiset union, capacity L+R)."""
+    return _set(0, exec, left, right, less)
+
+
+def set_intersection(exec: Executor, left, right, less: BinaryPredicateOp) -> DeviceVec:
+    """massively/src/api/algorithm/set.rs:112-145."""
+    return _set(1, exec, left, right, less)
+
+
+def set_difference(exec: Executor, left, right, less: BinaryPredicateOp) -> DeviceVec:
+    """massively/src/api/algorithm/set.rs:147-195."""
+    return _set(2, exec, left, right, less)

@@ -1,0 +1,146 @@
+"""Writes tests/golden/vectors.json: known-answer vectors of the reference.
+
+Every entry records the reference file:line its inputs and expected outputs
+come from (paths relative to the massively repository). Literal vectors are
+transcribed from the reference's tests and doc examples; the proptest
+regression inputs are parsed from the reference's committed
+``*.proptest-regressions`` data files (inputs only: their expected outputs are
+defined by the reference as the sequential oracle's result, which the tests
+compute with numpy library routines, not with the oracle under test).
+
+Run in the build container (it reads the reference checkout once):
+    python tests/golden/make_fixtures.py [/path/to/reference]
+"""
+from __future__ import annotations
+
+import json
+import re
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+
+U32_MAX = 0xFFFFFFFF
+U64_MAX = 0xFFFFFFFFFFFFFFFF
+I32_MIN, I32_MAX = -(2**31), 2**31 - 1
+I64_MIN, I64_MAX = -(2**63), 2**63 - 1
+
+
+def literal_vectors() -> list[dict]:
+    v = []
+    v.append(dict(name="inclusive_scan_add_doc", src="massively/src/api/algorithm/scan.rs:76-95",
+                  op="inclusive_scan", dtype="u32", reduce="add", input=[1, 2, 3, 4],
+                  expected=[1, 3, 6, 10]))
+    v.append(dict(name="exclusive_scan_add_doc", src="massively/src/api/algorithm/scan.rs:196-215",
+                  op="exclusive_scan", dtype="u32", reduce="add", init=0, input=[1, 2, 3, 4],
+                  expected=[0, 1, 3, 6]))
+    v.append(dict(name="exclusive_scan_init_once", src="massively/src/core/scan.rs:1467-1483",
+                  op="exclusive_scan_into", dtype="u32", reduce="add", init=10,
+                  input=[1, 2, 3, 4], output_before=[99] * 6, output_range=[1, 5],
+                  expected=[99, 10, 11, 13, 16, 99]))
+    v.append(dict(name="inclusive_scan_70001_ones", src="massively/src/core/scan.rs:1304-1319",
+                  op="inclusive_scan", dtype="u32", reduce="add", fill=1, len=70001,
+                  check_indices=[0, 255, 256, 65535, 70000],
+                  expected_at=[1, 256, 257, 65536, 70001], expected_last=70001))
+    v.append(dict(name="exclusive_scan_70001_ones_init10", src="massively/src/core/scan.rs:1486-1501",
+                  op="exclusive_scan", dtype="u32", reduce="add", init=10, fill=1, len=70001,
+                  check_indices=[0, 255, 256, 65535, 70000],
+                  expected_at=[10, 265, 266, 65545, 70010]))
+    v.append(dict(name="reduce_add_doc", src="massively/src/api/algorithm/reduce.rs:9-30",
+                  op="reduce", dtype="u32", reduce="add", init=0, input=[1, 2, 3, 4], expected=10))
+    v.append(dict(name="copy_where_doc", src="massively/src/api/algorithm/selection.rs:131-145",
+                  op="copy_where", dtype="u32", input=[10, 20, 30, 40], stencil=[1, 0, 1, 0],
+                  expected=[10, 30]))
+    v.append(dict(name="remove_where_doc", src="massively/src/api/algorithm/selection.rs:188-202",
+                  op="remove_where", dtype="u32", input=[10, 20, 30, 40], stencil=[1, 0, 1, 0],
+                  expected=[20, 40]))
+    v.append(dict(name="gather_doc", src="massively/src/api/algorithm/indexed.rs:157-171",
+                  op="gather", dtype="u32", values=[10, 20, 30], indices=[2, 0, 1],
+                  expected=[30, 10, 20]))
+    v.append(dict(name="scatter_doc", src="massively/src/api/algorithm/scatter.rs:225-247",
+                  op="scatter", dtype="u32", values=[10, 20, 30], indices=[2, 0, 1],
+                  output_before=[0, 0, 0], expected=[20, 30, 10]))
+    v.append(dict(name="merge_stable", src="massively/src/core/merge.rs:1262-1309",
+                  op="merge_by_key", dtype="u32", left_keys=[1, 2, 2, 5], right_keys=[2, 3, 4],
+                  left_values=[10, 20, 21, 50], right_values=[200, 300, 400],
+                  expected_keys=[1, 2, 2, 2, 3, 4, 5],
+                  expected_values=[10, 20, 21, 200, 300, 400, 50]))
+    v.append(dict(name="set_multiplicity", src="massively/src/core/set.rs:418-471",
+                  op="set_ops", dtype="u32", left=[1, 2, 2, 2, 4], right=[2, 2, 3, 4, 4],
+                  expected_union=[1, 2, 2, 2, 3, 4, 4], expected_intersection=[2, 2, 4],
+                  expected_difference=[1, 2]))
+    # Radix extremes: values are iota; expected = stable order of the keys.
+    for dt, keys in (("u32", [U32_MAX, 0, 1, 1 << 31, 0, U32_MAX]),
+                     ("u64", [U64_MAX, 0, 1, 1 << 63, 0, U64_MAX]),
+                     ("i32", [I32_MAX, I32_MIN, -1, 0, I32_MIN, I32_MAX]),
+                     ("i64", [I64_MAX, I64_MIN, -1, 0, I64_MIN, I64_MAX])):
+        v.append(dict(name=f"radix_extremes_{dt}", src="massively/tests/vector/api/radix.rs:26-49",
+                      op="radix_sort_by_key", dtype=dt, keys=keys, values=list(range(6)),
+                      expected=[1, 4, 2, 3, 0, 5]))
+    # Float total order (f32::total_cmp): -NaN < -inf < -1.5 < -0 (2), -0 (8) < +0 < 1.5 < inf < +NaN.
+    v.append(dict(name="radix_total_order_f32", src="massively/tests/vector/api/radix.rs:74-115",
+                  op="radix_sort_by_key", dtype="f32", key_bits=[
+                      0x7FC00001, 0x7F800000, 0x80000000, 0x00000000, 0xFF800000, 0xFFC00001,
+                      0x3FC00000, 0xBFC00000, 0x80000000],
+                  values=list(range(9)), expected=[5, 4, 7, 2, 8, 3, 6, 1, 0]))
+    v.append(dict(name="radix_total_order_f64", src="massively/tests/vector/api/radix.rs:74-115",
+                  op="radix_sort_by_key", dtype="f64", key_bits=[
+                      0x7FF8000000000001, 0x7FF0000000000000, 0x8000000000000000, 0,
+                      0xFFF0000000000000, 0xFFF8000000000001, 0x3FF8000000000000,
+                      0xBFF8000000000000, 0x8000000000000000],
+                  values=list(range(9)), expected=[5, 4, 7, 2, 8, 3, 6, 1, 0]))
+    v.append(dict(name="radix_doc_i32", src="massively/src/api/algorithm/by_key.rs:259-271",
+                  op="radix_sort_by_key", dtype="i32", keys=[3, -1, 2, -1],
+                  values=[30, 10, 20, 11], expected=[10, 11, 20, 30]))
+    v.append(dict(name="radix_stable_513", src="massively/tests/vector/api/radix.rs:262-275",
+                  op="radix_sort_by_key", dtype="u32",
+                  keys=[(i * 37) % 11 for i in range(513)], values=list(range(513)),
+                  expected="stable_argsort"))
+    v.append(dict(name="radix_empty_and_mismatch", src="massively/tests/vector/api/radix.rs:244-259",
+                  op="radix_errors", dtype="u32", empty=True, mismatch_keys=[1, 2],
+                  mismatch_values=[10], expected_error=["LengthMismatch", 2, 1]))
+    v.append(dict(name="seg_owned_results", src="massively/tests/seg/api.rs:286-323",
+                  op="seg_reduce", dtype="u32", reduce="add", init=0, values=[1, 1, 2, 3, 3],
+                  offsets=[0, 3, 5], expected=[4, 6]))
+    v.append(dict(name="seg_exclusive_from_reduce", src="massively/tests/seg/api.rs:326-348",
+                  op="seg_exclusive_scan", dtype="u32", reduce="add", init=0, values=[1, 2, 4],
+                  offsets=[0, 2, 3], expected=[0, 1, 0], expected_reduce=[3, 4]))
+    v.append(dict(name="segmentation_from_lengths", src="massively/src/seg/segmentation.rs:105-128",
+                  op="segmentation", lengths=[1, 2, 3], expected_offsets=[0, 1, 3, 6],
+                  expected_segment_ids=[0, 1, 1, 2, 2, 2]))
+    return v
+
+
+def regression_inputs(ref: Path) -> list[dict]:
+    out = []
+    base = ref / "verification/oracle/tests/vector"
+    for fname, kind in (("by_key.proptest-regressions", "pairs"),
+                        ("arity_value.proptest-regressions", "seed")):
+        path = base / fname
+        if not path.exists():
+            continue
+        for line_no, line in enumerate(path.read_text().splitlines(), start=1):
+            m = re.search(r"shrinks to (\w+) = (\[.*\])\s*$", line)
+            if not m:
+                continue
+            text = m.group(2).replace("(", "[").replace(")", "]")
+            data = json.loads(text)
+            out.append(dict(name=f"{fname.split('.')[0]}_{line_no}",
+                            src=f"verification/oracle/tests/vector/{fname}:{line_no}",
+                            kind=kind, data=data))
+    return out
+
+
+def main() -> None:
+    ref = Path(sys.argv[1]) if len(sys.argv) > 1 else Path("/root/reference")
+    doc = {
+        "about": "known-answer vectors transcribed from the massively reference tests/docs",
+        "vectors": literal_vectors(),
+        "regressions": regression_inputs(ref),
+    }
+    (HERE / "vectors.json").write_text(json.dumps(doc, indent=1) + "\n")
+    print(f"wrote {len(doc['vectors'])} vectors, {len(doc['regressions'])} regression inputs")
+
+
+if __name__ == "__main__":
+    main()
