@@ -1,21 +1,32 @@
-"""Benchmark of the vector-primitive hot path on one MI355X.
+"""Benchmark of the vector/segment primitive hot path on one MI355X.
 
-Workload (the HBM-scale vector-primitive configuration): inclusive_scan (add,
-wrapping) over 2^28 u32 uniform in [0, 1024) from seed 2. One step is one
-scan over the whole device-resident input; `value` is its throughput in GB/s
-of algorithmic traffic (8 bytes per element: read 4, write 4). The JSON line
-also carries the roofline of the scan kernel (HIP events on the executor's
-stream around each launch), the other primitives of the same configuration
-measured the same way, and the CPU baseline (the C oracle timed on a bounded
-sample on this host, one thread).
+Headline workload (the HBM-scale vector-primitive configuration):
+inclusive_scan (add, wrapping) over 2^28 u32 uniform in [0, 1024) from seed 2.
+One step is one scan over the device-resident input; `value` is its
+throughput in GB/s of algorithmic traffic (8 bytes per element: read 4,
+write 4), and `roofline` relates it to the 8 TB/s HBM3E peak (HIP events on
+the executor's stream around each launch). The same JSON line carries, under
+`extra`, the other primitives of the configurations measured the same way:
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--only scan] [--no-cpu]
+* sort_by_key: 2^27 u64 keys (uniform seed 6, Zipf s=1.1 over 2^20 values,
+  top 8 bits constant) + u32 payload; 2^28 u32 keys + u32 payload; the plain
+  2^20 u32 keys + iota values case (Gkeys/s);
+* reduce f32 2^28, copy_where 2^28 u32 at 10% / 50% (stencil form and fused
+  predicate form), gather / scatter through a seeded random permutation of
+  2^28, merge of 2 x 2^27 sorted u32, set ops at 25% overlap (2^27 each);
+* seg inclusive_scan (u32) / reduce (f32) / sort (u32 key, f32 value) over
+  2^26 elements in 2^20 log-normal segments.
+
+cpu_baseline: the C oracle (a restatement of the reference's sequential CPU
+path) timed on this host, one thread, on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--only a,b] [--no-cpu] [--quick]
 """
 from __future__ import annotations
 
 import argparse
 import json
-import os
+import math
 import statistics
 import sys
 import time
@@ -33,61 +44,212 @@ def log(msg: str) -> None:
 
 
 def timed(ex, fn, steps: int, warmup: int):
-    """Runs fn() warmup+steps times; returns per-step HIP-event ms on ex's stream."""
+    """Runs fn() warmup+steps times; per-step HIP-event milliseconds on ex's stream."""
     from massively_amd.hiptime import Event
 
     for _ in range(warmup):
         fn()
     ex.sync()
     evs = [(Event(), Event()) for _ in range(steps)]
-    t0 = time.perf_counter()
     for a, b in evs:
         a.record(ex.stream)
         fn()
         b.record(ex.stream)
     ex.sync()
-    wall = (time.perf_counter() - t0) * 1e3
-    ms = [a.elapsed_ms(b) for a, b in evs]
-    return ms, wall
+    return [a.elapsed_ms(b) for a, b in evs]
 
 
-def bench_scan(ex, m, n, steps, warmup):
-    import numpy as np
-    from massively_amd import random as mr
-
-    x = ex.alloc(np.uint32, n)
-    mr.uniform_u32(ex, 0, 1023, 2, x)
-    holder = {}
-
-    def step():
-        holder["y"] = m.vector.inclusive_scan(ex, x, m.op.Add)
-
-    ms, wall = timed(ex, step, steps, warmup)
+def summarize(name, ms, bytes_=None, items=None, **kw):
     avg = statistics.fmean(ms)
-    gbs = 8.0 * n / (avg * 1e-3) / 1e9
-    return dict(op="inclusive_scan_u32_add", n=n, ms=avg, ms_median=statistics.median(ms),
-                gbs=gbs, frac=gbs / HBM_PEAK_GBS, bytes_per_elem=8, wall_ms=wall)
+    d = dict(op=name, ms=round(avg, 5), ms_median=round(statistics.median(ms), 5), **kw)
+    if bytes_ is not None:
+        gbs = bytes_ / (avg * 1e-3) / 1e9
+        d.update(gbs=round(gbs, 1), frac_hbm=round(gbs / HBM_PEAK_GBS, 4), bytes=int(bytes_))
+    if items is not None:
+        d["gitems_per_s"] = round(items / (avg * 1e-3) / 1e9, 3)
+    log(f"[bench] {name}: {avg:.4f} ms" + (f"  {d['gbs']:.1f} GB/s ({d['frac_hbm']:.3f})" if bytes_ else "")
+        + (f"  {d['gitems_per_s']:.3f} G/s" if items else ""))
+    return d
 
 
-def bench_reduce(ex, m, n, steps, warmup):
-    import numpy as np
-    from massively_amd import random as mr
+class Ctx:
+    def __init__(self, ex, m, steps, warmup):
+        import numpy as np
 
-    # N(0,1)-like f32 input generated on the device is not needed for timing;
-    # the reduce is data-independent. Use uniform bits reinterpreted as small floats.
-    x = ex.alloc(np.uint32, n)
-    mr.uniform_u32(ex, 0x3F000000, 0x3F800000, 3, x)
-    xf = ex.wrap(x.device_ptr, np.float32, n)
-    ms, wall = timed(ex, lambda: m.vector.reduce(ex, xf, 0.0, m.op.Add), steps, warmup)
-    avg = statistics.fmean(ms)
-    gbs = 4.0 * n / (avg * 1e-3) / 1e9
-    return dict(op="reduce_f32_add", n=n, ms=avg, gbs=gbs, frac=gbs / HBM_PEAK_GBS,
-                bytes_per_elem=4, note="includes the 4-byte device-to-host result copy")
+        self.ex, self.m, self.np = ex, m, np
+        self.steps, self.warmup = steps, warmup
+        from massively_amd import random as mr
+
+        self.mr = mr
+
+
+def w_scan(c, log2n=28):
+    n = 1 << log2n
+    x = c.ex.alloc(c.np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, 1023, 2, x)
+    ms = timed(c.ex, lambda: c.m.vector.inclusive_scan(c.ex, x, c.m.op.Add), c.steps, c.warmup)
+    return summarize("inclusive_scan_u32_add_2^%d" % log2n, ms, 8.0 * n, n, n=n)
+
+
+def w_reduce(c):
+    n = 1 << 28
+    x = c.ex.alloc(c.np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0x3F000000, 0x3F800000, 3, x)  # f32 bit patterns in [0.5, 1)
+    xf = c.ex.wrap(x.device_ptr, c.np.float32, n)
+    ms = timed(c.ex, lambda: c.m.vector.reduce(c.ex, xf, 0.0, c.m.op.Add), c.steps, c.warmup)
+    return summarize("reduce_f32_add_2^28", ms, 4.0 * n, n, note="includes the scalar device-to-host copy")
+
+
+def w_copy_where(c):
+    n = 1 << 28
+    np, m = c.np, c.m
+    x = c.ex.alloc(np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, 2**32 - 1, 4, x)
+    u = c.ex.alloc(np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, 99, 5, u)
+    out = {}
+    for p in (10, 50):
+        # Stencil flags = (u < p), built on the device by a table lookup.
+        table = c.ex.to_device((np.arange(100) < p).astype(np.uint32))
+        st = m.vector.gather(c.ex, table, u)
+        ms = timed(c.ex, lambda: m.vector.copy_where(c.ex, x, st), c.steps, c.warmup)
+        out[f"stencil_{p}pct"] = summarize(f"copy_where_stencil_{p}pct_2^28", ms, (8.0 + 4.0 * p / 100) * n, n)
+        thr = int(p / 100 * 2**32)
+        ms = timed(c.ex, lambda: m.vector.copy_where_if(c.ex, x, x, m.op.Lt(thr)), c.steps, c.warmup)
+        out[f"predicate_{p}pct"] = summarize(f"copy_where_pred_{p}pct_2^28", ms, (4.0 + 4.0 * p / 100) * n, n)
+    return out
+
+
+def random_permutation(c, n, seed):
+    keys = c.ex.alloc(c.np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, 2**32 - 1, seed, keys)
+    iota = c.ex.alloc(c.np.uint32, n)
+    c.mr.iota_u32(c.ex, iota)
+    return c.m.vector.radix_sort_by_key(c.ex, keys, iota)
+
+
+def w_gather_scatter(c):
+    n = 1 << 28
+    np, m = c.np, c.m
+    idx = random_permutation(c, n, 9)
+    v = c.ex.alloc(np.uint32, n)
+    c.mr.iota_u32(c.ex, v)
+    ms = timed(c.ex, lambda: m.vector.gather(c.ex, v, idx), c.steps, c.warmup)
+    g = summarize("gather_u32_perm_2^28", ms, 12.0 * n, n)
+    out = c.ex.alloc(np.uint32, n)
+    ms = timed(c.ex, lambda: m.vector.scatter(c.ex, v, idx, out), c.steps, c.warmup)
+    s = summarize("scatter_u32_perm_2^28", ms, 12.0 * n, n)
+    return {"gather": g, "scatter": s}
+
+
+def _sort_case(c, name, keys, vals, n, key_bytes, val_bytes, passes):
+    m = c.m
+    ms = timed(c.ex, lambda: m.vector.radix_sort_by_key(c.ex, keys, vals), c.steps, c.warmup)
+    model = n * (key_bytes + passes * 2 * (key_bytes + val_bytes))
+    return summarize(name, ms, model, n, model_bytes_per_key=model / n)
+
+
+def w_sort(c):
+    np, m = c.np, c.m
+    out = {}
+    n = 1 << 27
+    keys = c.ex.alloc(np.uint64, n)
+    c.mr.uniform_u64(c.ex, 0, 2**64 - 1, 6, keys)
+    vals = c.ex.alloc(np.uint32, n)
+    c.mr.iota_u32(c.ex, vals)
+    out["u64_uniform_2^27"] = _sort_case(c, "sort_by_key_u64_uniform_2^27", keys, vals, n, 8, 4, 8)
+    hk = c.ex.to_host(keys)
+    rng = np.random.default_rng(6)
+    # Zipf s=1.1 over 2^20 distinct values: inverse-CDF sampling of ranks.
+    ranks = np.arange(1, (1 << 20) + 1, dtype=np.float64)
+    cdf = np.cumsum(ranks ** -1.1)
+    cdf /= cdf[-1]
+    r = np.searchsorted(cdf, rng.random(n)).astype(np.uint64)
+    zipf = (r * np.uint64(0x9E3779B97F4A7C15)) ^ np.uint64(0xD1B54A32D192ED03)
+    zk = c.ex.to_device(zipf)
+    out["u64_zipf_2^27"] = _sort_case(c, "sort_by_key_u64_zipf1.1_2^27", zk, vals, n, 8, 4, 8)
+    del zk, zipf, r
+    top = (np.uint64(0xA5) << np.uint64(56)) | (hk & np.uint64((1 << 56) - 1))
+    tk = c.ex.to_device(top)
+    out["u64_top8const_2^27"] = _sort_case(c, "sort_by_key_u64_top8const_2^27", tk, vals, n, 8, 4, 7)
+    del tk, top, hk
+    n28 = 1 << 28
+    k28 = c.ex.alloc(np.uint32, n28)
+    c.mr.uniform_u32(c.ex, 0, 2**32 - 1, 1, k28)
+    v28 = c.ex.alloc(np.uint32, n28)
+    c.mr.iota_u32(c.ex, v28)
+    out["u32_2^28"] = _sort_case(c, "sort_by_key_u32_2^28", k28, v28, n28, 4, 4, 4)
+    del k28, v28
+    n20 = 1 << 20
+    k20 = c.ex.alloc(np.uint32, n20)
+    c.mr.uniform_u32(c.ex, 0, 2**32 - 1, 1, k20)
+    v20 = c.ex.alloc(np.uint32, n20)
+    c.mr.iota_u32(c.ex, v20)
+    ms = timed(c.ex, lambda: m.vector.sort_by_key(c.ex, k20, v20, m.op.Less), max(c.steps, 20), c.warmup)
+    out["u32_iota_2^20"] = summarize("sort_by_key_u32_iota_2^20", ms, 36.0 * n20, n20)
+    return out
+
+
+def w_merge_sets(c):
+    np, m = c.np, c.m
+    import oracle
+
+    n = 1 << 27
+    a = c.ex.alloc(np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, (1 << 26) - 1, 11, a)
+    b = c.ex.alloc(np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, (1 << 26) - 1, 12, b)
+    sa, sb = m.vector.sort(c.ex, a, m.op.Less), m.vector.sort(c.ex, b, m.op.Less)
+    out = {}
+    ms = timed(c.ex, lambda: m.vector.merge(c.ex, sa, sb, m.op.Less), c.steps, c.warmup)
+    out["merge"] = summarize("merge_u32_2x2^27", ms, 8.0 * 2 * n, 2 * n)
+    # 25% overlap: sorted pcg images of [0, n) and [0.75n, 1.75n).
+    ha = np.sort(oracle.uniform_u32(0, 2**32 - 1, 0, n))
+    hb = np.sort(oracle.uniform_u32(0, 2**32 - 1, 0, n, start=3 * n // 4))
+    A, B = c.ex.to_device(ha), c.ex.to_device(hb)
+    for kind, fn, outlen in (("intersection", m.vector.set_intersection, n // 4),
+                             ("difference", m.vector.set_difference, 3 * n // 4),
+                             ("union", m.vector.set_union, 7 * n // 4)):
+        ms = timed(c.ex, lambda: fn(c.ex, A, B, m.op.Less), c.steps, c.warmup)
+        out[kind] = summarize(f"set_{kind}_25pct_2x2^27", ms, 4.0 * (2 * n) + 4.0 * outlen, 2 * n)
+    return out
+
+
+def lognormal_offsets(np, S=1 << 20, total=1 << 26, seed=7):
+    rng = np.random.default_rng(seed)
+    L = np.minimum(np.round(rng.lognormal(math.log(32), 1.1774, S)), 1 << 16).astype(np.int64)
+    L[rng.random(S) < 0.02] = 0
+    L = np.floor(L * (total / L.sum())).astype(np.int64)
+    L[-1] += total - int(L.sum())
+    return np.concatenate([[0], np.cumsum(L)]).astype(np.uint32)
+
+
+def w_seg(c):
+    np, m = c.np, c.m
+    from massively_amd import seg
+
+    off = lognormal_offsets(np)
+    n, S = int(off[-1]), off.size - 1
+    sg = seg.Segmentation.from_offsets(c.ex, c.ex.to_device(off))
+    x = c.ex.alloc(np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, 2**32 - 1, 7, x)
+    f = c.ex.alloc(np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0x3F000000, 0x3F800000, 8, f)
+    ff = c.ex.wrap(f.device_ptr, np.float32, n)
+    out = {}
+    it = seg.SegmentIterator(x, sg)
+    ms = timed(c.ex, lambda: seg.ForEachSegment(seg.InclusiveScan(m.op.Add)).run(c.ex, it), c.steps, c.warmup)
+    out["inclusive_scan_u32"] = summarize("seg_inclusive_scan_u32_2^26_in_2^20", ms, 8.0 * n + 4.0 * (S + 1), n)
+    itf = seg.SegmentIterator(ff, sg)
+    ms = timed(c.ex, lambda: seg.ForEachSegment(seg.Reduce(m.op.Add, 0.0)).run(c.ex, itf), c.steps, c.warmup)
+    out["reduce_f32"] = summarize("seg_reduce_f32_2^26_in_2^20", ms, 4.0 * n + 4.0 * (S + 1) + 4.0 * S, n)
+    its = seg.SegmentIterator((x, ff), sg)
+    ms = timed(c.ex, lambda: seg.ForEachSegment(seg.Sort(m.op.Less)).run(c.ex, its), max(2, c.steps // 4), 1)
+    out["sort_u32_f32"] = summarize("seg_sort_u32key_f32val_2^26_in_2^20", ms, 16.0 * n + 4.0 * (S + 1), n)
+    return out
 
 
 def cpu_baseline(n_sample: int, reps: int):
-    import numpy as np
-
     import oracle
 
     x = oracle.uniform_u32(0, 1023, 2, n_sample)
@@ -98,9 +260,33 @@ def cpu_baseline(n_sample: int, reps: int):
         oracle.inclusive_scan(x, "add")
         t.append(time.perf_counter() - t0)
     sec = statistics.median(t)
-    return dict(value=8.0 * n_sample / sec / 1e9, unit="GB/s", cores=1, kind="port",
+    return dict(value=round(8.0 * n_sample / sec / 1e9, 3), unit="GB/s", cores=1, kind="port",
                 sample=f"inclusive_scan add over {n_sample} u32 (seed 2), median of {reps} "
-                       f"single-thread runs of the C oracle")
+                       f"single-thread runs of the C oracle (restatement of the reference CPU path)")
+
+
+def cpu_sort_baseline(n_sample: int):
+    import numpy as np
+
+    import oracle
+
+    keys = oracle.uniform_u64(0, 2**64 - 1, 6, n_sample)
+    vals = np.arange(n_sample, dtype=np.uint32)
+    t0 = time.perf_counter()
+    oracle.radix_sort_by_key(keys, vals)
+    sec = time.perf_counter() - t0
+    return dict(value=round(n_sample / sec / 1e9, 5), unit="Gkeys/s", cores=1, kind="port",
+                sample=f"stable sort_by_key of {n_sample} u64 keys (seed 6) + u32 payload, one run")
+
+
+WORKLOADS = {
+    "reduce": w_reduce,
+    "copy_where": w_copy_where,
+    "gather_scatter": w_gather_scatter,
+    "sort": w_sort,
+    "merge_sets": w_merge_sets,
+    "seg": w_seg,
+}
 
 
 def main() -> int:
@@ -108,53 +294,59 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--log2n", type=int, default=28)
-    ap.add_argument("--only", default="")
+    ap.add_argument("--only", default="", help="comma list of: scan," + ",".join(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
-              "(the multi-GPU design is documented, not run)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported (the multi-GPU "
+              "sharding is a design, not a measured path)", file=sys.stderr)
         return 2
 
     import massively_amd as m
 
     ex = m.Executor(0)
-    n = 1 << a.log2n
-    log(f"[bench] scan n=2^{a.log2n} steps={a.steps} warmup={a.warmup}")
-    scan = bench_scan(ex, m, n, a.steps, a.warmup)
-    log(f"[bench] scan: {scan['ms']:.4f} ms  {scan['gbs']:.1f} GB/s  ({scan['frac']:.3f} of peak)")
+    c = Ctx(ex, m, a.steps, a.warmup)
+    only = set(filter(None, a.only.split(",")))
+    n = 1 << 28
+    log(f"[bench] scan n=2^28 steps={a.steps} warmup={a.warmup}")
+    scan = w_scan(c)
     extra = {}
-    if a.only in ("", "reduce"):
-        r = bench_reduce(ex, m, n, a.steps, a.warmup)
-        extra["reduce_f32"] = r
-        log(f"[bench] reduce: {r['ms']:.4f} ms  {r['gbs']:.1f} GB/s")
+    for name, fn in WORKLOADS.items():
+        if only and name not in only:
+            continue
+        log(f"[bench] workload {name}")
+        extra[name] = fn(c)
+        ex.trim()
 
     cpu = None
     if not a.no_cpu:
-        log("[bench] cpu baseline (oracle, 1 thread)")
+        log("[bench] cpu baseline (C oracle, 1 thread)")
         cpu = cpu_baseline(n, 3)
-        log(f"[bench] cpu: {cpu['value']:.2f} GB/s")
+        log(f"[bench] cpu scan: {cpu['value']:.2f} GB/s")
+        if not only or "sort" in only:
+            extra["cpu_sort_baseline"] = cpu_sort_baseline(1 << 23)
+            log(f"[bench] cpu sort: {extra['cpu_sort_baseline']['value']:.4f} Gkeys/s")
 
+    sort_g = extra.get("sort", {}).get("u32_2^28", {}).get("gitems_per_s")
     line = {
         "metric": METRIC,
-        "value": round(scan["gbs"], 2),
+        "value": scan["gbs"],
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(scan["ms"], 5),
+        "ms_per_step": scan["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
-        "data": "synthetic (seeded PCG streams of the reference generator, on device)",
-        "config": {"workload": f"inclusive_scan add, 2^{a.log2n} u32 uniform in [0,1024), "
-                               "wrapping; device-resident input",
-                   "n": n},
-        "roofline": {"bound": "hbm", "achieved": round(scan["gbs"], 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(scan["frac"], 4), "traffic": None},
+        "data": "synthetic (seeded PCG streams of the reference generator, generated on device)",
+        "config": {"workload": "inclusive_scan add over 2^28 u32 uniform in [0,1024) (seed 2), "
+                               "wrapping; device-resident input", "n": n},
+        "roofline": {"bound": "hbm", "achieved": scan["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": scan["frac_hbm"], "traffic": None},
         "cpu_baseline": cpu,
+        "sort_gkeys_per_s_u32_2^28": sort_g,
         "extra": extra,
     }
     print(json.dumps(line), flush=True)

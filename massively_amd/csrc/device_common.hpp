@@ -417,4 +417,25 @@ __device__ inline typename Bits<K>::U radix_encode(K k) {
   }
 }
 
+// Comparison modes of the ordering primitives: radix total order, and the
+// built-in comparators Less / Greater (under which -0.0 == +0.0).
+enum OrderMode : int { ORD_TOTAL = 0, ORD_LESS = 1, ORD_GREATER = 2 };
+
+// Unsigned word whose numeric order is the requested key order.
+template <typename K, int MODE>
+__device__ inline typename Bits<K>::U order_bits(K k) {
+  using U = typename Bits<K>::U;
+  constexpr U sign = (U)1 << (sizeof(U) * 8 - 1);
+  U e;
+  if constexpr (IsFloat<K>::value && MODE != ORD_TOTAL) {
+    U raw = bit_cast<U>(k);
+    if ((raw & (U)~sign) == 0) raw = 0;  // -0 == +0 under `<`
+    e = (raw & sign) ? (U)~raw : (U)(raw ^ sign);
+  } else {
+    e = radix_encode<K>(k);
+  }
+  if constexpr (MODE == ORD_GREATER) e = (U)~e;
+  return e;
+}
+
 }  // namespace msv

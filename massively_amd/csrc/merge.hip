@@ -1,0 +1,530 @@
+// Merge-path merge, merge_by_key and ordered multiset operations for gfx950.
+//
+// merge / merge_by_key (massively/src/api/algorithm/merge.rs:109-282,
+// core/merge.rs): stable; on equal keys the left element comes first
+// (core/merge.rs:182-195, oracle vector.rs:493-513). A partition kernel finds
+// the merge-path split of every 2048-output tile with one binary search per
+// tile; each block loads its two input windows into LDS, every thread finds
+// its own split inside them and merges 8 outputs, and the outputs (and the
+// values, read at their source positions) leave through LDS as contiguous runs.
+//
+// set_union / set_intersection / set_difference (api/algorithm/set.rs:44-195,
+// multiset rules core/set.rs:18-46, oracle vector.rs:595-664): the oracle walks
+// both inputs once, pairing equal keys (the k-th occurrence on the left with
+// the k-th on the right). Splitting that walk is valid at any point that does
+// not separate a matched pair, so tile and thread splits are taken from the
+// merge path and then moved, inside a run of equal keys, to the nearest
+// pair-aligned point (both sides consumed equally, or one side's run
+// exhausted). Each thread then runs the oracle's loop on its own range; the
+// emitted counts are compacted with a block scan plus a decoupled lookback
+// over tiles, and the block of the last tile stores the output's
+// device-resident length.
+#include "device_common.hpp"
+#include "msv_internal.hpp"
+
+namespace msv {
+
+constexpr int kMergeBlock = 256;
+constexpr int kMergeIpt = 8;
+constexpr uint32_t kMergeTile = kMergeBlock * kMergeIpt;
+
+// Keys are compared through their order words (order_bits).
+template <typename K, int MODE>
+struct Ord {
+  __device__ static inline typename Bits<K>::U w(K k) { return order_bits<K, MODE>(k); }
+};
+
+// Number of A elements among the first d outputs of the stable merge
+// (ties: A first). Works on global or LDS pointers.
+template <typename K, int MODE>
+__device__ inline uint32_t merge_path(const K* A, uint32_t na, const K* B, uint32_t nb, uint32_t d) {
+  uint32_t lo = d > nb ? d - nb : 0;
+  uint32_t hi = d < na ? d : na;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (!(Ord<K, MODE>::w(B[d - 1 - mid]) < Ord<K, MODE>::w(A[mid]))) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename K, int MODE>
+__device__ inline uint32_t lower_bound_w(const K* A, uint32_t lo, uint32_t hi,
+                                         typename Bits<K>::U v) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (Ord<K, MODE>::w(A[mid]) < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename K, int MODE>
+__device__ inline uint32_t upper_bound_w(const K* A, uint32_t lo, uint32_t hi,
+                                         typename Bits<K>::U v) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (!(v < Ord<K, MODE>::w(A[mid]))) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Merge-path split at diagonal d, moved to the nearest pair-aligned point of
+// the oracle's set walk when it falls inside a run of equal keys.
+template <typename K, int MODE>
+__device__ inline void set_split(const K* A, uint32_t na, const K* B, uint32_t nb, uint32_t d,
+                                 uint32_t& i_out, uint32_t& j_out) {
+  using U = typename Bits<K>::U;
+  uint32_t i = merge_path<K, MODE>(A, na, B, nb, d);
+  uint32_t j = d - i;
+  // Previous element in merge order (ties: B after A) and next element (ties: A first).
+  const bool has_prev = (i > 0) || (j > 0);
+  const bool has_next = (i < na) || (j < nb);
+  if (has_prev && has_next) {
+    U prev, next;
+    if (i > 0 && j > 0) {
+      const U a = Ord<K, MODE>::w(A[i - 1]), b = Ord<K, MODE>::w(B[j - 1]);
+      prev = a < b ? b : (b < a ? a : b);
+    } else {
+      prev = i > 0 ? Ord<K, MODE>::w(A[i - 1]) : Ord<K, MODE>::w(B[j - 1]);
+    }
+    if (i < na && j < nb) {
+      const U a = Ord<K, MODE>::w(A[i]), b = Ord<K, MODE>::w(B[j]);
+      next = b < a ? b : a;
+    } else {
+      next = i < na ? Ord<K, MODE>::w(A[i]) : Ord<K, MODE>::w(B[j]);
+    }
+    if (prev == next) {
+      const U v = next;
+      const uint32_t lba = lower_bound_w<K, MODE>(A, 0, i, v);
+      const uint32_t lbb = lower_bound_w<K, MODE>(B, 0, j, v);
+      const uint32_t uba = upper_bound_w<K, MODE>(A, i, na, v);
+      const uint32_t ubb = upper_bound_w<K, MODE>(B, j, nb, v);
+      const uint32_t ma = uba - lba, mb = ubb - lbb;
+      const uint32_t mp = ma < mb ? ma : mb;
+      const uint32_t x = (i - lba) + (j - lbb);
+      uint32_t ra, rb;
+      if (x <= 2 * mp) {
+        ra = rb = x / 2;
+      } else if (ma > mb) {
+        rb = mb;
+        ra = x - mb;
+      } else {
+        ra = ma;
+        rb = x - ma;
+      }
+      i = lba + ra;
+      j = lbb + rb;
+    }
+  }
+  i_out = i;
+  j_out = j;
+}
+
+// ---- partition ------------------------------------------------------------
+template <typename K, int MODE, bool SET>
+__global__ void merge_partition_kernel(const K* __restrict__ A, uint32_t na,
+                                       const K* __restrict__ B, uint32_t nb, uint32_t ntiles,
+                                       uint32_t* __restrict__ sa, uint32_t* __restrict__ sb) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  const uint64_t total = (uint64_t)na + nb;
+  const uint32_t d = (uint32_t)min((uint64_t)t * kMergeTile, total);
+  uint32_t i, j;
+  if constexpr (SET) {
+    set_split<K, MODE>(A, na, B, nb, d, i, j);
+  } else {
+    i = merge_path<K, MODE>(A, na, B, nb, d);
+    j = d - i;
+  }
+  sa[t] = i;
+  sb[t] = j;
+}
+
+template <int VS>
+struct MergeValue { using T = uint32_t; };
+template <>
+struct MergeValue<8> { using T = uint64_t; };
+
+// ---- merge ----------------------------------------------------------------
+template <typename K, int MODE, int VS>
+__global__ __launch_bounds__(kMergeBlock) void merge_tile_kernel(
+    const K* __restrict__ A, uint32_t na, const K* __restrict__ B, uint32_t nb,
+    const void* __restrict__ Av, const void* __restrict__ Bv, const uint32_t* __restrict__ sa,
+    const uint32_t* __restrict__ sb, K* __restrict__ out_k, void* __restrict__ out_v) {
+  using VT = typename MergeValue<VS>::T;
+  __shared__ K s_keys[kMergeTile];
+  __shared__ uint32_t s_src[VS ? kMergeTile : 1];
+  const uint32_t tid = threadIdx.x, b = blockIdx.x;
+  const uint32_t i0 = sa[b], i1 = sa[b + 1], j0 = sb[b], j1 = sb[b + 1];
+  const uint32_t la = i1 - i0, lb = j1 - j0, L = la + lb;
+  for (uint32_t k = tid; k < L; k += kMergeBlock) s_keys[k] = k < la ? A[i0 + k] : B[j0 + k - la];
+  __syncthreads();
+  const K* LA = s_keys;
+  const K* LB = s_keys + la;
+  const uint32_t d = min(tid * (uint32_t)kMergeIpt, L);
+  uint32_t ia = merge_path<K, MODE>(LA, la, LB, lb, d);
+  uint32_t jb = d - ia;
+  K ok[kMergeIpt];
+  uint32_t os[kMergeIpt];
+#pragma unroll
+  for (int q = 0; q < kMergeIpt; ++q) {
+    if (d + q < L) {
+      bool take_a;
+      if (ia >= la) take_a = false;
+      else if (jb >= lb) take_a = true;
+      else take_a = !(Ord<K, MODE>::w(LB[jb]) < Ord<K, MODE>::w(LA[ia]));
+      if (take_a) {
+        ok[q] = LA[ia];
+        os[q] = i0 + ia;
+        ++ia;
+      } else {
+        ok[q] = LB[jb];
+        os[q] = na + j0 + jb;
+        ++jb;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kMergeIpt; ++q) {
+    if (d + q < L) {
+      s_keys[d + q] = ok[q];
+      if constexpr (VS > 0) s_src[d + q] = os[q];
+    }
+  }
+  __syncthreads();
+  const size_t o = (size_t)i0 + j0;
+  for (uint32_t k = tid; k < L; k += kMergeBlock) {
+    out_k[o + k] = s_keys[k];
+    if constexpr (VS > 0) {
+      const uint32_t s = s_src[k];
+      ((VT*)out_v)[o + k] = s < na ? ((const VT*)Av)[s] : ((const VT*)Bv)[s - na];
+    }
+  }
+}
+
+// ---- ordered set operations ---------------------------------------------------
+enum SetKind : int { SET_UNION = 0, SET_INTERSECTION = 1, SET_DIFFERENCE = 2 };
+
+template <typename K, int MODE, int KIND>
+__global__ __launch_bounds__(kMergeBlock) void set_tile_kernel(
+    const K* __restrict__ A, uint32_t na, const K* __restrict__ B, uint32_t nb,
+    const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb, K* __restrict__ out,
+    uint32_t* __restrict__ out_len, uint64_t* counter, uint64_t* status, uint32_t ntiles) {
+  constexpr int NW = kMergeBlock / kWave;
+  constexpr uint32_t CAP = kMergeTile + 2;
+  __shared__ K s_keys[CAP];
+  __shared__ uint32_t s_split[kMergeBlock + 1][2];
+  __shared__ uint32_t s_wave[NW];
+  __shared__ uint32_t s_prefix;
+  __shared__ TileTicket s_ticket;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
+  if (tid == 0) s_ticket = draw_tile(counter, ntiles);
+  __syncthreads();
+  const uint32_t b = s_ticket.tile, gen = s_ticket.gen;
+  const uint32_t i0 = sa[b], i1 = sa[b + 1], j0 = sb[b], j1 = sb[b + 1];
+  const uint32_t la = i1 - i0, lb = j1 - j0, L = la + lb;
+  for (uint32_t k = tid; k < L; k += kMergeBlock) s_keys[k] = k < la ? A[i0 + k] : B[j0 + k - la];
+  __syncthreads();
+  const K* LA = s_keys;
+  const K* LB = s_keys + la;
+  {
+    uint32_t ia, jb;
+    set_split<K, MODE>(LA, la, LB, lb, min(tid * (uint32_t)kMergeIpt, L), ia, jb);
+    s_split[tid][0] = ia;
+    s_split[tid][1] = jb;
+    if (tid == 0) {
+      s_split[kMergeBlock][0] = la;
+      s_split[kMergeBlock][1] = lb;
+    }
+  }
+  __syncthreads();
+  uint32_t ia = s_split[tid][0], jb = s_split[tid][1];
+  const uint32_t ia_e = s_split[tid + 1][0], jb_e = s_split[tid + 1][1];
+  // The oracle's loop (vector.rs:595-664) on this thread's range.
+  // A thread range holds at most kMergeIpt + 2 steps (both of its ends can
+  // move by one when they are pair-aligned).
+  K ok[kMergeIpt + 2];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int q = 0; q < kMergeIpt + 2; ++q) {
+    if (ia < ia_e || jb < jb_e) {
+      int step;  // 0: left only, 1: right only, 2: matched pair
+      if (ia < ia_e && jb < jb_e) {
+        const auto wa = Ord<K, MODE>::w(LA[ia]), wb = Ord<K, MODE>::w(LB[jb]);
+        step = wa < wb ? 0 : (wb < wa ? 1 : 2);
+      } else {
+        step = ia < ia_e ? 0 : 1;
+      }
+      if (step == 0) {
+        if (KIND != SET_INTERSECTION) ok[cnt++] = LA[ia];
+        ++ia;
+      } else if (step == 1) {
+        if (KIND == SET_UNION) ok[cnt++] = LB[jb];
+        ++jb;
+      } else {
+        if (KIND != SET_DIFFERENCE) ok[cnt++] = LA[ia];
+        ++ia;
+        ++jb;
+      }
+    }
+  }
+  // Block scan of emitted counts, then lookback over tiles.
+  const uint32_t incl = wave_inclusive_scan<uint32_t, OP_ADD>(cnt);
+  if (lane == 63) s_wave[wid] = incl;
+  __syncthreads();
+  uint32_t woff = 0, agg = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    if (w < (int)wid) woff += s_wave[w];
+    agg += s_wave[w];
+  }
+  if (wid == 0) {
+    TileStatus<uint32_t> st{status, gen};
+    uint32_t excl = 0;
+    if (b == 0) {
+      if (lane == 0) st.publish(0, kFlagInc, 0, agg);
+    } else {
+      if (lane == 0) st.publish(b, kFlagAgg, 0, agg);
+      excl = wave_lookback<uint32_t, OP_ADD>(st, b);
+      if (lane == 0) st.publish(b, kFlagInc, 0, excl + agg);
+    }
+    if (lane == 0) {
+      s_prefix = excl;
+      if (b == ntiles - 1) *out_len = excl + agg;
+    }
+  }
+  __syncthreads();  // everyone is done reading s_keys as input
+  const uint32_t my = woff + incl - cnt;
+#pragma unroll
+  for (int q = 0; q < kMergeIpt + 2; ++q)
+    if ((uint32_t)q < cnt) s_keys[my + q] = ok[q];
+  __syncthreads();
+  const size_t o = s_prefix;
+  for (uint32_t k = tid; k < agg; k += kMergeBlock) out[o + k] = s_keys[k];
+}
+
+__global__ void set_len_zero_kernel(uint32_t* p) { *p = 0; }
+
+// ---- host ---------------------------------------------------------------------
+template <typename K, int MODE, bool SET>
+static msv_status partition(Ctx& c, const K* A, uint32_t na, const K* B, uint32_t nb,
+                            uint32_t ntiles, uint32_t** sa, uint32_t** sb) {
+  *sa = (uint32_t*)c.scratch(sizeof(uint32_t) * (ntiles + 1));
+  *sb = (uint32_t*)c.scratch(sizeof(uint32_t) * (ntiles + 1));
+  if (!*sa || !*sb) return MSV_ERR_OUT_OF_MEMORY;
+  hipLaunchKernelGGL((merge_partition_kernel<K, MODE, SET>), dim3((ntiles + 1 + 255) / 256),
+                     dim3(256), 0, c.stream(), A, na, B, nb, ntiles, *sa, *sb);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+template <typename K, int MODE, int VS>
+static msv_status merge_impl(Ctx& c, const void* A, uint32_t na, const void* B, uint32_t nb,
+                             const void* Av, const void* Bv, void* out_k, void* out_v) {
+  const uint64_t total = (uint64_t)na + nb;
+  if (total == 0) return MSV_OK;
+  const uint32_t ntiles = (uint32_t)((total + kMergeTile - 1) / kMergeTile);
+  uint32_t *sa, *sb;
+  MSV_TRY((partition<K, MODE, false>(c, (const K*)A, na, (const K*)B, nb, ntiles, &sa, &sb)));
+  hipLaunchKernelGGL((merge_tile_kernel<K, MODE, VS>), dim3(ntiles), dim3(kMergeBlock), 0,
+                     c.stream(), (const K*)A, na, (const K*)B, nb, Av, Bv, sa, sb, (K*)out_k,
+                     out_v);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+template <typename K, int MODE>
+static msv_status set_impl(Ctx& c, int kind, const void* A, uint32_t na, const void* B,
+                           uint32_t nb, void* out, uint32_t* out_len) {
+  const uint64_t total = (uint64_t)na + nb;
+  if (total == 0) {
+    hipLaunchKernelGGL(set_len_zero_kernel, dim3(1), dim3(1), 0, c.stream(), out_len);
+    MSV_HIP(hipGetLastError());
+    return MSV_OK;
+  }
+  const uint32_t ntiles = (uint32_t)((total + kMergeTile - 1) / kMergeTile);
+  uint32_t *sa, *sb;
+  MSV_TRY((partition<K, MODE, true>(c, (const K*)A, na, (const K*)B, nb, ntiles, &sa, &sb)));
+  uint64_t *counter, *status;
+  MSV_TRY(c.lookback(ntiles, &counter, &status));
+#define MSV_SET_LAUNCH(KIND)                                                                    \
+  hipLaunchKernelGGL((set_tile_kernel<K, MODE, KIND>), dim3(ntiles), dim3(kMergeBlock), 0,      \
+                     c.stream(), (const K*)A, na, (const K*)B, nb, sa, sb, (K*)out, out_len,    \
+                     counter, status, ntiles)
+  if (kind == SET_UNION) MSV_SET_LAUNCH(SET_UNION);
+  else if (kind == SET_INTERSECTION) MSV_SET_LAUNCH(SET_INTERSECTION);
+  else MSV_SET_LAUNCH(SET_DIFFERENCE);
+#undef MSV_SET_LAUNCH
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+template <typename K>
+static msv_status merge_by_mode(Ctx& c, int mode, size_t vs, const void* A, uint32_t na,
+                                const void* B, uint32_t nb, const void* Av, const void* Bv,
+                                void* ok, void* ov) {
+#define MSV_MERGE_VS(MODE)                                                               \
+  if (vs == 0) return merge_impl<K, MODE, 0>(c, A, na, B, nb, Av, Bv, ok, ov);           \
+  if (vs == 4) return merge_impl<K, MODE, 4>(c, A, na, B, nb, Av, Bv, ok, ov);           \
+  return merge_impl<K, MODE, 8>(c, A, na, B, nb, Av, Bv, ok, ov);
+  if (mode == ORD_GREATER) {
+    MSV_MERGE_VS(ORD_GREATER)
+  }
+  MSV_MERGE_VS(ORD_LESS)
+#undef MSV_MERGE_VS
+}
+
+template <typename K>
+static msv_status set_by_mode(Ctx& c, int mode, int kind, const void* A, uint32_t na,
+                              const void* B, uint32_t nb, void* out, uint32_t* out_len) {
+  if (mode == ORD_GREATER) return set_impl<K, ORD_GREATER>(c, kind, A, na, B, nb, out, out_len);
+  return set_impl<K, ORD_LESS>(c, kind, A, na, B, nb, out, out_len);
+}
+
+#define MSV_KEY_SWITCH(FN, ...)                                   \
+  switch (kdt) {                                                  \
+    case MSV_U32: return FN<uint32_t>(__VA_ARGS__);               \
+    case MSV_I32: return FN<int32_t>(__VA_ARGS__);                \
+    case MSV_F32: return FN<float>(__VA_ARGS__);                  \
+    case MSV_U64: return FN<uint64_t>(__VA_ARGS__);               \
+    case MSV_I64: return FN<int64_t>(__VA_ARGS__);                \
+    case MSV_F64: return FN<double>(__VA_ARGS__);                 \
+  }
+
+msv_status merge_run(Ctx& c, msv_dtype kdt, int mode, size_t vs, const void* A, uint32_t na,
+                     const void* B, uint32_t nb, const void* Av, const void* Bv, void* ok,
+                     void* ov) {
+  MSV_KEY_SWITCH(merge_by_mode, c, mode, vs, A, na, B, nb, Av, Bv, ok, ov)
+  set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad key dtype");
+  return MSV_ERR_INVALID_ARGUMENT;
+}
+
+msv_status set_run(Ctx& c, msv_dtype kdt, int mode, int kind, const void* A, uint32_t na,
+                   const void* B, uint32_t nb, void* out, uint32_t* out_len) {
+  MSV_KEY_SWITCH(set_by_mode, c, mode, kind, A, na, B, nb, out, out_len)
+  set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad key dtype");
+  return MSV_ERR_INVALID_ARGUMENT;
+}
+
+}  // namespace msv
+
+using namespace msv;
+
+static msv_status cmp_mode(msv_cmp cmp, int* mode) {
+  if (cmp == MSV_CMP_LESS) *mode = ORD_LESS;
+  else if (cmp == MSV_CMP_GREATER) *mode = ORD_GREATER;
+  else {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "unknown comparator %d", (int)cmp);
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  return MSV_OK;
+}
+
+static msv_status check_total(uint32_t a, uint32_t b) {
+  if ((uint64_t)a + b > 0xffffffffull) {
+    set_error(MSV_ERR_LENGTH_TOO_LARGE, (uint64_t)a + b, 0, "length does not fit in u32: %llu",
+              (unsigned long long)((uint64_t)a + b));
+    return MSV_ERR_LENGTH_TOO_LARGE;
+  }
+  return MSV_OK;
+}
+
+extern "C" {
+
+msv_status msv_merge(msv_executor* ex, msv_slice left, msv_slice right, msv_cmp cmp,
+                     msv_vec** out) {
+  clear_error();
+  int mode;
+  MSV_TRY(cmp_mode(cmp, &mode));
+  View a, b;
+  MSV_TRY(resolve(ex, left, &a));
+  MSV_TRY(resolve(ex, right, &b));
+  if (a.dtype != b.dtype) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "left and right dtypes differ");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  MSV_TRY(check_total(a.len, b.len));
+  MSV_TRY(new_vec(ex, a.dtype, a.len + b.len, false, out));
+  Ctx c(ex);
+  msv_status s = merge_run(c, a.dtype, mode, 0, a.ptr, a.len, b.ptr, b.len, nullptr, nullptr,
+                           (*out)->ptr, nullptr);
+  if (s != MSV_OK) {
+    msv_vec_release(*out);
+    *out = nullptr;
+  }
+  return s;
+}
+
+msv_status msv_merge_by_key(msv_executor* ex, msv_slice left_keys, msv_slice left_values,
+                            msv_slice right_keys, msv_slice right_values, msv_cmp cmp,
+                            msv_vec** out_keys, msv_vec** out_values) {
+  clear_error();
+  int mode;
+  MSV_TRY(cmp_mode(cmp, &mode));
+  View lk, lv, rk, rv;
+  MSV_TRY(resolve(ex, left_keys, &lk));
+  MSV_TRY(resolve(ex, left_values, &lv));
+  MSV_TRY(resolve(ex, right_keys, &rk));
+  MSV_TRY(resolve(ex, right_values, &rv));
+  if (lk.len != lv.len) {
+    set_error(MSV_ERR_LENGTH_MISMATCH, lk.len, lv.len, "input lengths differ: %u != %u", lk.len, lv.len);
+    return MSV_ERR_LENGTH_MISMATCH;
+  }
+  if (rk.len != rv.len) {
+    set_error(MSV_ERR_LENGTH_MISMATCH, rk.len, rv.len, "input lengths differ: %u != %u", rk.len, rv.len);
+    return MSV_ERR_LENGTH_MISMATCH;
+  }
+  if (lk.dtype != rk.dtype || lv.dtype != rv.dtype || !out_keys || !out_values) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "dtype mismatch or null output");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  MSV_TRY(check_total(lk.len, rk.len));
+  MSV_TRY(new_vec(ex, lk.dtype, lk.len + rk.len, false, out_keys));
+  msv_status s = new_vec(ex, lv.dtype, lv.len + rv.len, false, out_values);
+  if (s == MSV_OK) {
+    Ctx c(ex);
+    s = merge_run(c, lk.dtype, mode, dtype_size(lv.dtype), lk.ptr, lk.len, rk.ptr, rk.len, lv.ptr,
+                  rv.ptr, (*out_keys)->ptr, (*out_values)->ptr);
+  }
+  if (s != MSV_OK) {
+    msv_vec_release(*out_keys);
+    *out_keys = nullptr;
+    if (*out_values) msv_vec_release(*out_values);
+    *out_values = nullptr;
+  }
+  return s;
+}
+
+msv_status msv_set_op(msv_executor* ex, msv_set_kind kind, msv_slice left, msv_slice right,
+                      msv_cmp cmp, msv_vec** out) {
+  clear_error();
+  int mode;
+  MSV_TRY(cmp_mode(cmp, &mode));
+  if (kind < MSV_SET_UNION || kind > MSV_SET_DIFFERENCE) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "unknown set operation %d", (int)kind);
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  View a, b;
+  MSV_TRY(resolve(ex, left, &a));
+  MSV_TRY(resolve(ex, right, &b));
+  if (a.dtype != b.dtype) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "left and right dtypes differ");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  MSV_TRY(check_total(a.len, b.len));
+  const uint32_t cap = kind == MSV_SET_UNION ? a.len + b.len : a.len;
+  MSV_TRY(new_vec(ex, a.dtype, cap, true, out));
+  Ctx c(ex);
+  msv_status s = set_run(c, a.dtype, mode, (int)kind, a.ptr, a.len, b.ptr, b.len, (*out)->ptr,
+                         (*out)->d_len);
+  if (s != MSV_OK) {
+    msv_vec_release(*out);
+    *out = nullptr;
+  }
+  return s;
+}
+
+}  // extern "C"

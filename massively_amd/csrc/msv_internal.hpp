@@ -157,5 +157,13 @@ msv_status scan_run(Ctx& c, msv_dtype dt, msv_op op, bool exclusive, const void*
                     uint32_t n, const void* init_host);
 msv_status reduce_run(Ctx& c, msv_dtype dt, msv_op op, const void* in, uint32_t n,
                       const void* init_host, void* out_host);
+msv_status gather_run(Ctx& c, msv_dtype dt, const void* values, uint32_t vlen,
+                      const uint32_t* idx, void* out, uint32_t n);
+msv_status scatter_run(Ctx& c, msv_dtype dt, const void* values, const uint32_t* idx,
+                       uint32_t n, void* out, uint32_t olen);
+msv_status iota_run(Ctx& c, uint32_t* out, uint32_t n, uint32_t start);
+// mode: 0 total order, 1 Less, 2 Greater; vsize 0 (keys only), 4 or 8.
+msv_status radix_sort_run(Ctx& c, msv_dtype kdt, int mode, size_t vsize, const void* keys,
+                          const void* vals, uint32_t n, void* out_k, void* out_v);
 
 }  // namespace msv

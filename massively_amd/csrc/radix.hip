@@ -27,30 +27,12 @@
 
 namespace msv {
 
-enum OrderMode : int { ORD_TOTAL = 0, ORD_LESS = 1, ORD_GREATER = 2 };
-
 constexpr int kRadixBlock = 256;
 constexpr int kRadixIpt = 16;
 constexpr int kRadixWaves = kRadixBlock / kWave;
 constexpr uint32_t kRadixTile = kRadixBlock * kRadixIpt;
 constexpr int kRadixBins = 256;
 constexpr int kHistBlock = 256;
-
-template <typename K, int MODE>
-__device__ inline typename Bits<K>::U order_bits(K k) {
-  using U = typename Bits<K>::U;
-  constexpr U sign = (U)1 << (sizeof(U) * 8 - 1);
-  U e;
-  if constexpr (IsFloat<K>::value && MODE != ORD_TOTAL) {
-    U raw = bit_cast<U>(k);
-    if ((raw & (U)~sign) == 0) raw = 0;  // -0 == +0 under `<`
-    e = (raw & sign) ? (U)~raw : (U)(raw ^ sign);
-  } else {
-    e = radix_encode<K>(k);
-  }
-  if constexpr (MODE == ORD_GREATER) e = (U)~e;
-  return e;
-}
 
 struct RadixPass {
   uint32_t active;

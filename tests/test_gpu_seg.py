@@ -1,0 +1,173 @@
+"""GPU parity: seg:: scans, reduce and sort, and the Segmentation constructors.
+
+Integer results are bit-exact. f32/f64 segmented reduce/scan use fixed-shape
+passes: bitwise reproducible, and each segment's result lies within
+(2 * ceil(log2 L) + 40) * eps * sum|x| of the FP64 value of that segment
+(L = segment length), a bound covering the tile-local tree depth plus the
+chain of tile carries.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import massively_amd as m
+import oracle
+from massively_amd import seg
+from tests.conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(23)
+BOUNDARY_LENGTHS = [0, 1, 2, 31, 32, 33, 127, 128, 129, 255, 256, 257]
+
+
+def offsets_of(lengths):
+    return np.concatenate([[0], np.cumsum(np.asarray(lengths, np.int64))]).astype(np.uint32)
+
+
+def lognormal_lengths(s, median=32, sigma=1.1774, cap=1 << 16, empty=0.02):
+    L = np.minimum(np.round(RNG.lognormal(math.log(median), sigma, s)), cap).astype(np.int64)
+    L[RNG.random(s) < empty] = 0
+    return L
+
+
+def cases():
+    yield [3, 2]
+    yield BOUNDARY_LENGTHS
+    yield [0, 0, 5, 0, 4096, 0, 1, 4095, 4097, 0, 0]
+    yield [0]
+    yield [0, 0, 0]
+    yield list(RNG.integers(0, 6, 3000))
+    yield list(lognormal_lengths(4000))
+    yield [70001]
+    yield [1] * 9000
+    yield [20000, 0, 30000, 1, 2, 3]
+
+
+def test_golden(exec_):
+    v = golden("seg_owned_results")
+    vals = exec_.to_device(np.array(v["values"], np.uint32))
+    offs = exec_.to_device(np.array(v["offsets"], np.uint32))
+    r = seg.ForEachSegment(seg.Reduce(m.op.Add, 0)).run(exec_, seg.SegmentIterator(vals, offs))
+    assert exec_.to_host(r).tolist() == v["expected"]
+    v = golden("seg_exclusive_from_reduce")
+    vals = exec_.to_device(np.array(v["values"], np.uint32))
+    offs = exec_.to_device(np.array(v["offsets"], np.uint32))
+    it = seg.ForEachSegment(seg.ExclusiveScan(m.op.Add, 0)).run(exec_, seg.SegmentIterator(vals, offs))
+    assert exec_.to_host(it.values).tolist() == v["expected"]
+    r = seg.ForEachSegment(seg.Reduce(m.op.Add, 0)).run(exec_, seg.SegmentIterator(vals, offs))
+    assert exec_.to_host(r).tolist() == v["expected_reduce"]
+    v = golden("segmentation_from_lengths")
+    sg = seg.Segmentation.from_lengths(exec_, exec_.to_device(np.array(v["lengths"], np.uint32)))
+    assert exec_.to_host(sg.offsets()).tolist() == v["expected_offsets"]
+    assert sg.value_count() == 6 and sg.segment_count() == 3
+
+
+def test_segmentation_validation(exec_):
+    with pytest.raises(m.InvalidSegmentation):
+        seg.Segmentation.from_offsets(exec_, exec_.to_device(np.array([1, 2], np.uint32)))
+    with pytest.raises(m.InvalidSegmentation):
+        seg.Segmentation.from_offsets(exec_, exec_.to_device(np.array([0, 5, 3], np.uint32)))
+    with pytest.raises(m.InvalidSegmentation):
+        seg.Segmentation.from_offsets(exec_, exec_.to_device(np.zeros(0, np.uint32)))
+    with pytest.raises(m.LengthTooLarge):
+        seg.Segmentation.from_lengths(exec_, exec_.to_device(np.array([2**31, 2**31], np.uint32)))
+    sg = seg.Segmentation.from_lengths(exec_, exec_.to_device(np.zeros(0, np.uint32)))
+    assert sg.segment_count() == 0 and sg.value_count() == 0
+    sg = seg.Segmentation.from_offsets(exec_, exec_.to_device(np.array([0, 3, 3, 7], np.uint32)))
+    assert sg.value_count() == 7 and sg.segment_count() == 3
+    with pytest.raises(m.LengthMismatch):
+        seg.ForEachSegment(seg.Reduce(m.op.Add, 0)).run(
+            exec_, seg.SegmentIterator(exec_.to_device(np.zeros(6, np.uint32)), sg))
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.int32, np.uint64, np.int64])
+def test_int_seg_scans_and_reduce(exec_, dtype):
+    for lengths in cases():
+        off = offsets_of(lengths)
+        n = int(off[-1])
+        info = np.iinfo(dtype)
+        x = RNG.integers(info.min, info.max, n, dtype=dtype, endpoint=True)
+        sg = seg.Segmentation.from_offsets(exec_, exec_.to_device(off))
+        dx = exec_.to_device(x)
+        for name, op in (("add", m.op.Add), ("min", m.op.Min), ("max", m.op.Max), ("xor", m.op.Xor)):
+            y = seg.ForEachSegment(seg.InclusiveScan(op)).run(exec_, seg.SegmentIterator(dx, sg))
+            assert np.array_equal(exec_.to_host(y.values), oracle.seg_inclusive_scan(x, off, name)), (len(lengths), name)
+            init = dtype(7)
+            y = seg.ForEachSegment(seg.ExclusiveScan(op, init)).run(exec_, seg.SegmentIterator(dx, sg))
+            assert np.array_equal(exec_.to_host(y.values), oracle.seg_exclusive_scan(x, off, init, name)), (len(lengths), name)
+            r = seg.ForEachSegment(seg.Reduce(op, init)).run(exec_, seg.SegmentIterator(dx, sg))
+            assert np.array_equal(exec_.to_host(r), oracle.seg_reduce(x, off, init, name)), (len(lengths), name)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_float_seg_reduce_and_scan(exec_, dtype):
+    eps = 2.0**-24 if dtype == np.float32 else 2.0**-53
+    for lengths in cases():
+        off = offsets_of(lengths)
+        n = int(off[-1])
+        x = RNG.standard_normal(n).astype(dtype)
+        sg = seg.Segmentation.from_offsets(exec_, exec_.to_device(off))
+        dx = exec_.to_device(x)
+        r1 = exec_.to_host(seg.ForEachSegment(seg.Reduce(m.op.Add, 0.0)).run(exec_, seg.SegmentIterator(dx, sg)))
+        r2 = exec_.to_host(seg.ForEachSegment(seg.Reduce(m.op.Add, 0.0)).run(exec_, seg.SegmentIterator(dx, sg)))
+        assert r1.tobytes() == r2.tobytes()
+        x64 = x.astype(np.float64)
+        for s in range(len(lengths)):
+            a, b = off[s], off[s + 1]
+            L = int(b - a)
+            exact = float(np.sum(x64[a:b]))
+            bound = (2 * math.ceil(math.log2(max(L, 2))) + 40) * eps * float(np.sum(np.abs(x64[a:b])))
+            assert abs(float(r1[s]) - exact) <= bound, (s, L)
+        for name, op in (("min", m.op.Min), ("max", m.op.Max)):
+            y = seg.ForEachSegment(seg.InclusiveScan(op)).run(exec_, seg.SegmentIterator(dx, sg))
+            assert np.array_equal(exec_.to_host(y.values), oracle.seg_inclusive_scan(x, off, name))
+            r = exec_.to_host(seg.ForEachSegment(seg.Reduce(op, 0.0)).run(exec_, seg.SegmentIterator(dx, sg)))
+            assert np.array_equal(r, oracle.seg_reduce(x, off, 0.0, name))
+
+
+@pytest.mark.parametrize("kdtype", [np.uint32, np.float32, np.int64])
+def test_seg_sort(exec_, kdtype):
+    for lengths in cases():
+        off = offsets_of(lengths)
+        n = int(off[-1])
+        if np.dtype(kdtype).kind == "f":
+            keys = np.round(RNG.standard_normal(n) * 3).astype(kdtype)
+        else:
+            keys = RNG.integers(0, 9, n).astype(kdtype)
+        vals = RNG.standard_normal(n).astype(np.float32)
+        sg = seg.Segmentation.from_offsets(exec_, exec_.to_device(off))
+        for mode, cmp in (("less", m.op.Less), ("greater", m.op.Greater)):
+            out = seg.ForEachSegment(seg.Sort(cmp)).run(
+                exec_, seg.SegmentIterator((exec_.to_device(keys), exec_.to_device(vals)), sg))
+            k, v = out.values
+            perm = oracle.seg_sort_perm(keys, off, mode)
+            assert exec_.to_host(k).tobytes() == keys[perm].tobytes(), (len(lengths), mode)
+            assert exec_.to_host(v).tobytes() == vals[perm].tobytes(), (len(lengths), mode)
+
+
+def test_batched_seg_config_2p26(exec_):
+    """2^26 elements in 2^20 log-normal segments (median 32, cap 2^16, 2% empty):
+    u32 scan and f32 reduce vs the oracle."""
+    S = 1 << 20
+    L = lognormal_lengths(S)
+    total = int(L.sum())
+    target = 1 << 26
+    L = np.floor(L * (target / total)).astype(np.int64)
+    L[-1] += target - int(L.sum())
+    off = offsets_of(L)
+    n = int(off[-1])
+    assert n == target
+    x = RNG.integers(0, 2**32, n, dtype=np.uint32)
+    sg = seg.Segmentation.from_offsets(exec_, exec_.to_device(off))
+    y = seg.ForEachSegment(seg.InclusiveScan(m.op.Add)).run(exec_, seg.SegmentIterator(exec_.to_device(x), sg))
+    assert np.array_equal(exec_.to_host(y.values), oracle.seg_inclusive_scan(x, off, "add"))
+    f = RNG.standard_normal(n).astype(np.float32)
+    r = exec_.to_host(seg.ForEachSegment(seg.Reduce(m.op.Add, 0.0)).run(exec_, seg.SegmentIterator(exec_.to_device(f), sg)))
+    exact = np.add.reduceat(f.astype(np.float64), off[:-1].astype(np.int64)) if n else np.zeros(S)
+    exact[L == 0] = 0.0
+    absum = np.add.reduceat(np.abs(f.astype(np.float64)), off[:-1].astype(np.int64))
+    absum[L == 0] = 0.0
+    bound = (2 * np.ceil(np.log2(np.maximum(L, 2))) + 40) * 2.0**-24 * absum
+    assert np.all(np.abs(r.astype(np.float64) - exact) <= bound)
