@@ -251,35 +251,35 @@ __device__ inline uint64_t add_agent(uint64_t* p, uint64_t v) {
 // ---------------------------------------------------------------------------
 // Decoupled lookback.
 //
-// A launch takes tile ids from a 64-bit counter whose high half is the launch
-// generation. The block that draws the last id (ntiles-1) has seen every other
-// block draw first, so it rewinds the low half and bumps the generation with a
-// single atomic add; the counter is never cleared by the host and stays valid
-// under stream-ordered replay.
+// Tiles are numbered by blockIdx.x. Every launch carries a generation number
+// chosen by the host (29 bits, never reused before the status array is
+// cleared), so status words left by earlier launches are simply invalid and
+// the array never needs clearing between launches.
 //
-// Status word (64 bit): [63:35] generation (29 bits) | [34] head bit |
-// [33:32] flag (1 = aggregate, 2 = inclusive prefix) | [31:0] payload.
-// A word is valid for a launch iff its flag is nonzero and its generation
-// matches; stale words of earlier launches therefore never need clearing.
-// 64-bit payloads use two words per tile (low, high half) that each carry the
-// flag; a reader accepts them only when both are valid with equal flags.
+// Status word (64 bit): [63:35] generation | [34] head bit | [33:32] flag
+// (1 = aggregate, 2 = inclusive prefix) | [31:0] payload. 64-bit payloads use
+// two words per tile (low, high half) that each carry the flag; a reader
+// accepts them only when both are valid with equal flags. All status traffic
+// is relaxed agent-scope atomics, coherent across the eight XCDs, so no
+// release/acquire fence (and no L2 write-back) is needed.
+//
+// Forward progress never depends on the order in which workgroups are
+// dispatched: a wave that has polled a predecessor for kSelfHelpPolls rounds
+// without seeing its status computes that tile's aggregate itself from the
+// input (self-help) and continues.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kFlagAgg = 1;
 constexpr uint32_t kFlagInc = 2;
 constexpr uint32_t kGenMask = (1u << 29) - 1;
+constexpr uint32_t kSelfHelpPolls = 1024;
 
-struct TileTicket {
-  uint32_t tile;
-  uint32_t gen;
-};
-
-__device__ inline TileTicket draw_tile(uint64_t* counter, uint32_t ntiles) {
-  const uint64_t old = add_agent(counter, 1ull);
-  TileTicket t;
-  t.tile = (uint32_t)old;
-  t.gen = (uint32_t)(old >> 32) & kGenMask;
-  if (t.tile == ntiles - 1) add_agent(counter, (1ull << 32) - (uint64_t)ntiles);
-  return t;
+// Tile ids drawn from an atomic counter (kernels whose fallback would need a
+// whole block keep this form). The block that draws the last id (ntiles-1)
+// has seen every other block draw first, so it rewinds the counter to zero.
+__device__ inline uint32_t draw_tile(uint64_t* counter, uint32_t ntiles) {
+  const uint32_t tile = (uint32_t)add_agent(counter, 1ull);
+  if (tile == ntiles - 1) add_agent(counter, (uint64_t)0 - (uint64_t)ntiles);
+  return tile;
 }
 
 __device__ inline uint64_t pack_status(uint32_t gen, uint32_t flag, uint32_t head,
@@ -306,6 +306,23 @@ struct TileStatus {
       const uint64_t u = bit_cast<uint64_t>(v);
       st_agent(words + 2 * (size_t)tile, pack_status(gen, flag, head, (uint32_t)u));
       st_agent(words + 2 * (size_t)tile + 1, pack_status(gen, flag, head, (uint32_t)(u >> 32)));
+    }
+  }
+  // One poll: returns 0 if the status of `tile` is not yet valid.
+  __device__ inline uint32_t try_read(uint32_t tile, uint32_t& head, T& v) const {
+    if constexpr (kWords == 1) {
+      const uint64_t w = ld_agent(words + tile);
+      const uint32_t f = status_flag(w, gen);
+      head = (uint32_t)(w >> 34) & 1u;
+      v = bit_cast<T>((uint32_t)w);
+      return f;
+    } else {
+      const uint64_t lo = ld_agent(words + 2 * (size_t)tile);
+      const uint64_t hi = ld_agent(words + 2 * (size_t)tile + 1);
+      const uint32_t fl = status_flag(lo, gen), fh = status_flag(hi, gen);
+      head = (uint32_t)(lo >> 34) & 1u;
+      v = bit_cast<T>(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+      return (fl != 0 && fl == fh) ? fl : 0u;
     }
   }
   // Spins until the status of `tile` is valid; returns flag and payload.
@@ -357,6 +374,48 @@ __device__ inline T wave_lookback(const TileStatus<T>& st, uint32_t tile) {
       // commutative, so the order of combination inside the window is free.
       prefix = C::apply(wave_reduce<T, OP>(contrib), prefix);
       return prefix;
+    }
+    prefix = C::apply(wave_reduce<T, OP>(v), prefix);
+    base -= kWave;
+  }
+}
+
+// Lookback with self-help: `agg_of(q)` is called by the whole wave with a
+// uniform tile index q and returns that tile's aggregate (uniform). It is used
+// only for predecessors whose status stays invalid for kSelfHelpPolls polls.
+template <typename T, int OP, typename AggFn>
+__device__ inline T wave_lookback_sh(const TileStatus<T>& st, uint32_t tile, AggFn agg_of) {
+  using C = Combine<T, OP>;
+  const uint32_t lane = __lane_id();
+  T prefix = C::identity();
+  int64_t base = (int64_t)tile - 1;
+  for (;;) {
+    const int64_t p = base - (int64_t)lane;
+    T v = C::identity();
+    uint32_t head = 0;
+    uint32_t flag = p >= 0 ? 0u : kFlagInc;
+    for (uint32_t polls = 0;; ++polls) {
+      if (flag == 0) flag = st.try_read((uint32_t)p, head, v);
+      const uint64_t pending = ballot(flag == 0);
+      if (!pending) break;
+      if (polls >= kSelfHelpPolls) {
+        for (uint64_t m = pending; m; m &= m - 1) {
+          const uint32_t k = (uint32_t)__builtin_ctzll(m);
+          const T a = agg_of((uint32_t)(base - (int64_t)k));
+          if (lane == k) {
+            v = a;
+            flag = kFlagAgg;
+          }
+        }
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    const uint64_t inc_mask = ballot(flag == kFlagInc);
+    if (inc_mask) {
+      const uint32_t first = (uint32_t)__builtin_ctzll(inc_mask);
+      const T contrib = lane <= first ? v : C::identity();
+      return C::apply(wave_reduce<T, OP>(contrib), prefix);
     }
     prefix = C::apply(wave_reduce<T, OP>(v), prefix);
     base -= kWave;

@@ -212,18 +212,19 @@ template <typename K, int MODE, int KIND>
 __global__ __launch_bounds__(kMergeBlock) void set_tile_kernel(
     const K* __restrict__ A, uint32_t na, const K* __restrict__ B, uint32_t nb,
     const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb, K* __restrict__ out,
-    uint32_t* __restrict__ out_len, uint64_t* counter, uint64_t* status, uint32_t ntiles) {
+    uint32_t* __restrict__ out_len, uint64_t* counter, uint64_t* status, uint32_t gen,
+    uint32_t ntiles) {
   constexpr int NW = kMergeBlock / kWave;
   constexpr uint32_t CAP = kMergeTile + 2;
   __shared__ K s_keys[CAP];
   __shared__ uint32_t s_split[kMergeBlock + 1][2];
   __shared__ uint32_t s_wave[NW];
   __shared__ uint32_t s_prefix;
-  __shared__ TileTicket s_ticket;
+  __shared__ uint32_t s_tile;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
-  if (tid == 0) s_ticket = draw_tile(counter, ntiles);
+  if (tid == 0) s_tile = draw_tile(counter, ntiles);
   __syncthreads();
-  const uint32_t b = s_ticket.tile, gen = s_ticket.gen;
+  const uint32_t b = s_tile;
   const uint32_t i0 = sa[b], i1 = sa[b + 1], j0 = sb[b], j1 = sb[b + 1];
   const uint32_t la = i1 - i0, lb = j1 - j0, L = la + lb;
   for (uint32_t k = tid; k < L; k += kMergeBlock) s_keys[k] = k < la ? A[i0 + k] : B[j0 + k - la];
@@ -349,11 +350,12 @@ static msv_status set_impl(Ctx& c, int kind, const void* A, uint32_t na, const v
   uint32_t *sa, *sb;
   MSV_TRY((partition<K, MODE, true>(c, (const K*)A, na, (const K*)B, nb, ntiles, &sa, &sb)));
   uint64_t *counter, *status;
-  MSV_TRY(c.lookback(ntiles, &counter, &status));
+  uint32_t gen;
+  MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));
 #define MSV_SET_LAUNCH(KIND)                                                                    \
   hipLaunchKernelGGL((set_tile_kernel<K, MODE, KIND>), dim3(ntiles), dim3(kMergeBlock), 0,      \
                      c.stream(), (const K*)A, na, (const K*)B, nb, sa, sb, (K*)out, out_len,    \
-                     counter, status, ntiles)
+                     counter, status, gen, ntiles)
   if (kind == SET_UNION) MSV_SET_LAUNCH(SET_UNION);
   else if (kind == SET_INTERSECTION) MSV_SET_LAUNCH(SET_INTERSECTION);
   else MSV_SET_LAUNCH(SET_DIFFERENCE);

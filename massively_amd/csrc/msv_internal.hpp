@@ -83,13 +83,13 @@ struct msv_executor {
   std::atomic<int> refs;
   bool destroyed;
   msv::Allocator allocator;
-  // Decoupled-lookback state: one tile counter (tile id | generation << 32)
-  // and a growable status-word array, shared by every lookback kernel on this
-  // executor's stream.
+  // Decoupled-lookback state shared by every lookback kernel on this
+  // executor's stream: a tile counter (for kernels that draw tile ids), a
+  // growable status-word array, and the generation of the next launch.
   uint64_t* d_counter;
   uint64_t* d_status;
   size_t status_words;
-  uint64_t lookback_launches;
+  uint32_t lookback_gen;
   explicit msv_executor(int dev) : allocator(dev) {}
 };
 
@@ -143,8 +143,9 @@ class Ctx {
   msv_executor* ex() const { return ex_; }
   // nullptr + error on failure.
   void* scratch(size_t bytes);
-  // Lookback state sized for `words` status words; counts one launch.
-  msv_status lookback(size_t words, uint64_t** counter, uint64_t** status);
+  // Lookback state sized for `words` status words, and a fresh generation
+  // for one launch.
+  msv_status lookback(size_t words, uint64_t** counter, uint64_t** status, uint32_t* gen);
 
  private:
   msv_executor* ex_;

@@ -15,12 +15,13 @@
 //     a digit position whose histogram has a single bin of count n is
 //     skipped; the ping-pong source/destination of every remaining pass is
 //     written into a device-side table, so no host round trip is needed;
-//  3. one onesweep kernel per digit: a 256-thread block owns a tile of 4096
-//     keys (each wave a contiguous 1024-key run, lane l holding elements
-//     j*64+l), ranks keys stably inside each wave by matching 8-bit digits
-//     with ballots (wave-level multi-split), combines the wave counts into
-//     tile digit offsets, resolves each digit's global offset by decoupled
-//     lookback over (tile, digit) status words, reorders keys (then values)
+//  3. one onesweep kernel per digit: block b owns tile b of 4096 keys (each
+//     wave a contiguous 1024-key run, lane l holding elements j*64+l), ranks
+//     keys stably inside each wave by matching 8-bit digits with ballots
+//     (wave-level multi-split), combines the wave counts into tile digit
+//     offsets, resolves each digit's global offset by decoupled lookback over
+//     (tile, digit) status words (recounting a predecessor's digits from its
+//     keys if its status does not appear), reorders keys (then values)
 //     through LDS into digit-sorted runs and writes the runs out.
 #include "device_common.hpp"
 #include "msv_internal.hpp"
@@ -142,7 +143,7 @@ struct ValueType<8> { using T = uint64_t; };
 template <typename K, int MODE, int VS>
 __global__ __launch_bounds__(kRadixBlock) void radix_pass_kernel(
     const RadixPass* __restrict__ table, int pass, const uint32_t* __restrict__ g_start, uint32_t n,
-    uint64_t* counter, uint64_t* status, uint32_t ntiles) {
+    uint64_t* status, uint32_t gen, uint32_t ntiles) {
   using U = typename Bits<K>::U;
   using VT = typename ValueType<VS>::T;
   constexpr int SLOT = (VS > (int)sizeof(K)) ? VS : (int)sizeof(K);
@@ -151,16 +152,15 @@ __global__ __launch_bounds__(kRadixBlock) void radix_pass_kernel(
   __shared__ uint32_t s_tile_start[kRadixBins];         // in-tile start of each digit
   __shared__ uint32_t s_goff[kRadixBins];               // global position - in-tile position
   __shared__ uint32_t s_wsum[kRadixBins / kWave];
-  __shared__ TileTicket s_ticket;
+  __shared__ uint32_t s_help[kRadixBins];               // self-help histogram
 
   const RadixPass rp = table[pass];
   if (!rp.active) return;
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
-  if (tid == 0) s_ticket = draw_tile(counter, ntiles);
   for (uint32_t i = tid; i < kRadixWaves * kRadixBins; i += kRadixBlock) (&s_hist[0][0])[i] = 0;
   __syncthreads();
-  const uint32_t tile = s_ticket.tile, gen = s_ticket.gen;
+  const uint32_t tile = blockIdx.x;
   const size_t base = (size_t)tile * kRadixTile + (size_t)wid * (kWave * kRadixIpt);
   const K* __restrict__ src_k = (const K*)rp.src_k;
   const uint32_t shift = rp.shift;
@@ -220,15 +220,43 @@ __global__ __launch_bounds__(kRadixBlock) void radix_pass_kernel(
   const uint32_t tstart = woff + incl - cnt;
   s_tile_start[d_me] = tstart;
 
-  // Decoupled lookback for this digit: sum of the digit's counts in all
-  // earlier tiles.
+  // Decoupled lookback, one digit per thread, stepping the whole block through
+  // the predecessor tiles together: at tile p every digit not yet resolved
+  // adds p's count and stops at an inclusive prefix. If p's status does not
+  // appear within kSelfHelpPolls polls, the block counts p's digits itself.
   uint32_t excl = 0;
   if (tile > 0) {
-    for (int64_t p = (int64_t)tile - 1; p >= 0; --p) {
-      uint32_t head, v;
-      const uint32_t f = st.wait((uint32_t)p * kRadixBins + d_me, head, v);
-      excl += v;
-      if (f == kFlagInc) break;
+    bool done = false;
+    uint32_t p = tile - 1;
+    uint32_t polls = 0;
+    for (;;) {
+      uint32_t f = 0, v = 0, head;
+      if (!done) f = st.try_read(p * kRadixBins + d_me, head, v);
+      const bool waiting = !done && f == 0;
+      if (__syncthreads_or(waiting)) {
+        if (++polls < kSelfHelpPolls) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        s_help[d_me] = 0;
+        __syncthreads();
+        for (int j = 0; j < kRadixIpt; ++j) {
+          const size_t i = (size_t)p * kRadixTile + (size_t)j * kRadixBlock + tid;
+          if (i < n) atomicAdd(&s_help[(uint32_t)(order_bits<K, MODE>(src_k[i]) >> shift) & 0xffu], 1u);
+        }
+        __syncthreads();
+        if (waiting) {
+          f = kFlagAgg;
+          v = s_help[d_me];
+        }
+      }
+      polls = 0;
+      if (!done) {
+        excl += v;
+        if (f == kFlagInc || p == 0) done = true;
+      }
+      if (!__syncthreads_or(!done)) break;
+      --p;
     }
     st.publish(sidx, kFlagInc, 0, excl + cnt);
   }
@@ -310,9 +338,10 @@ static msv_status radix_sort_impl(Ctx& c, const void* keys, const void* vals, ui
   const uint32_t ntiles = (n + kRadixTile - 1) / kRadixTile;
   for (int p = 0; p < P; ++p) {
     uint64_t *counter, *status;
-    MSV_TRY(c.lookback((size_t)ntiles * kRadixBins, &counter, &status));
+    uint32_t gen;
+    MSV_TRY(c.lookback((size_t)ntiles * kRadixBins, &counter, &status, &gen));
     hipLaunchKernelGGL((radix_pass_kernel<K, MODE, VS>), dim3(ntiles), dim3(kRadixBlock), 0,
-                       c.stream(), table, p, start, n, counter, status, ntiles);
+                       c.stream(), table, p, start, n, status, gen, ntiles);
     MSV_HIP(hipGetLastError());
   }
   return MSV_OK;

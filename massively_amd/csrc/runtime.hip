@@ -191,14 +191,13 @@ void* Ctx::scratch(size_t bytes) {
   return p;
 }
 
-msv_status Ctx::lookback(size_t words, uint64_t** counter, uint64_t** status) {
+msv_status Ctx::lookback(size_t words, uint64_t** counter, uint64_t** status, uint32_t* gen) {
   msv_executor* ex = ex_;
-  // Generations are 29 bits wide; before they could wrap, clear every status
-  // word so no stale word can ever match a live generation.
-  if (ex->lookback_launches >= (1ull << 29) - 2) {
-    MSV_HIP(hipMemsetAsync(ex->d_status, 0, ex->status_words * 8, ex->stream));
-    MSV_HIP(hipMemsetAsync(ex->d_counter, 0, 8, ex->stream));
-    ex->lookback_launches = 0;
+  // Generations are 29 bits; before one would repeat, every status word is
+  // cleared so no stale word can match a live generation.
+  if (ex->lookback_gen >= (1u << 29) - 1) {
+    if (ex->d_status) MSV_HIP(hipMemsetAsync(ex->d_status, 0, ex->status_words * 8, ex->stream));
+    ex->lookback_gen = 1;
   }
   if (words > ex->status_words) {
     size_t grow = ex->status_words ? ex->status_words : (1u << 16);
@@ -220,9 +219,9 @@ msv_status Ctx::lookback(size_t words, uint64_t** counter, uint64_t** status) {
     ex->d_status = (uint64_t*)p;
     ex->status_words = grow;
   }
-  ex->lookback_launches++;
   *counter = ex->d_counter;
   *status = ex->d_status;
+  *gen = ex->lookback_gen++;
   return MSV_OK;
 }
 
@@ -264,7 +263,7 @@ msv_status msv_executor_create(int32_t device, msv_executor** out) {
   ex->d_counter = nullptr;
   ex->d_status = nullptr;
   ex->status_words = 0;
-  ex->lookback_launches = 0;
+  ex->lookback_gen = 1;
   hipError_t e = hipStreamCreateWithFlags(&ex->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc((void**)&ex->d_counter, 8);
   if (e == hipSuccess) e = hipMemsetAsync(ex->d_counter, 0, 8, ex->stream);

@@ -1,13 +1,16 @@
 // vector::inclusive_scan / exclusive_scan / reduce for gfx950.
 //
-// Integer scans: one pass, decoupled lookback. A 256-thread block owns a
-// tile of BLOCK*ROWS 16-byte vectors (4096 u32). Each thread holds ROWS
-// vectors loaded row-striped (row r of thread t starts at (r*BLOCK+t)*V), so
-// every load and store instruction moves 1 KiB contiguous per wave. The scan
-// order inside the tile is row-major: a thread scans its vector, the wave
-// scans the vector totals with DPP, and the ROWS x NW wave totals are folded
-// from LDS. Wave 0 publishes the tile aggregate, looks back over up to 64
-// predecessor tiles per step, then publishes the inclusive prefix.
+// Integer scans: one pass, decoupled lookback. A 256-thread block owns tile
+// blockIdx.x of BLOCK*ROWS 16-byte vectors (64 KiB for inputs of 2^22 or more
+// elements, 16 KiB below). Each thread holds ROWS vectors loaded row-striped
+// (row r of thread t starts at (r*BLOCK+t)*V), so every load and store
+// instruction moves 1 KiB contiguous per wave, non-temporally. The scan order
+// inside the tile is row-major: a thread scans its vector, the wave scans the
+// vector totals with DPP, and the ROWS x NW wave totals are folded from LDS.
+// Wave 0 publishes the tile aggregate, looks back over up to 64 predecessor
+// tiles per step (recomputing a predecessor's aggregate from the input if its
+// status does not appear), then publishes the inclusive prefix. Tile ids are
+// block indices: no atomic counter serialises the launch.
 //
 // Float scans and all reductions: fixed-shape trees (tile partials, then an
 // exclusive scan of partials, then the tile scan with its prefix). The
@@ -18,219 +21,10 @@
 // Reference behaviour: massively/src/core/scan.rs:1125-1234 (inclusive,
 // exclusive with init placed before the first element), core/reduce.rs:992-1009
 // (reduce returns op(init, total)).
-#include "device_common.hpp"
 #include "msv_internal.hpp"
+#include "scan_kernels.hpp"
 
 namespace msv {
-
-constexpr int kScanBlock = 256;
-constexpr int kScanRows = 4;
-constexpr int kReduceRows = 16;
-
-template <typename T>
-struct alignas(16) Vec16 {
-  static constexpr int N = 16 / sizeof(T);
-  T v[N];
-};
-
-template <typename T, int BLOCK, int ROWS>
-__device__ inline void load_tile(const T* __restrict__ in, size_t base, uint32_t n, bool vec_ok,
-                                 T (&x)[ROWS][16 / sizeof(T)], T fill) {
-  constexpr int V = 16 / sizeof(T);
-  const uint32_t tid = threadIdx.x;
-  if (vec_ok) {
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-      const Vec16<T> q =
-          *reinterpret_cast<const Vec16<T>*>(in + base + ((size_t)r * BLOCK + tid) * V);
-#pragma unroll
-      for (int k = 0; k < V; ++k) x[r][k] = q.v[k];
-    }
-  } else {
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-#pragma unroll
-      for (int k = 0; k < V; ++k) {
-        const size_t i = base + ((size_t)r * BLOCK + tid) * V + k;
-        x[r][k] = (i < n) ? in[i] : fill;
-      }
-    }
-  }
-}
-
-template <typename T, int BLOCK, int ROWS>
-__device__ inline void store_tile(T* __restrict__ out, size_t base, uint32_t n, bool vec_ok,
-                                  const T (&y)[ROWS][16 / sizeof(T)]) {
-  constexpr int V = 16 / sizeof(T);
-  const uint32_t tid = threadIdx.x;
-  if (vec_ok) {
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-      Vec16<T> q;
-#pragma unroll
-      for (int k = 0; k < V; ++k) q.v[k] = y[r][k];
-      *reinterpret_cast<Vec16<T>*>(out + base + ((size_t)r * BLOCK + tid) * V) = q;
-    }
-  } else {
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-#pragma unroll
-      for (int k = 0; k < V; ++k) {
-        const size_t i = base + ((size_t)r * BLOCK + tid) * V + k;
-        if (i < n) out[i] = y[r][k];
-      }
-    }
-  }
-}
-
-enum ScanMode : int { SCAN_LOOKBACK = 0, SCAN_PREFIXED = 1 };
-
-// One tile per block. SCAN_LOOKBACK draws tile ids from `counter` and
-// resolves the tile prefix by decoupled lookback; SCAN_PREFIXED uses
-// blockIdx.x and `prefixes[tile]` (or the identity when prefixes is null).
-template <typename T, int OP, bool EXCL, int MODE, int BLOCK, int ROWS>
-__global__ __launch_bounds__(BLOCK) void scan_tile_kernel(const T* __restrict__ in,
-                                                          T* __restrict__ out, uint32_t n, T init,
-                                                          bool aligned, uint64_t* counter,
-                                                          uint64_t* status, uint32_t ntiles,
-                                                          const T* __restrict__ prefixes) {
-  using C = Combine<T, OP>;
-  constexpr int V = 16 / sizeof(T);
-  constexpr int NW = BLOCK / kWave;
-  constexpr uint32_t TILE = (uint32_t)BLOCK * ROWS * V;
-  __shared__ T s_wave[ROWS][NW];
-  __shared__ T s_prefix;
-  __shared__ TileTicket s_ticket;
-
-  const uint32_t tid = threadIdx.x;
-  const uint32_t lane = tid & (kWave - 1);
-  const uint32_t wid = tid / kWave;
-
-  uint32_t tile;
-  uint32_t gen = 0;
-  if constexpr (MODE == SCAN_LOOKBACK) {
-    if (tid == 0) s_ticket = draw_tile(counter, ntiles);
-    __syncthreads();
-    tile = s_ticket.tile;
-    gen = s_ticket.gen;
-  } else {
-    tile = blockIdx.x;
-  }
-  const size_t base = (size_t)tile * TILE;
-  const bool vec_ok = aligned && (base + TILE <= n);
-  const T id = C::identity();
-
-  T x[ROWS][V];
-  load_tile<T, BLOCK, ROWS>(in, base, n, vec_ok, x, id);
-
-  // Vector-local inclusive scans and wave scans of the vector totals.
-  T ws[ROWS];
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) {
-#pragma unroll
-    for (int k = 1; k < V; ++k) x[r][k] = C::apply(x[r][k - 1], x[r][k]);
-    ws[r] = wave_inclusive_scan<T, OP>(x[r][V - 1]);
-    if (lane == kWave - 1) s_wave[r][wid] = ws[r];
-  }
-  __syncthreads();
-
-  // Exclusive prefix of (row r, wave wid) inside the tile; `agg` = tile total.
-  T pre[ROWS];
-  T agg = id;
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) {
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      if (w == (int)wid) pre[r] = agg;
-      agg = C::apply(agg, s_wave[r][w]);
-    }
-  }
-
-  if constexpr (MODE == SCAN_LOOKBACK) {
-    if (wid == 0) {
-      TileStatus<T> st{status, gen};
-      T excl = id;
-      if (tile == 0) {
-        if (lane == 0) st.publish(0, kFlagInc, 0, agg);
-      } else {
-        if (lane == 0) st.publish(tile, kFlagAgg, 0, agg);
-        excl = wave_lookback<T, OP>(st, tile);
-        if (lane == 0) st.publish(tile, kFlagInc, 0, C::apply(excl, agg));
-      }
-      if (lane == 0) s_prefix = excl;
-    }
-  } else {
-    if (tid == 0) s_prefix = prefixes ? prefixes[tile] : id;
-  }
-  __syncthreads();
-
-  T tp = s_prefix;
-  if constexpr (EXCL) tp = C::apply(init, tp);
-
-  T y[ROWS][V];
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) {
-    const T wex = shift_up1<T>(ws[r], id);
-    const T p = C::apply(C::apply(tp, pre[r]), wex);
-    if constexpr (EXCL) {
-      y[r][0] = p;
-#pragma unroll
-      for (int k = 1; k < V; ++k) y[r][k] = C::apply(p, x[r][k - 1]);
-    } else {
-#pragma unroll
-      for (int k = 0; k < V; ++k) y[r][k] = C::apply(p, x[r][k]);
-    }
-  }
-  store_tile<T, BLOCK, ROWS>(out, base, n, vec_ok, y);
-}
-
-// Pairwise tree over a register array (depth log2(N)).
-template <typename T, int OP, int N>
-__device__ inline T tree_fold(T (&v)[N]) {
-  using C = Combine<T, OP>;
-#pragma unroll
-  for (int s = 1; s < N; s *= 2) {
-#pragma unroll
-    for (int i = 0; i + s < N; i += 2 * s) v[i] = C::apply(v[i], v[i + s]);
-  }
-  return v[0];
-}
-
-// Reduces one tile of BLOCK*ROWS vectors per block into partials[blockIdx.x]
-// with a fixed tree: thread (pairwise over ROWS*V) -> wave (DPP) -> block
-// (pairwise over waves). With `apply_init`, block 0 writes op(init, total)
-// (used when one block covers the whole input).
-template <typename T, int OP, int BLOCK, int ROWS>
-__global__ __launch_bounds__(BLOCK) void reduce_tile_kernel(const T* __restrict__ in, uint32_t n,
-                                                            bool aligned, T* __restrict__ partials,
-                                                            bool apply_init, T init) {
-  using C = Combine<T, OP>;
-  constexpr int V = 16 / sizeof(T);
-  constexpr int NW = BLOCK / kWave;
-  constexpr uint32_t TILE = (uint32_t)BLOCK * ROWS * V;
-  __shared__ T s_wave[NW];
-  const uint32_t tid = threadIdx.x;
-  const size_t base = (size_t)blockIdx.x * TILE;
-  const bool vec_ok = aligned && (base + TILE <= n);
-  T x[ROWS][V];
-  load_tile<T, BLOCK, ROWS>(in, base, n, vec_ok, x, C::identity());
-  T flat[ROWS * V];
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r)
-#pragma unroll
-    for (int k = 0; k < V; ++k) flat[r * V + k] = x[r][k];
-  T t = tree_fold<T, OP, ROWS * V>(flat);
-  t = wave_reduce<T, OP>(t);
-  if ((tid & (kWave - 1)) == 0) s_wave[tid / kWave] = t;
-  __syncthreads();
-  if (tid == 0) {
-    T w[NW];
-#pragma unroll
-    for (int i = 0; i < NW; ++i) w[i] = s_wave[i];
-    T total = tree_fold<T, OP, NW>(w);
-    partials[blockIdx.x] = apply_init ? C::apply(init, total) : total;
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Host dispatch.
@@ -263,6 +57,7 @@ static msv_status reduce_impl(Ctx& c, const T* in, uint32_t n, T init, T* d_resu
 
 template <typename T, int OP, bool EXCL>
 static msv_status float_scan_impl(Ctx& c, const T* in, T* out, uint32_t n, T init) {
+  constexpr int kScanRows = kScanRowsSmall;
   constexpr uint32_t TILE = (uint32_t)kScanBlock * kScanRows * (16 / sizeof(T));
   const bool aligned = is_aligned16<T>(in) && is_aligned16<T>(out);
   const uint32_t ntiles = (n + TILE - 1) / TILE;
@@ -280,24 +75,31 @@ static msv_status float_scan_impl(Ctx& c, const T* in, T* out, uint32_t n, T ini
   }
   hipLaunchKernelGGL((scan_tile_kernel<T, OP, EXCL, SCAN_PREFIXED, kScanBlock, kScanRows>),
                      dim3(ntiles), dim3(kScanBlock), 0, c.stream(), in, out, n, init, aligned,
-                     (uint64_t*)nullptr, (uint64_t*)nullptr, ntiles, prefixes);
+                     (uint64_t*)nullptr, 0u, prefixes);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+template <typename T, int OP, bool EXCL, int ROWS>
+static msv_status int_scan_rows(Ctx& c, const T* in, T* out, uint32_t n, T init) {
+  constexpr uint32_t TILE = (uint32_t)kScanBlock * ROWS * (16 / sizeof(T));
+  const uint32_t ntiles = (n + TILE - 1) / TILE;
+  uint64_t *counter, *status;
+  uint32_t gen;
+  MSV_TRY(c.lookback((size_t)ntiles * TileStatus<T>::kWords, &counter, &status, &gen));
+  const bool aligned = is_aligned16<T>(in) && is_aligned16<T>(out);
+  hipLaunchKernelGGL((scan_tile_kernel<T, OP, EXCL, SCAN_LOOKBACK, kScanBlock, ROWS>),
+                     dim3(ntiles), dim3(kScanBlock), 0, c.stream(), in, out, n, init, aligned,
+                     status, gen, (const T*)nullptr);
   MSV_HIP(hipGetLastError());
   return MSV_OK;
 }
 
 template <typename T, int OP, bool EXCL>
 static msv_status int_scan_impl(Ctx& c, const T* in, T* out, uint32_t n, T init) {
-  constexpr uint32_t TILE = (uint32_t)kScanBlock * kScanRows * (16 / sizeof(T));
-  const uint32_t ntiles = (n + TILE - 1) / TILE;
-  uint64_t* counter;
-  uint64_t* status;
-  MSV_TRY(c.lookback((size_t)ntiles * TileStatus<T>::kWords, &counter, &status));
-  const bool aligned = is_aligned16<T>(in) && is_aligned16<T>(out);
-  hipLaunchKernelGGL((scan_tile_kernel<T, OP, EXCL, SCAN_LOOKBACK, kScanBlock, kScanRows>),
-                     dim3(ntiles), dim3(kScanBlock), 0, c.stream(), in, out, n, init, aligned,
-                     counter, status, ntiles, (const T*)nullptr);
-  MSV_HIP(hipGetLastError());
-  return MSV_OK;
+  if (n >= kScanLargeMin)
+    return int_scan_rows<T, OP, EXCL, scan_rows_large<T>()>(c, in, out, n, init);
+  return int_scan_rows<T, OP, EXCL, kScanRowsSmall>(c, in, out, n, init);
 }
 
 template <typename T, int OP>

@@ -53,7 +53,7 @@ template <typename T, int OP, int MODE, int OUT>
 __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
     const T* __restrict__ in, uint32_t n, const uint32_t* __restrict__ offsets, uint32_t nseg,
     T* __restrict__ out, T init, bool aligned, uint64_t* counter, uint64_t* status,
-    uint32_t ntiles, uint32_t* __restrict__ tile_f, T* __restrict__ tile_v) {
+    uint32_t gen, uint32_t ntiles, uint32_t* __restrict__ tile_f, T* __restrict__ tile_v) {
   using C = Combine<T, OP>;
   constexpr int V = 16 / sizeof(T);
   constexpr int NW = kSegBlock / kWave;
@@ -64,19 +64,18 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
   __shared__ uint32_t s_cf;
   __shared__ T s_cv;
   __shared__ uint32_t s_first;
-  __shared__ TileTicket s_ticket;
+  __shared__ uint32_t s_tile;
   __shared__ T s_vals[OUT == SEG_REDUCE ? TILE : 1];
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
-  uint32_t tile, gen = 0;
+  uint32_t tile;
   if constexpr (MODE == SEG_LOOKBACK) {
-    if (tid == 0) s_ticket = draw_tile(counter, ntiles);
+    if (tid == 0) s_tile = draw_tile(counter, ntiles);
   }
   for (uint32_t i = tid; i < TILE / 32; i += kSegBlock) s_heads[i] = 0;
   __syncthreads();
   if constexpr (MODE == SEG_LOOKBACK) {
-    tile = s_ticket.tile;
-    gen = s_ticket.gen;
+    tile = s_tile;
   } else {
     tile = blockIdx.x;
   }
@@ -365,14 +364,15 @@ static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
   const bool aligned = (vec_ptrs & 15u) == 0;
 #define MSV_SEG_LAUNCH(MODE, OUTK, CNT, ST, TF, TV)                                               \
   hipLaunchKernelGGL((seg_tile_kernel<T, OP, MODE, OUTK>), dim3(ntiles), dim3(kSegBlock), 0,      \
-                     c.stream(), in, n, offsets, nseg, out, init, aligned, CNT, ST, ntiles, TF, TV)
+                     c.stream(), in, n, offsets, nseg, out, init, aligned, CNT, ST, gen, ntiles, TF, TV)
 #define MSV_SEG_BY_OUT(MODE, CNT, ST, TF, TV)                  \
   if (outk == SEG_INCL) MSV_SEG_LAUNCH(MODE, SEG_INCL, CNT, ST, TF, TV);        \
   else if (outk == SEG_EXCL) MSV_SEG_LAUNCH(MODE, SEG_EXCL, CNT, ST, TF, TV);   \
   else MSV_SEG_LAUNCH(MODE, SEG_REDUCE, CNT, ST, TF, TV);
+  uint32_t gen = 0;
   if constexpr (!IsFloat<T>::value) {
     uint64_t *counter, *status;
-    MSV_TRY(c.lookback((size_t)ntiles * TileStatus<T>::kWords, &counter, &status));
+    MSV_TRY(c.lookback((size_t)ntiles * TileStatus<T>::kWords, &counter, &status, &gen));
     MSV_SEG_BY_OUT(SEG_LOOKBACK, counter, status, (uint32_t*)nullptr, (T*)nullptr)
   } else {
     uint32_t* af = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
@@ -382,7 +382,7 @@ static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
     if (!af || !av || !pf || !pv) return MSV_ERR_OUT_OF_MEMORY;
     hipLaunchKernelGGL((seg_tile_kernel<T, OP, SEG_AGGREGATE, SEG_INCL>), dim3(ntiles),
                        dim3(kSegBlock), 0, c.stream(), in, n, offsets, nseg, out, init, aligned,
-                       (uint64_t*)nullptr, (uint64_t*)nullptr, ntiles, af, av);
+                       (uint64_t*)nullptr, (uint64_t*)nullptr, 0u, ntiles, af, av);
     MSV_HIP(hipGetLastError());
     hipLaunchKernelGGL((seg_pairs_exclusive_kernel<T, OP>), dim3(1), dim3(1024), 0, c.stream(),
                        af, av, ntiles, pf, pv);

@@ -75,18 +75,15 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
                                                            uint32_t n, bool aligned,
                                                            T* __restrict__ out,
                                                            uint32_t* __restrict__ out_len,
-                                                           uint64_t* counter, uint64_t* status,
+                                                           uint64_t* status, uint32_t gen,
                                                            uint32_t ntiles) {
   constexpr int NW = kSelBlock / kWave;
   __shared__ T s_stage[kSelTile];
   __shared__ uint32_t s_wave[kSelRows][NW];
   __shared__ uint32_t s_prefix;
-  __shared__ TileTicket s_ticket;
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
-  if (tid == 0) s_ticket = draw_tile(counter, ntiles);
-  __syncthreads();
-  const uint32_t tile = s_ticket.tile, gen = s_ticket.gen;
+  const uint32_t tile = blockIdx.x;
   const size_t base = (size_t)tile * kSelTile;
   const bool full = aligned && base + kSelTile <= n;
 
@@ -137,7 +134,21 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
       if (lane == 0) st.publish(0, kFlagInc, 0, agg);
     } else {
       if (lane == 0) st.publish(tile, kFlagAgg, 0, agg);
-      excl = wave_lookback<uint32_t, OP_ADD>(st, tile);
+      // Self-help: count the selected elements of tile q from the input.
+      auto count_of = [&](uint32_t q) -> uint32_t {
+        const size_t b = (size_t)q * kSelTile;
+        const size_t e = min((size_t)n, b + kSelTile);
+        uint32_t c = 0;
+        for (size_t i = b + lane; i < e; i += kWave) {
+          bool f;
+          if constexpr (FORM == 0) f = stencil[i] != 0;
+          else if constexpr (FORM == 1) f = pred(stencil[i]);
+          else f = pred(bit_cast<S>(in[i]));
+          c += (f != invert) ? 1u : 0u;
+        }
+        return wave_reduce<uint32_t, OP_ADD>(c);
+      };
+      excl = wave_lookback_sh<uint32_t, OP_ADD>(st, tile, count_of);
       if (lane == 0) st.publish(tile, kFlagInc, 0, excl + agg);
     }
     if (lane == 0) {
@@ -183,11 +194,12 @@ static msv_status select_launch(Ctx& c, const void* in, const void* st, const ms
   }
   const uint32_t ntiles = (n + kSelTile - 1) / kSelTile;
   uint64_t *counter, *status;
-  MSV_TRY(c.lookback(ntiles, &counter, &status));
+  uint32_t gen;
+  MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));
   const bool aligned = (((uintptr_t)in | (uintptr_t)st) & 15u) == 0;
   hipLaunchKernelGGL((select_kernel<T, S, FORM>), dim3(ntiles), dim3(kSelBlock), 0, c.stream(),
                      (const T*)in, (const S*)st, pred, invert, n, aligned, (T*)out, out_len,
-                     counter, status, ntiles);
+                     status, gen, ntiles);
   MSV_HIP(hipGetLastError());
   return MSV_OK;
 }
