@@ -20,7 +20,10 @@ the executor's stream around each launch). The same JSON line carries, under
 cpu_baseline: the C oracle (a restatement of the reference's sequential CPU
 path) timed on this host, one thread, on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--only a,b] [--no-cpu] [--quick]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--only a,b] [--no-cpu]
+
+`--only copy` adds the calibration copy (torch's copy kernel over 2^28 u32),
+which the default run leaves out.
 """
 from __future__ import annotations
 
@@ -279,7 +282,43 @@ def cpu_sort_baseline(n_sample: int):
                 sample=f"stable sort_by_key of {n_sample} u64 keys (seed 6) + u32 payload, one run")
 
 
+def w_copy(c):
+    """Calibration leg: a plain device copy of 2^28 u32 (1 GiB read + 1 GiB
+    written) by torch's copy kernel. Its FETCH_SIZE / WRITE_SIZE counters
+    scale the counters of the other kernels to bytes, and its rate is the
+    practical streaming ceiling the scan is compared against."""
+    import torch
+
+    n = 1 << 28
+    x = torch.arange(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(c.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(c.steps)]
+    for a, b in evs:
+        a.record()
+        y.copy_(x)
+        b.record()
+    torch.cuda.synchronize()
+    ms = [a.elapsed_time(b) for a, b in evs]
+    return summarize("device_copy_u32_2^28", ms, 8.0 * n, n)
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch of a kernel from a committed counter summary
+    (a `traffic_bytes_per_launch: <int>` line in profiles/<name>.md), or None."""
+    f = ROOT / "profiles" / f"{name}.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1].strip())
+    return None
+
+
 WORKLOADS = {
+    "copy": w_copy,
     "reduce": w_reduce,
     "copy_where": w_copy_where,
     "gather_scatter": w_gather_scatter,
@@ -312,7 +351,7 @@ def main() -> int:
     scan = w_scan(c)
     extra = {}
     for name, fn in WORKLOADS.items():
-        if only and name not in only:
+        if (only and name not in only) or (not only and name == "copy"):
             continue
         log(f"[bench] workload {name}")
         extra[name] = fn(c)
@@ -344,7 +383,7 @@ def main() -> int:
         "config": {"workload": "inclusive_scan add over 2^28 u32 uniform in [0,1024) (seed 2), "
                                "wrapping; device-resident input", "n": n},
         "roofline": {"bound": "hbm", "achieved": scan["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": scan["frac_hbm"], "traffic": None},
+                     "frac": scan["frac_hbm"], "traffic": load_traffic("pmc_inclusive_scan_u32_2^28")},
         "cpu_baseline": cpu,
         "sort_gkeys_per_s_u32_2^28": sort_g,
         "extra": extra,

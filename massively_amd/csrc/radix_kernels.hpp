@@ -1,0 +1,320 @@
+// LSD radix sort kernels (device code), shared by radix.hip and the
+// tuning probes. See radix.hip for the pipeline description.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace msv {
+
+constexpr int kRadixIpt = 16;     // keys per thread in a pass tile
+constexpr int kRadixBins = 256;   // 8-bit digits
+constexpr int kHistBlock = 256;
+
+// Element moved by the passes: the key (1 or 2 dwords) followed by the value
+// (0, 1 or 2 dwords). Between passes elements are kept interleaved (AoS), so
+// each pass reorders one array and writes one stream of runs; the first pass
+// reads the caller's separate key/value arrays and the last writes them.
+template <int EW>
+struct alignas(4) Elem {
+  uint32_t w[EW];
+};
+
+template <typename K, int EW>
+__device__ inline K elem_key(const Elem<EW>& e) {
+  if constexpr (sizeof(K) == 4) return bit_cast<K>(e.w[0]);
+  else return bit_cast<K>(((uint64_t)e.w[1] << 32) | e.w[0]);
+}
+
+struct RadixPass {
+  uint32_t active;
+  uint32_t shift;
+  uint32_t in_aos;   // source is an element array (else separate keys/values)
+  uint32_t out_aos;  // destination is an element array
+  const void* src_k;
+  const void* src_v;
+  void* dst_k;
+  void* dst_v;
+};
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// 1. All-digit histogram: each block counts 16 consecutive groups of 256 keys
+//    (loads issued before the LDS updates), per-wave LDS counters.
+template <typename K, int MODE>
+__global__ __launch_bounds__(kHistBlock) void radix_hist_kernel(const K* __restrict__ keys,
+                                                                uint32_t n,
+                                                                uint32_t* __restrict__ g_hist) {
+  constexpr int P = sizeof(K);
+  constexpr int NW = kHistBlock / kWave;
+  constexpr int J = 16;
+  __shared__ uint32_t s_hist[NW][P][kRadixBins];
+  const uint32_t tid = threadIdx.x, wid = tid / 64u;
+  for (uint32_t i = tid; i < NW * P * kRadixBins; i += kHistBlock) (&s_hist[0][0][0])[i] = 0;
+  __syncthreads();
+  for (size_t base = (size_t)blockIdx.x * kHistBlock * J; base < n;
+       base += (size_t)gridDim.x * kHistBlock * J) {
+    typename Bits<K>::U e[J];
+    bool ok[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const size_t i = base + (size_t)j * kHistBlock + tid;
+      ok[j] = i < n;
+      e[j] = ok[j] ? order_bits<K, MODE>(keys[i]) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      if (!ok[j]) continue;
+#pragma unroll
+      for (int p = 0; p < P; ++p) atomicAdd(&s_hist[wid][p][(uint32_t)(e[j] >> (8 * p)) & 0xffu], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < P * kRadixBins; i += kHistBlock) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) c += (&s_hist[w][0][0])[i];
+    if (c) atomicAdd(&g_hist[i], c);
+  }
+}
+
+// 2. Pass plan: which digit positions need a pass (a position whose
+//    histogram has a single bin of count n is skipped) and the source and
+//    destination of each remaining pass: caller arrays -> element buffer A0
+//    -> A1 -> A0 ... -> caller arrays.
+template <int P>
+__global__ __launch_bounds__(kRadixBins) void radix_plan_kernel(
+    const uint32_t* __restrict__ g_hist, uint32_t n, RadixPass* __restrict__ table,
+    const void* in_k, const void* in_v, void* out_k, void* out_v, void* aos0, void* aos1) {
+  __shared__ uint32_t s_trivial[P];
+  const uint32_t t = threadIdx.x;
+  for (int p = 0; p < P; ++p) {
+    const int triv = __syncthreads_or(g_hist[p * kRadixBins + t] == n);
+    if (t == 0) s_trivial[p] = triv ? 1u : 0u;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int active = 0;
+    for (int p = 0; p < P; ++p) active += s_trivial[p] ? 0 : 1;
+    // All digits constant (all keys equal): one pass still moves the data.
+    const bool force_last = active == 0;
+    if (force_last) active = 1;
+    int j = 0;
+    for (int p = 0; p < P; ++p) {
+      RadixPass rp;
+      rp.shift = 8u * (uint32_t)p;
+      rp.active = (!s_trivial[p] || (force_last && p == P - 1)) ? 1u : 0u;
+      rp.in_aos = rp.out_aos = 0;
+      rp.src_k = rp.src_v = nullptr;
+      rp.dst_k = rp.dst_v = nullptr;
+      if (rp.active) {
+        rp.in_aos = j > 0;
+        rp.out_aos = j < active - 1;
+        if (j == 0) {
+          rp.src_k = in_k;
+          rp.src_v = in_v;
+        } else {
+          rp.src_k = ((j - 1) & 1) ? aos1 : aos0;
+        }
+        if (j == active - 1) {
+          rp.dst_k = out_k;
+          rp.dst_v = out_v;
+        } else {
+          rp.dst_k = (j & 1) ? aos1 : aos0;
+        }
+        ++j;
+      }
+      table[p] = rp;
+    }
+  }
+}
+
+template <typename K, int EW>
+__device__ inline K load_key(const RadixPass& rp, size_t i) {
+  if (rp.in_aos) return elem_key<K, EW>(((const Elem<EW>*)rp.src_k)[i]);
+  return ((const K*)rp.src_k)[i];
+}
+
+template <typename K, int EW>
+__device__ inline Elem<EW> load_elem(const RadixPass& rp, size_t i) {
+  if (rp.in_aos) return ((const Elem<EW>*)rp.src_k)[i];
+  Elem<EW> e;
+  const K k = ((const K*)rp.src_k)[i];
+  if constexpr (sizeof(K) == 4) {
+    e.w[0] = bit_cast<uint32_t>(k);
+  } else {
+    const uint64_t u = bit_cast<uint64_t>(k);
+    e.w[0] = (uint32_t)u;
+    e.w[1] = (uint32_t)(u >> 32);
+  }
+  constexpr int KW = sizeof(K) / 4;
+#pragma unroll
+  for (int v = 0; v < EW - KW; ++v) e.w[KW + v] = ((const uint32_t*)rp.src_v)[i * (EW - KW) + v];
+  return e;
+}
+
+template <typename K, int EW>
+__device__ inline void store_elem(const RadixPass& rp, size_t i, const Elem<EW>& e) {
+  if (rp.out_aos) {
+    ((Elem<EW>*)rp.dst_k)[i] = e;
+    return;
+  }
+  constexpr int KW = sizeof(K) / 4;
+  ((K*)rp.dst_k)[i] = elem_key<K, EW>(e);
+#pragma unroll
+  for (int v = 0; v < EW - KW; ++v) ((uint32_t*)rp.dst_v)[i * (EW - KW) + v] = e.w[KW + v];
+}
+
+// 3. Per-pass reduce-then-scan over tiles of BLOCK * IPT keys.
+//  3a. upsweep: digit counts of every tile, digit-major
+//      (counts[d * ntiles + tile]), so that one exclusive scan of `counts`
+//      yields the global position of each (digit, tile) run.
+template <typename K, int MODE, int EW, int BLOCK, int IPT = kRadixIpt>
+__global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const RadixPass* __restrict__ table,
+                                                             int pass, uint32_t n, uint32_t ntiles,
+                                                             uint32_t* __restrict__ counts) {
+  constexpr int NW = BLOCK / kWave;
+  constexpr uint32_t TILE = (uint32_t)BLOCK * IPT;
+  __shared__ uint32_t s_cnt[NW][kRadixBins];
+  const RadixPass rp = table[pass];
+  if (!rp.active) return;
+  const uint32_t tid = threadIdx.x, wid = tid / 64u;
+  for (uint32_t i = tid; i < NW * kRadixBins; i += BLOCK) (&s_cnt[0][0])[i] = 0;
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * TILE;
+  const uint32_t shift = rp.shift;
+  uint32_t dig[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const size_t i = base + (size_t)j * BLOCK + tid;
+    dig[j] = i < n ? ((uint32_t)(order_bits<K, MODE>(load_key<K, EW>(rp, i)) >> shift) & 0xffu)
+                   : 0x100u;
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (dig[j] < 0x100u) atomicAdd(&s_cnt[wid][dig[j]], 1u);
+  __syncthreads();
+  for (uint32_t d = tid; d < (uint32_t)kRadixBins; d += BLOCK) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) c += s_cnt[w][d];
+    counts[(size_t)d * ntiles + blockIdx.x] = c;
+  }
+}
+
+//  3b. downsweep: rank the tile's keys stably, reorder the elements through
+//      LDS into digit-sorted runs, and write the runs to their offsets
+//      (offsets[d * ntiles + tile], the exclusive scan of `counts`).
+template <typename K, int MODE, int EW, int BLOCK, int IPT = kRadixIpt>
+__global__ __launch_bounds__(BLOCK) void radix_downsweep_kernel(
+    const RadixPass* __restrict__ table, int pass, const uint32_t* __restrict__ offsets,
+    uint32_t n, uint32_t ntiles) {
+  constexpr int NW = BLOCK / kWave;
+  constexpr uint32_t TILE = (uint32_t)BLOCK * IPT;
+  __shared__ Elem<EW> s_el[TILE];
+  __shared__ uint32_t s_hist[NW][kRadixBins];     // per-wave counts, then wave offsets
+  __shared__ uint32_t s_tile_start[kRadixBins];   // in-tile start of each digit
+  __shared__ uint32_t s_goff[kRadixBins];         // global position - in-tile position
+  __shared__ uint32_t s_wsum[kRadixBins / kWave];
+
+  const RadixPass rp = table[pass];
+  if (!rp.active) return;
+
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
+  const bool digit_thread = tid < (uint32_t)kRadixBins;
+  for (uint32_t i = tid; i < NW * kRadixBins; i += BLOCK) (&s_hist[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t tile = blockIdx.x;
+  const size_t base = (size_t)tile * TILE + (size_t)wid * (kWave * IPT);
+  const uint32_t shift = rp.shift;
+
+  Elem<EW> el[IPT];
+  uint32_t dig[IPT];
+  uint32_t rank[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const size_t i = base + (size_t)j * kWave + lane;
+    if (i < n) {
+      el[j] = load_elem<K, EW>(rp, i);
+      dig[j] = (uint32_t)(order_bits<K, MODE>(elem_key<K, EW>(el[j])) >> shift) & 0xffu;
+    } else {
+      dig[j] = 0x100u;
+    }
+  }
+  uint32_t goff = 0;
+  if (digit_thread) goff = offsets[(size_t)tid * ntiles + tile];
+
+  // Wave-level multi-split: rank each key among the wave's earlier keys with
+  // the same digit (input order: j-major, lane-minor).
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t d = dig[j];
+    const bool valid = d < 0x100u;
+    // peers = lanes whose digit equals ours. One ballot per digit bit; a lane
+    // differs from us where the ballot bit differs from our own bit, so the
+    // differences are collected with 32-bit XOR/OR against our bit spread to
+    // 0 / all-ones (signed bit extract).
+    uint32_t diff_lo = 0, diff_hi = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint32_t s = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
+      const uint64_t m = ballot(s != 0);
+      diff_lo |= (uint32_t)m ^ s;
+      diff_hi |= (uint32_t)(m >> 32) ^ s;
+    }
+    const uint64_t vmask = ballot(valid);
+    const uint64_t peers = vmask & ~(((uint64_t)diff_hi << 32) | diff_lo);
+    uint32_t before = 0;
+    if (valid) before = s_hist[wid][d];
+    const uint32_t below = (uint32_t)__builtin_popcountll(peers & lt);
+    if (valid && below == 0) s_hist[wid][d] = before + (uint32_t)__builtin_popcountll(peers);
+    rank[j] = before + below;
+  }
+  __syncthreads();
+
+  // Per digit (thread t <-> digit t): wave offsets, tile count, tile start.
+  uint32_t cnt = 0;
+  if (digit_thread) {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t c = s_hist[w][tid];
+      s_hist[w][tid] = cnt;
+      cnt += c;
+    }
+  }
+  uint32_t incl = 0;
+  if (digit_thread) {
+    incl = wave_inclusive_scan<uint32_t, OP_ADD>(cnt);
+    if (lane == 63) s_wsum[wid] = incl;
+  }
+  __syncthreads();
+  if (digit_thread) {
+    uint32_t woff = 0;
+    for (uint32_t k = 0; k < wid; ++k) woff += s_wsum[k];
+    const uint32_t tstart = woff + incl - cnt;
+    s_tile_start[tid] = tstart;
+    s_goff[tid] = goff - tstart;
+  }
+  __syncthreads();
+
+  // Reorder elements into digit-sorted runs in LDS, then write the runs out.
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (dig[j] < 0x100u) s_el[s_tile_start[dig[j]] + s_hist[wid][dig[j]] + rank[j]] = el[j];
+  __syncthreads();
+  const uint32_t tile_n = (uint32_t)min((size_t)TILE, (size_t)n - (size_t)tile * TILE);
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    const uint32_t i = tid + (uint32_t)q * BLOCK;
+    if (i < tile_n) {
+      const Elem<EW> e = s_el[i];
+      const uint32_t d = (uint32_t)(order_bits<K, MODE>(elem_key<K, EW>(e)) >> shift) & 0xffu;
+      store_elem<K, EW>(rp, (size_t)s_goff[d] + i, e);
+    }
+  }
+}
+
+}  // namespace msv

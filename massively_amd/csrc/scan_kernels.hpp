@@ -10,7 +10,7 @@ constexpr int kScanBlock = 256;
 constexpr int kScanRowsSmall = 4;   // 16 KiB per tile (4096 u32)
 // 64 KiB per tile for long inputs (amortises the lookback): 16384 u32 / 8192 u64.
 template <typename T>
-constexpr int scan_rows_large() { return sizeof(T) == 4 ? 16 : 8; }
+constexpr int scan_rows_large() { return sizeof(T) == 4 ? 32 : 16; }
 constexpr int kReduceRows = 16;
 // Inputs at least this long use the large tile (>= 256 tiles).
 constexpr uint32_t kScanLargeMin = 1u << 22;
