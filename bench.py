@@ -247,8 +247,14 @@ def w_seg(c):
     ms = timed(c.ex, lambda: seg.ForEachSegment(seg.Reduce(m.op.Add, 0.0)).run(c.ex, itf), c.steps, c.warmup)
     out["reduce_f32"] = summarize("seg_reduce_f32_2^26_in_2^20", ms, 4.0 * n + 4.0 * (S + 1) + 4.0 * S, n)
     its = seg.SegmentIterator((x, ff), sg)
-    ms = timed(c.ex, lambda: seg.ForEachSegment(seg.Sort(m.op.Less)).run(c.ex, its), max(2, c.steps // 4), 1)
-    out["sort_u32_f32"] = summarize("seg_sort_u32key_f32val_2^26_in_2^20", ms, 16.0 * n + 4.0 * (S + 1), n)
+    # seg::Sort: one persistent launch over all size classes (tile shape chosen
+    # by the autotuner on the warm-up call) vs one launch per size class.
+    for launch, tag in ((c.ex.SEG_LAUNCH_PERSISTENT, ""), (c.ex.SEG_LAUNCH_PER_CLASS, "_per_class")):
+        c.ex.set_option(c.ex.OPT_SEG_LAUNCH, launch)
+        ms = timed(c.ex, lambda: seg.ForEachSegment(seg.Sort(m.op.Less)).run(c.ex, its), c.steps, 1)
+        out["sort_u32_f32" + tag] = summarize("seg_sort_u32key_f32val_2^26_in_2^20" + tag, ms,
+                                              16.0 * n + 4.0 * (S + 1), n)
+    c.ex.set_option(c.ex.OPT_SEG_LAUNCH, c.ex.SEG_LAUNCH_PERSISTENT)
     return out
 
 

@@ -127,6 +127,31 @@ void* msv_executor_stream(msv_executor* ex);
 /* Returns cached device memory to the driver (synchronises). */
 msv_status msv_executor_trim(msv_executor* ex);
 
+/* Tuning options of the batching layer. No counterpart in the reference
+ * (its launch shapes are fixed per kernel); results never depend on them.
+ *  MSV_OPT_SEG_LAUNCH: how seg::Sort runs its work items: one persistent
+ *    launch over a work queue (MSV_SEG_LAUNCH_PERSISTENT, default) or one
+ *    launch per size class (MSV_SEG_LAUNCH_PER_CLASS).
+ *  MSV_OPT_AUTOTUNE: 1 (default) times the tile-shape candidates on the first
+ *    seg::Sort of each (key width, element width, mean-segment-length class)
+ *    and keeps the fastest for the process; 0 uses the built-in default.
+ *  MSV_OPT_SEGSORT_CONFIG: -1 (default) = as above, else force candidate k
+ *    (0 <= k < msv_segsort_config_count()); MSV_ERR_UNSUPPORTED if candidate
+ *    k does not fit the element width. */
+typedef enum msv_option {
+  MSV_OPT_SEG_LAUNCH = 1,
+  MSV_OPT_AUTOTUNE = 2,
+  MSV_OPT_SEGSORT_CONFIG = 3
+} msv_option;
+#define MSV_SEG_LAUNCH_PERSISTENT 0
+#define MSV_SEG_LAUNCH_PER_CLASS 1
+msv_status msv_executor_set_option(msv_executor* ex, msv_option opt, int64_t value);
+msv_status msv_executor_get_option(msv_executor* ex, msv_option opt, int64_t* value);
+/* Number of seg::Sort tile-shape candidates, and candidate k's shape. */
+uint32_t msv_segsort_config_count(void);
+msv_status msv_segsort_config_info(uint32_t k, uint32_t* block, uint32_t* items_per_thread,
+                                   uint32_t* digit_bits);
+
 /* ---- device vectors (runtime.rs:79-177, allocation.rs:852-861) -------- */
 /* Executor::alloc: uninitialised vector of `len` elements. */
 msv_status msv_alloc(msv_executor* ex, msv_dtype dtype, uint32_t len, msv_vec** out);

@@ -15,8 +15,9 @@ from .error import (
     LengthTooLarge,
     OutputTooShort,
     UnboundColumn,
+    Unsupported,
 )
-from .runtime import DeviceSlice, DeviceVec, Executor
+from .runtime import DeviceSlice, DeviceVec, Executor, segsort_configs
 
 __all__ = [
     "Executor",
@@ -29,6 +30,8 @@ __all__ = [
     "InvalidSegmentation",
     "UnboundColumn",
     "ForeignExecutor",
+    "Unsupported",
+    "segsort_configs",
     "op",
     "seg",
     "vector",

@@ -82,6 +82,10 @@ PROTOTYPES = {
     "msv_executor_id": (_U64, [_P]),
     "msv_executor_stream": (_P, [_P]),
     "msv_executor_trim": (_I, [_P]),
+    "msv_executor_set_option": (_I, [_P, _I, C.c_int64]),
+    "msv_executor_get_option": (_I, [_P, _I, C.POINTER(C.c_int64)]),
+    "msv_segsort_config_count": (_U32, []),
+    "msv_segsort_config_info": (_I, [_U32, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32)]),
     "msv_alloc": (_I, [_P, _I, _U32, _PP]),
     "msv_to_device": (_I, [_P, _I, _P, _U32, _PP]),
     "msv_to_host": (_I, [_P, _S, _P, _U32, C.POINTER(_U32)]),

@@ -90,6 +90,10 @@ struct msv_executor {
   uint64_t* d_status;
   size_t status_words;
   uint32_t lookback_gen;
+  // Tuning options (msv_executor_set_option).
+  int64_t opt_seg_launch = MSV_SEG_LAUNCH_PERSISTENT;
+  int64_t opt_autotune = 1;
+  int64_t opt_segsort_config = -1;
   explicit msv_executor(int dev) : allocator(dev) {}
 };
 
@@ -166,5 +170,10 @@ msv_status iota_run(Ctx& c, uint32_t* out, uint32_t n, uint32_t start);
 // mode: 0 total order, 1 Less, 2 Greater; vsize 0 (keys only), 4 or 8.
 msv_status radix_sort_run(Ctx& c, msv_dtype kdt, int mode, size_t vsize, const void* keys,
                           const void* vals, uint32_t n, void* out_k, void* out_v);
+// Stable sort of every segment of offsets[0..nseg] on its own (mode 1 Less,
+// 2 Greater); vsize 0, 4 or 8.
+msv_status seg_sort_run(Ctx& c, msv_dtype kdt, int mode, const void* keys, const void* vals,
+                        size_t vsize, uint32_t n, const uint32_t* offsets, uint32_t nseg,
+                        void* out_k, void* out_v);
 
 }  // namespace msv

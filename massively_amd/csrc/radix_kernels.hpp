@@ -36,7 +36,24 @@ struct RadixPass {
   void* dst_v;
 };
 
-__global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
+// Lanes of the wave holding a valid digit equal to ours (the low DBITS bits of
+// d). One ballot per digit bit; a lane differs from us where its ballot bit
+// differs from our own bit, so the differences are collected with 32-bit
+// XOR/OR against our bit spread to 0 / all-ones (signed bit extract).
+template <int DBITS>
+__device__ inline uint64_t match_digit(uint32_t d, bool valid) {
+  uint32_t diff_lo = 0, diff_hi = 0;
+#pragma unroll
+  for (int b = 0; b < DBITS; ++b) {
+    const uint32_t s = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
+    const uint64_t m = ballot(s != 0);
+    diff_lo |= (uint32_t)m ^ s;
+    diff_hi |= (uint32_t)(m >> 32) ^ s;
+  }
+  return ballot(valid) & ~(((uint64_t)diff_hi << 32) | diff_lo);
+}
+
+static __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -253,20 +270,7 @@ __global__ __launch_bounds__(BLOCK) void radix_downsweep_kernel(
   for (int j = 0; j < IPT; ++j) {
     const uint32_t d = dig[j];
     const bool valid = d < 0x100u;
-    // peers = lanes whose digit equals ours. One ballot per digit bit; a lane
-    // differs from us where the ballot bit differs from our own bit, so the
-    // differences are collected with 32-bit XOR/OR against our bit spread to
-    // 0 / all-ones (signed bit extract).
-    uint32_t diff_lo = 0, diff_hi = 0;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint32_t s = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
-      const uint64_t m = ballot(s != 0);
-      diff_lo |= (uint32_t)m ^ s;
-      diff_hi |= (uint32_t)(m >> 32) ^ s;
-    }
-    const uint64_t vmask = ballot(valid);
-    const uint64_t peers = vmask & ~(((uint64_t)diff_hi << 32) | diff_lo);
+    const uint64_t peers = match_digit<8>(d, valid);
     uint32_t before = 0;
     if (valid) before = s_hist[wid][d];
     const uint32_t below = (uint32_t)__builtin_popcountll(peers & lt);

@@ -314,6 +314,46 @@ msv_status msv_executor_trim(msv_executor* ex) {
   return MSV_OK;
 }
 
+msv_status msv_executor_set_option(msv_executor* ex, msv_option opt, int64_t value) {
+  clear_error();
+  if (!ex) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null executor");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  switch (opt) {
+    case MSV_OPT_SEG_LAUNCH:
+      if (value != MSV_SEG_LAUNCH_PERSISTENT && value != MSV_SEG_LAUNCH_PER_CLASS) break;
+      ex->opt_seg_launch = value;
+      return MSV_OK;
+    case MSV_OPT_AUTOTUNE:
+      if (value != 0 && value != 1) break;
+      ex->opt_autotune = value;
+      return MSV_OK;
+    case MSV_OPT_SEGSORT_CONFIG:
+      if (value < -1 || value >= (int64_t)msv_segsort_config_count()) break;
+      ex->opt_segsort_config = value;
+      return MSV_OK;
+  }
+  set_error(MSV_ERR_INVALID_ARGUMENT, (uint64_t)opt, (uint64_t)value, "bad option %d = %lld",
+            (int)opt, (long long)value);
+  return MSV_ERR_INVALID_ARGUMENT;
+}
+
+msv_status msv_executor_get_option(msv_executor* ex, msv_option opt, int64_t* value) {
+  clear_error();
+  if (!ex || !value) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null executor or output");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  switch (opt) {
+    case MSV_OPT_SEG_LAUNCH: *value = ex->opt_seg_launch; return MSV_OK;
+    case MSV_OPT_AUTOTUNE: *value = ex->opt_autotune; return MSV_OK;
+    case MSV_OPT_SEGSORT_CONFIG: *value = ex->opt_segsort_config; return MSV_OK;
+  }
+  set_error(MSV_ERR_INVALID_ARGUMENT, (uint64_t)opt, 0, "bad option %d", (int)opt);
+  return MSV_ERR_INVALID_ARGUMENT;
+}
+
 msv_status msv_alloc(msv_executor* ex, msv_dtype dtype, uint32_t len, msv_vec** out) {
   clear_error();
   if (!ex || !dtype_valid(dtype)) {

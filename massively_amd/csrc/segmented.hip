@@ -337,14 +337,6 @@ __global__ void validate_offsets_kernel(const uint32_t* __restrict__ offsets, ui
   if (i + 1 == m) status[1] = o;
 }
 
-__global__ void head_scatter_kernel(const uint32_t* __restrict__ offsets, uint32_t nseg,
-                                    uint32_t* __restrict__ sid) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nseg) return;
-  const uint32_t st = offsets[s], en = offsets[s + 1];
-  if (en > st) sid[st] = s;
-}
-
 // ---- host -------------------------------------------------------------------------
 template <typename T, int OP>
 static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
@@ -426,46 +418,6 @@ msv_status seg_scan_run(Ctx& c, msv_dtype dt, msv_op op, int outk, const void* i
   }
   set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
   return MSV_ERR_INVALID_ARGUMENT;
-}
-
-// Segment id of every element: ids[offsets[s]] = s for non-empty s, then an
-// inclusive max-scan carries each id over its segment.
-static msv_status segment_ids(Ctx& c, const uint32_t* offsets, uint32_t nseg, uint32_t n,
-                              uint32_t* ids) {
-  if (n == 0) return MSV_OK;
-  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3((n + 255) / 256), dim3(256), 0, c.stream(), ids,
-                     n, 0u);
-  MSV_HIP(hipGetLastError());
-  hipLaunchKernelGGL(head_scatter_kernel, dim3((nseg + 255) / 256), dim3(256), 0, c.stream(),
-                     offsets, nseg, ids);
-  MSV_HIP(hipGetLastError());
-  return scan_run(c, MSV_U32, MSV_OP_MAX, false, ids, ids, n, nullptr);
-}
-
-msv_status seg_sort_run(Ctx& c, msv_dtype kdt, int mode, const void* keys, const void* vals,
-                        size_t vsize, uint32_t n, const uint32_t* offsets, uint32_t nseg,
-                        void* out_k, void* out_v) {
-  if (n == 0) return MSV_OK;
-  uint32_t* ids = (uint32_t*)c.scratch(sizeof(uint32_t) * n);
-  uint32_t* iota = (uint32_t*)c.scratch(sizeof(uint32_t) * n);
-  uint32_t* perm1 = (uint32_t*)c.scratch(sizeof(uint32_t) * n);
-  uint32_t* ids1 = (uint32_t*)c.scratch(sizeof(uint32_t) * n);
-  uint32_t* perm2 = (uint32_t*)c.scratch(sizeof(uint32_t) * n);
-  void* ktmp = c.scratch((size_t)n * dtype_size(kdt));
-  void* idtmp = c.scratch(sizeof(uint32_t) * n);
-  if (!ids || !iota || !perm1 || !ids1 || !perm2 || !ktmp || !idtmp) return MSV_ERR_OUT_OF_MEMORY;
-  MSV_TRY(segment_ids(c, offsets, nseg, n, ids));
-  MSV_TRY(iota_run(c, iota, n, 0));
-  // Stable by key, then stable by segment id: (segment, key, position) order.
-  MSV_TRY(radix_sort_run(c, kdt, mode, 4, keys, iota, n, ktmp, perm1));
-  MSV_TRY(gather_run(c, MSV_U32, ids, n, perm1, ids1, n));
-  MSV_TRY(radix_sort_run(c, MSV_U32, 0, 4, ids1, perm1, n, idtmp, perm2));
-  MSV_TRY(gather_run(c, kdt, keys, n, perm2, out_k, n));
-  if (vals && out_v) {
-    const msv_dtype vdt = vsize == 8 ? MSV_U64 : MSV_U32;
-    MSV_TRY(gather_run(c, vdt, vals, n, perm2, out_v, n));
-  }
-  return MSV_OK;
 }
 
 }  // namespace msv

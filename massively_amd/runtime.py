@@ -58,6 +58,19 @@ class Executor:
     def trim(self) -> None:
         check(self._lib.msv_executor_trim(self.handle))
 
+    # Tuning options of the batching layer (results never depend on them).
+    OPT_SEG_LAUNCH, OPT_AUTOTUNE, OPT_SEGSORT_CONFIG = 1, 2, 3
+    SEG_LAUNCH_PERSISTENT, SEG_LAUNCH_PER_CLASS = 0, 1
+
+    def set_option(self, opt: int, value: int) -> None:
+        check(self._lib.msv_executor_set_option(self.handle, opt, value))
+
+    def get_option(self, opt: int) -> int:
+        v = C.c_int64()
+        check(self._lib.msv_executor_get_option(self.handle, opt, C.byref(v)))
+        return v.value
+
+
     def alloc(self, dtype, length: int) -> "DeviceVec":
         dt = L.DTYPE_OF_NP[np.dtype(dtype)]
         h = C.c_void_p()
@@ -204,3 +217,14 @@ def as_slice(x) -> DeviceSlice:
 
 
 NULL_SLICE = L.MsvSlice(None, 0, 0)
+
+
+def segsort_configs():
+    """The seg::Sort tile-shape candidates: [(block, items_per_thread, digit_bits)]."""
+    lib = L.lib()
+    out = []
+    for k in range(lib.msv_segsort_config_count()):
+        b, i, d = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib.msv_segsort_config_info(k, C.byref(b), C.byref(i), C.byref(d)))
+        out.append((b.value, i.value, d.value))
+    return out

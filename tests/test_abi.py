@@ -87,3 +87,13 @@ def test_predicate_threshold_bits():
     assert p.a == (2**64 - 3)
     p = m.op.BitsEq(0x30, 0x10).c_struct(np.dtype(np.uint32))
     assert (p.kind, p.a, p.b) == (7, 0x30, 0x10)
+
+
+def test_host_mirror_surface():
+    """The Python mirror keeps the reference API surface (no GPU needed)."""
+    import massively_amd as m
+
+    for a in ("alloc", "to_device", "to_host", "wrap", "sync", "trim", "set_option", "get_option"):
+        assert hasattr(m.Executor, a), a
+    cfgs = m.segsort_configs()
+    assert len(cfgs) >= 2 and all(b % 64 == 0 and i > 0 and d in (4, 8) for b, i, d in cfgs)

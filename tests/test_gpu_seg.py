@@ -171,3 +171,71 @@ def test_batched_seg_config_2p26(exec_):
     absum[L == 0] = 0.0
     bound = (2 * np.ceil(np.log2(np.maximum(L, 2))) + 40) * 2.0**-24 * absum
     assert np.all(np.abs(r.astype(np.float64) - exact) <= bound)
+
+
+def _mixed_lengths():
+    # Tiny and empty segments, segments around every candidate's half tile and
+    # tile (512 .. 8192), and segments longer than any tile.
+    L = list(RNG.integers(0, 40, 2000))
+    for t in (512, 1024, 2048, 4096, 8192):
+        L += [t // 2 - 1, t // 2, t // 2 + 1, 0, t - 1, t, t + 1, 3]
+    L += [20000, 1, 0, 65536, 2, 9000]
+    L += list(lognormal_lengths(3000))
+    return L
+
+
+@pytest.mark.parametrize("launch", [m.Executor.SEG_LAUNCH_PERSISTENT, m.Executor.SEG_LAUNCH_PER_CLASS])
+@pytest.mark.parametrize("kdtype,vdtype", [(np.uint32, np.float32), (np.int32, None), (np.float64, np.uint64),
+                                           (np.uint64, np.uint32)])
+def test_seg_sort_every_tile_config(exec_, launch, kdtype, vdtype):
+    """Every tile-shape candidate and both launch forms give the oracle's order."""
+    off = offsets_of(_mixed_lengths())
+    n = int(off[-1])
+    if np.dtype(kdtype).kind == "f":
+        keys = np.round(RNG.standard_normal(n) * 50).astype(kdtype)
+        keys[::7] = -0.0
+    else:
+        keys = RNG.integers(0, 1 << 20, n).astype(kdtype)
+    vals = None if vdtype is None else RNG.integers(0, 2**32, n).astype(vdtype)
+    sg = seg.Segmentation.from_offsets(exec_, exec_.to_device(off))
+    dk = exec_.to_device(keys)
+    cols = dk if vals is None else (dk, exec_.to_device(vals))
+    ran = 0
+    try:
+        exec_.set_option(m.Executor.OPT_SEG_LAUNCH, launch)
+        for k in range(len(m.segsort_configs())):
+            exec_.set_option(m.Executor.OPT_SEGSORT_CONFIG, k)
+            for mode, cmp in (("less", m.op.Less), ("greater", m.op.Greater)):
+                try:
+                    out = seg.ForEachSegment(seg.Sort(cmp)).run(exec_, seg.SegmentIterator(cols, sg))
+                except m.Unsupported:
+                    break  # this tile shape does not fit the element width
+                perm = oracle.seg_sort_perm(keys, off, mode)
+                got = out.values if vals is None else out.values[0]
+                assert exec_.to_host(got).tobytes() == keys[perm].tobytes(), (k, mode)
+                if vals is not None:
+                    assert exec_.to_host(out.values[1]).tobytes() == vals[perm].tobytes(), (k, mode)
+                ran += 1
+    finally:
+        exec_.set_option(m.Executor.OPT_SEGSORT_CONFIG, -1)
+        exec_.set_option(m.Executor.OPT_SEG_LAUNCH, m.Executor.SEG_LAUNCH_PERSISTENT)
+    assert ran >= 2
+
+
+def test_seg_sort_config_2p26_autotuned(exec_):
+    """The batched-segment configuration: 2^26 u32 keys + f32 values in 2^20
+    log-normal segments, autotuned tile shape, persistent launch."""
+    S = 1 << 20
+    L = lognormal_lengths(S)
+    L = np.floor(L * ((1 << 26) / int(L.sum()))).astype(np.int64)
+    L[-1] += (1 << 26) - int(L.sum())
+    off = offsets_of(L)
+    n = int(off[-1])
+    keys = RNG.integers(0, 2**32, n, dtype=np.uint32)
+    vals = RNG.standard_normal(n).astype(np.float32)
+    sg = seg.Segmentation.from_offsets(exec_, exec_.to_device(off))
+    out = seg.ForEachSegment(seg.Sort(m.op.Less)).run(
+        exec_, seg.SegmentIterator((exec_.to_device(keys), exec_.to_device(vals)), sg))
+    perm = oracle.seg_sort_perm(keys, off, "less")
+    assert exec_.to_host(out.values[0]).tobytes() == keys[perm].tobytes()
+    assert exec_.to_host(out.values[1]).tobytes() == vals[perm].tobytes()
