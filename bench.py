@@ -258,6 +258,99 @@ def w_seg(c):
     return out
 
 
+def served_trace(np, jobs=500, seed=10):
+    """The served mixed stream: kinds uniform over sort_by_key / inclusive_scan /
+    reduce / copy_where, sizes log-uniform in 2^10..2^26, key dtypes uniform
+    over u32 / u64 / f32, one seeded input per job."""
+    from massively_amd import op, serve
+
+    rng = np.random.default_rng(seed)
+    reqs, bytes_ = [], []
+    for i in range(jobs):
+        kind = ("sort_by_key", "inclusive_scan", "reduce", "copy_where")[rng.integers(0, 4)]
+        dtype = (np.uint32, np.uint64, np.float32)[rng.integers(0, 3)]
+        n = int(round(2 ** rng.uniform(10, 26)))
+        es = np.dtype(dtype).itemsize
+        if kind == "sort_by_key":
+            r = serve.Request(kind, dtype, n, seed=1000 + i, op=op.Less)
+            passes = 2 * es
+            b = n * (es + passes * 2 * (es + 4))
+        elif kind == "inclusive_scan":
+            r = serve.Request(kind, dtype, n, seed=1000 + i, op=op.Add)
+            b = 2 * n * es
+        elif kind == "reduce":
+            r = serve.Request(kind, dtype, n, seed=1000 + i, op=op.Add)
+            b = n * es
+        else:
+            thr = 0.75 if dtype == np.float32 else (1 << (8 * es - 1))
+            r = serve.Request(kind, dtype, n, seed=1000 + i, pred=op.Lt(thr))
+            b = 1.5 * n * es
+        reqs.append(r)
+        bytes_.append(b)
+    return reqs, bytes_
+
+
+def run_served(np, mode, reqs, arrivals_us=None):
+    """Submits the requests (at the given arrival times, or all at once),
+    returns (makespan_us, latencies_us)."""
+    from massively_amd import serve
+
+    srv = serve.Server(streams=4, max_batch=256, mode=mode, hbm_budget=64 << 30)
+    try:
+        # Warm-up: one pass over a few requests (allocator and kernel caches).
+        for t in [srv.submit(r) for r in reqs[:16]]:
+            srv.wait(t)
+            srv.release(t)
+        t0 = srv.now_us()
+        tickets = []
+        for i, r in enumerate(reqs):
+            if arrivals_us is not None:
+                due = t0 + arrivals_us[i]
+                while srv.now_us() < due:
+                    pass
+            tickets.append(srv.submit(r))
+        lat, last = [], t0
+        for t in tickets:
+            info = srv.wait(t)
+            lat.append(info["done_us"] - info["submit_us"])
+            last = max(last, info["done_us"])
+            srv.release(t)
+        return last - t0, lat
+    finally:
+        srv.close()
+
+
+def w_served(c):
+    """Served mixed stream: capacity (all requests at once) and Poisson
+    arrivals at 80% of the one-launch-per-job capacity, for continuous
+    batching vs one request at a time through the device-wide kernels."""
+    np = c.np
+    reqs, bytes_ = served_trace(np)
+    total_bytes = float(sum(bytes_))
+    out = {"jobs": len(reqs), "algorithmic_bytes": total_bytes}
+    cap = {}
+    for mode in ("batched", "serial"):
+        span, lat = run_served(np, mode, reqs)
+        cap[mode] = len(reqs) / (span * 1e-6)
+        out[f"capacity_{mode}"] = dict(makespan_ms=round(span / 1e3, 3), jobs_per_s=round(cap[mode], 1),
+                                       gbs=round(total_bytes / (span * 1e-6) / 1e9, 1))
+        log(f"[bench] served capacity {mode}: {span / 1e3:.2f} ms for {len(reqs)} jobs = "
+            f"{cap[mode]:.0f} jobs/s, {total_bytes / (span * 1e-6) / 1e9:.1f} GB/s")
+    rate = 0.8 * cap["serial"]
+    rng = np.random.default_rng(10)
+    arrivals = np.cumsum(rng.exponential(1e6 / rate, len(reqs)))
+    out["poisson_rate_jobs_per_s"] = round(rate, 1)
+    for mode in ("batched", "serial"):
+        span, lat = run_served(np, mode, reqs, arrivals)
+        p50, p99 = np.percentile(lat, 50), np.percentile(lat, 99)
+        out[f"poisson_{mode}"] = dict(p50_us=round(float(p50), 1), p99_us=round(float(p99), 1),
+                                      mean_us=round(float(np.mean(lat)), 1),
+                                      jobs_per_s=round(len(reqs) / (span * 1e-6), 1))
+        log(f"[bench] served poisson {mode} @ {rate:.0f} jobs/s: p50 {p50:.0f} us, p99 {p99:.0f} us")
+    out["capacity_ratio_batched_over_serial"] = round(cap["batched"] / cap["serial"], 3)
+    return out
+
+
 def cpu_baseline(n_sample: int, reps: int):
     import oracle
 
@@ -331,6 +424,7 @@ WORKLOADS = {
     "sort": w_sort,
     "merge_sets": w_merge_sets,
     "seg": w_seg,
+    "served": w_served,
 }
 
 
@@ -373,6 +467,7 @@ def main() -> int:
             log(f"[bench] cpu sort: {extra['cpu_sort_baseline']['value']:.4f} Gkeys/s")
 
     sort_g = extra.get("sort", {}).get("u32_2^28", {}).get("gitems_per_s")
+    served = extra.get("served", {})
     line = {
         "metric": METRIC,
         "value": scan["gbs"],
@@ -392,6 +487,8 @@ def main() -> int:
                      "frac": scan["frac_hbm"], "traffic": load_traffic("pmc_inclusive_scan_u32_2^28")},
         "cpu_baseline": cpu,
         "sort_gkeys_per_s_u32_2^28": sort_g,
+        "served_p50_p99_us": [served.get("poisson_batched", {}).get("p50_us"),
+                              served.get("poisson_batched", {}).get("p99_us")] if served else None,
         "extra": extra,
     }
     print(json.dumps(line), flush=True)

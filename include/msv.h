@@ -276,6 +276,105 @@ msv_status msv_random_uniform_u64(msv_executor* ex, uint64_t min, uint64_t max, 
 /* out[i] = start + i (u32). */
 msv_status msv_iota_u32(msv_executor* ex, uint32_t start, msv_slice out);
 
+/* ---- batched jobs (no counterpart in the reference) -------------------
+ * For callers with many independent jobs (the serving loop below, or any
+ * batch of small requests): every job's inputs and outputs are caller-owned
+ * device memory. Jobs small enough for one workgroup (scan / reduce /
+ * copy_where up to 2^17 elements, sort_by_key up to 2^15) run as one
+ * persistent launch per job family; larger jobs run with the device-wide
+ * kernels. All work is enqueued on the executor's stream (no host sync). A
+ * job's result does not depend on which other jobs share the call. */
+typedef enum msv_job_kind {
+  MSV_JOB_INCLUSIVE_SCAN = 1, /* out[n] = inclusive_scan(in, op) */
+  MSV_JOB_EXCLUSIVE_SCAN = 2, /* out[n] = exclusive_scan(in, init, op) */
+  MSV_JOB_REDUCE = 3,         /* out[1] = reduce(in, init, op) */
+  MSV_JOB_COPY_WHERE = 4,     /* out[<= n] = in where pred(in) != invert; *(uint32_t*)out2 = count */
+  MSV_JOB_SORT_BY_KEY = 5     /* out2[n] = sorted keys, out[n] = values moved with them */
+} msv_job_kind;
+/* Key order of a sort job: radix total order (radix_sort_by_key), or the
+ * Less / Greater comparators (sort_by_key). */
+#define MSV_ORDER_TOTAL 0u
+#define MSV_ORDER_LESS 1u
+#define MSV_ORDER_GREATER 2u
+typedef struct msv_job {
+  uint32_t kind;        /* msv_job_kind */
+  uint32_t dtype;       /* element dtype (key dtype for sort jobs) */
+  uint32_t op;          /* msv_op for scans and reduce; MSV_ORDER_* for sorts */
+  uint32_t n;           /* input length */
+  uint32_t value_bytes; /* sort: 0 (keys only), 4 or 8 */
+  uint32_t invert;      /* copy_where: non-zero keeps the elements failing pred */
+  msv_predicate pred;   /* copy_where predicate on the input elements */
+  uint64_t init;        /* exclusive scan / reduce init (bit pattern of dtype) */
+  const void* in;       /* input (keys for sorts) */
+  const void* in2;      /* sort: values, or null */
+  void* out;            /* scan / copy_where: n elements; reduce: 1; sort: values */
+  void* out2;           /* sort: sorted keys; copy_where: uint32_t count */
+} msv_job;
+msv_status msv_run_jobs(msv_executor* ex, const msv_job* jobs, uint32_t count);
+
+/* ---- serving loop (no counterpart in the reference) -------------------
+ * A request loop on its own host thread: requests are admitted in arrival
+ * order while the device memory they need fits the HBM budget, their inputs
+ * are created on the device (from a seed, or copied from host memory), and
+ * admitted requests are batched continuously: every batch goes to the next
+ * free stream as soon as one is free, so requests that arrive while others
+ * run overlap with them on the device. Outputs stay device-resident until
+ * fetched and released. MSV_SERVE_SERIAL runs one request at a time through
+ * the device-wide kernels (one launch sequence per job), the form the batched
+ * mode is measured against. Results are the same in both modes for integer
+ * and sort jobs, and the same for one request alone or in any mix within a
+ * mode (floating-point scan/reduce of jobs small enough for the batched
+ * kernels follow that kernel's fixed order). */
+typedef struct msv_server msv_server;
+#define MSV_SERVE_BATCHED 0u
+#define MSV_SERVE_SERIAL 1u
+typedef struct msv_server_config {
+  int32_t device;
+  uint32_t streams;    /* streams (executors) batches rotate over, 1..16 */
+  uint32_t max_batch;  /* requests per batch, >= 1 */
+  uint32_t mode;       /* MSV_SERVE_BATCHED or MSV_SERVE_SERIAL */
+  uint64_t hbm_budget; /* bytes of device memory admitted requests may hold */
+} msv_server_config;
+typedef struct msv_request {
+  uint32_t kind;        /* msv_job_kind */
+  uint32_t dtype;       /* element / key dtype */
+  uint32_t op;          /* as msv_job */
+  uint32_t n;
+  uint32_t value_bytes; /* sort: 0, or 4 for a u32 payload = input position */
+  uint32_t invert;      /* copy_where */
+  msv_predicate pred;   /* copy_where */
+  uint64_t init;        /* exclusive scan / reduce */
+  uint64_t seed;        /* input = the library generator's uniform stream for dtype */
+  const void* host_input; /* if non-null: n elements copied in instead */
+} msv_request;
+typedef struct msv_request_info {
+  uint32_t state;        /* 0 queued, 1 running, 2 done, 3 failed */
+  msv_status status;
+  double submit_us;      /* server clock, microseconds */
+  double admit_us;
+  double done_us;
+  uint32_t out_len;      /* elements in the output (copy_where: the count) */
+  uint32_t reserved;
+  uint64_t bytes;        /* device bytes the request held while it ran */
+} msv_request_info;
+msv_status msv_server_create(const msv_server_config* cfg, msv_server** out);
+/* Waits for every admitted request, then frees everything. */
+msv_status msv_server_destroy(msv_server* s);
+msv_status msv_server_submit(msv_server* s, const msv_request* r, uint64_t* ticket);
+/* Blocks until the request is done (or failed) and fills *info. */
+msv_status msv_server_wait(msv_server* s, uint64_t ticket, msv_request_info* info);
+/* Copies a finished request's outputs to host memory: `out` gets the main
+ * output (scan / copy_where elements, the reduce value, sorted keys) and
+ * `out2` (may be null) the sorted values of a sort. */
+msv_status msv_server_result(msv_server* s, uint64_t ticket, void* out, uint64_t out_bytes,
+                             void* out2, uint64_t out2_bytes);
+/* Frees a finished request's outputs (returns its bytes to the budget). */
+msv_status msv_server_release(msv_server* s, uint64_t ticket);
+/* Waits until every submitted request is done. */
+msv_status msv_server_drain(msv_server* s);
+/* Server clock (microseconds since creation), the clock of the timestamps. */
+double msv_server_now_us(msv_server* s);
+
 #ifdef __cplusplus
 }
 #endif

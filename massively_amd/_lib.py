@@ -65,6 +65,61 @@ class MsvPredicate(C.Structure):
     ]
 
 
+class MsvJob(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("dtype", C.c_uint32),
+        ("op", C.c_uint32),
+        ("n", C.c_uint32),
+        ("value_bytes", C.c_uint32),
+        ("invert", C.c_uint32),
+        ("pred", MsvPredicate),
+        ("init", C.c_uint64),
+        ("in_", C.c_void_p),
+        ("in2", C.c_void_p),
+        ("out", C.c_void_p),
+        ("out2", C.c_void_p),
+    ]
+
+
+class MsvServerConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("streams", C.c_uint32),
+        ("max_batch", C.c_uint32),
+        ("mode", C.c_uint32),
+        ("hbm_budget", C.c_uint64),
+    ]
+
+
+class MsvRequest(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("dtype", C.c_uint32),
+        ("op", C.c_uint32),
+        ("n", C.c_uint32),
+        ("value_bytes", C.c_uint32),
+        ("invert", C.c_uint32),
+        ("pred", MsvPredicate),
+        ("init", C.c_uint64),
+        ("seed", C.c_uint64),
+        ("host_input", C.c_void_p),
+    ]
+
+
+class MsvRequestInfo(C.Structure):
+    _fields_ = [
+        ("state", C.c_uint32),
+        ("status", C.c_int32),
+        ("submit_us", C.c_double),
+        ("admit_us", C.c_double),
+        ("done_us", C.c_double),
+        ("out_len", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("bytes", C.c_uint64),
+    ]
+
+
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
 _S = MsvSlice
@@ -84,6 +139,15 @@ PROTOTYPES = {
     "msv_executor_trim": (_I, [_P]),
     "msv_executor_set_option": (_I, [_P, _I, C.c_int64]),
     "msv_executor_get_option": (_I, [_P, _I, C.POINTER(C.c_int64)]),
+    "msv_run_jobs": (_I, [_P, C.POINTER(MsvJob), _U32]),
+    "msv_server_create": (_I, [C.POINTER(MsvServerConfig), _PP]),
+    "msv_server_destroy": (_I, [_P]),
+    "msv_server_submit": (_I, [_P, C.POINTER(MsvRequest), C.POINTER(_U64)]),
+    "msv_server_wait": (_I, [_P, _U64, C.POINTER(MsvRequestInfo)]),
+    "msv_server_result": (_I, [_P, _U64, _P, _U64, _P, _U64]),
+    "msv_server_release": (_I, [_P, _U64]),
+    "msv_server_drain": (_I, [_P]),
+    "msv_server_now_us": (C.c_double, [_P]),
     "msv_segsort_config_count": (_U32, []),
     "msv_segsort_config_info": (_I, [_U32, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32)]),
     "msv_alloc": (_I, [_P, _I, _U32, _PP]),

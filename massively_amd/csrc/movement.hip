@@ -148,6 +148,33 @@ msv_status scatter_run(Ctx& c, msv_dtype dt, const void* values, const uint32_t*
   return MSV_OK;
 }
 
+msv_status uniform_run(Ctx& c, msv_dtype dt, void* out, uint32_t n, uint64_t seed) {
+  if (n == 0) return MSV_OK;
+  const dim3 grid((n + 255) / 256), block(256);
+  switch (dt) {
+    case MSV_U32:
+    case MSV_I32:
+      hipLaunchKernelGGL(uniform_u32_kernel, grid, block, 0, c.stream(), (uint32_t*)out, n, 0u,
+                         seed_key(seed), 0u, 0xffffffffu);
+      break;
+    case MSV_F32:  // bit patterns of [0.5, 1)
+      hipLaunchKernelGGL(uniform_u32_kernel, grid, block, 0, c.stream(), (uint32_t*)out, n, 0u,
+                         seed_key(seed), 0x3F000000u, 0x007FFFFFu);
+      break;
+    case MSV_U64:
+    case MSV_I64:
+      hipLaunchKernelGGL(uniform_u64_kernel, grid, block, 0, c.stream(), (uint64_t*)out, n, 0u,
+                         seed_key(seed), 0ull, ~0ull);
+      break;
+    case MSV_F64:  // bit patterns of [0.5, 1)
+      hipLaunchKernelGGL(uniform_u64_kernel, grid, block, 0, c.stream(), (uint64_t*)out, n, 0u,
+                         seed_key(seed), 0x3FE0000000000000ull, 0x000FFFFFFFFFFFFFull);
+      break;
+  }
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
 msv_status iota_run(Ctx& c, uint32_t* out, uint32_t n, uint32_t start) {
   if (n == 0) return MSV_OK;
   hipLaunchKernelGGL(iota_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream(), out, n,

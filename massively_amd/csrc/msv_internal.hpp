@@ -162,14 +162,29 @@ msv_status scan_run(Ctx& c, msv_dtype dt, msv_op op, bool exclusive, const void*
                     uint32_t n, const void* init_host);
 msv_status reduce_run(Ctx& c, msv_dtype dt, msv_op op, const void* in, uint32_t n,
                       const void* init_host, void* out_host);
+// reduce into device memory (no host sync).
+msv_status reduce_run_device(Ctx& c, msv_dtype dt, msv_op op, const void* in, uint32_t n,
+                             const void* init_host, void* d_out);
+// copy_where with a built-in predicate on the input itself; the count goes to
+// the device word out_len.
+msv_status select_pred_run(Ctx& c, msv_dtype dt, const void* in, const msv_predicate* p,
+                           bool invert, uint32_t n, void* out, uint32_t* out_len);
 msv_status gather_run(Ctx& c, msv_dtype dt, const void* values, uint32_t vlen,
                       const uint32_t* idx, void* out, uint32_t n);
 msv_status scatter_run(Ctx& c, msv_dtype dt, const void* values, const uint32_t* idx,
                        uint32_t n, void* out, uint32_t olen);
 msv_status iota_run(Ctx& c, uint32_t* out, uint32_t n, uint32_t start);
+// Seeded synthetic input of the library generator: full-range bits for
+// integers, [0.5, 1) for floats.
+msv_status uniform_run(Ctx& c, msv_dtype dt, void* out, uint32_t n, uint64_t seed);
+// Every job through the device-wide kernels, one after another (the
+// one-launch-per-job form the batched path is compared with).
+msv_status run_jobs_plain(Ctx& c, const msv_job* jobs, uint32_t count);
 // mode: 0 total order, 1 Less, 2 Greater; vsize 0 (keys only), 4 or 8.
 msv_status radix_sort_run(Ctx& c, msv_dtype kdt, int mode, size_t vsize, const void* keys,
                           const void* vals, uint32_t n, void* out_k, void* out_v);
+// Enqueues a batch of validated-on-entry jobs (msv_run_jobs).
+msv_status run_jobs(Ctx& c, const msv_job* jobs, uint32_t count);
 // Stable sort of every segment of offsets[0..nseg] on its own (mode 1 Less,
 // 2 Greater); vsize 0, 4 or 8.
 msv_status seg_sort_run(Ctx& c, msv_dtype kdt, int mode, const void* keys, const void* vals,

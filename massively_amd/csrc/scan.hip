@@ -156,6 +156,22 @@ static msv_status reduce_by_op(Ctx& c, msv_op op, const void* in, uint32_t n, co
   return MSV_ERR_INVALID_ARGUMENT;
 }
 
+template <typename T, int OP>
+static msv_status reduce_typed_device(Ctx& c, const void* in, uint32_t n, const void* init_host,
+                                      void* d_out) {
+  T init;
+  memcpy(&init, init_host, sizeof(T));
+  return reduce_impl<T, OP>(c, (const T*)in, n, init, (T*)d_out);
+}
+
+template <typename T>
+static msv_status reduce_device_by_op(Ctx& c, msv_op op, const void* in, uint32_t n,
+                                      const void* init, void* d_out) {
+  MSV_DISPATCH_OP(T, reduce_typed_device, c, in, n, init, d_out)
+  set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "operator %d is not defined for this dtype", (int)op);
+  return MSV_ERR_INVALID_ARGUMENT;
+}
+
 #define MSV_DISPATCH_DTYPE(FN, ...)                         \
   switch (dt) {                                             \
     case MSV_U32: return FN<uint32_t>(__VA_ARGS__);         \
@@ -176,6 +192,13 @@ msv_status scan_run(Ctx& c, msv_dtype dt, msv_op op, bool exclusive, const void*
 msv_status reduce_run(Ctx& c, msv_dtype dt, msv_op op, const void* in, uint32_t n,
                       const void* init_host, void* out_host) {
   MSV_DISPATCH_DTYPE(reduce_by_op, c, op, in, n, init_host, out_host)
+  set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
+  return MSV_ERR_INVALID_ARGUMENT;
+}
+
+msv_status reduce_run_device(Ctx& c, msv_dtype dt, msv_op op, const void* in, uint32_t n,
+                             const void* init_host, void* d_out) {
+  MSV_DISPATCH_DTYPE(reduce_device_by_op, c, op, in, n, init_host, d_out)
   set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
   return MSV_ERR_INVALID_ARGUMENT;
 }

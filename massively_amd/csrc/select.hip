@@ -13,37 +13,13 @@
 // total count into the output's device-resident length (no host sync).
 #include "device_common.hpp"
 #include "msv_internal.hpp"
+#include "predicate.hpp"
 
 namespace msv {
 
 constexpr int kSelBlock = 256;
 constexpr int kSelRows = 4;
 constexpr uint32_t kSelTile = kSelBlock * kSelRows * 4;
-
-// Built-in predicates (massively/src/core/predicate.rs:40-52 built-ins).
-enum PredKind : uint32_t {
-  PRED_NONZERO = 0, PRED_ZERO, PRED_LT, PRED_LE, PRED_GT, PRED_GE, PRED_BITS_ANY, PRED_BITS_EQ
-};
-
-template <typename S>
-struct Pred {
-  uint32_t kind;
-  S a;
-  typename Bits<S>::U ua, ub;
-  __device__ inline bool operator()(S x) const {
-    using U = typename Bits<S>::U;
-    switch (kind) {
-      case PRED_NONZERO: return x != (S)0;
-      case PRED_ZERO: return x == (S)0;
-      case PRED_LT: return x < a;
-      case PRED_LE: return x <= a;
-      case PRED_GT: return x > a;
-      case PRED_GE: return x >= a;
-      case PRED_BITS_ANY: return (bit_cast<U>(x) & ua) != 0;
-      default: return (bit_cast<U>(x) & ua) == ub;
-    }
-  }
-};
 
 template <typename T>
 __device__ inline void load4(const T* __restrict__ p, size_t i, uint32_t n, bool full, T (&v)[4]) {
@@ -182,16 +158,7 @@ static msv_status select_launch(Ctx& c, const void* in, const void* st, const ms
     MSV_HIP(hipGetLastError());
     return MSV_OK;
   }
-  Pred<S> pred{};
-  if (p) {
-    using U = typename Bits<S>::U;
-    pred.kind = p->kind;
-    pred.ua = (U)p->a;
-    pred.ub = (U)p->b;
-    pred.a = bit_cast<S>((U)p->a);
-  } else {
-    pred.kind = PRED_NONZERO;
-  }
+  const Pred<S> pred = make_pred<S>(p);
   const uint32_t ntiles = (n + kSelTile - 1) / kSelTile;
   uint64_t *counter, *status;
   uint32_t gen;
@@ -216,6 +183,21 @@ static msv_status select_with_stencil_type(Ctx& c, size_t vsize, bool same, cons
   }
   if (vsize == 4) return select_launch<uint32_t, S, 1>(c, in, st, p, invert, n, out, out_len);
   return select_launch<uint64_t, S, 1>(c, in, st, p, invert, n, out, out_len);
+}
+
+msv_status select_pred_run(Ctx& c, msv_dtype dt, const void* in, const msv_predicate* p,
+                           bool invert, uint32_t n, void* out, uint32_t* out_len) {
+  const size_t vs = dtype_size(dt);
+  switch (dt) {
+    case MSV_U32: return select_with_stencil_type<uint32_t>(c, vs, true, in, in, p, invert, n, out, out_len);
+    case MSV_I32: return select_with_stencil_type<int32_t>(c, vs, true, in, in, p, invert, n, out, out_len);
+    case MSV_F32: return select_with_stencil_type<float>(c, vs, true, in, in, p, invert, n, out, out_len);
+    case MSV_U64: return select_with_stencil_type<uint64_t>(c, vs, true, in, in, p, invert, n, out, out_len);
+    case MSV_I64: return select_with_stencil_type<int64_t>(c, vs, true, in, in, p, invert, n, out, out_len);
+    case MSV_F64: return select_with_stencil_type<double>(c, vs, true, in, in, p, invert, n, out, out_len);
+  }
+  set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
+  return MSV_ERR_INVALID_ARGUMENT;
 }
 
 }  // namespace msv
