@@ -147,6 +147,23 @@ typedef enum msv_option {
 #define MSV_SEG_LAUNCH_PER_CLASS 1
 msv_status msv_executor_set_option(msv_executor* ex, msv_option opt, int64_t value);
 msv_status msv_executor_get_option(msv_executor* ex, msv_option opt, int64_t* value);
+/* HIP graph capture of library calls (no counterpart in the reference): the
+ * calls made on `ex` between begin and end are recorded, not run, and
+ * msv_graph_launch replays them on the executor's stream. Each replay reads
+ * the buffers the calls were given at capture time (write new inputs into
+ * them between replays) and writes the same output vectors. Scratch memory
+ * and lookback state of the captured calls belong to the graph (cleared by
+ * kernel nodes on every replay, never by memset nodes). Calls that need a
+ * host synchronisation (to_host, to_device, sync, trim, reduce with a host
+ * result, Segmentation constructors, whole-length slices of vectors with a
+ * device-resident length) fail with MSV_ERR_UNSUPPORTED while capturing. */
+typedef struct msv_graph msv_graph;
+msv_status msv_executor_begin_capture(msv_executor* ex);
+msv_status msv_executor_end_capture(msv_executor* ex, msv_graph** out);
+msv_status msv_graph_launch(msv_graph* g);
+/* Waits for the executor's stream, then frees the graph and its memory. */
+msv_status msv_graph_destroy(msv_graph* g);
+
 /* Number of seg::Sort tile-shape candidates, and candidate k's shape. */
 uint32_t msv_segsort_config_count(void);
 msv_status msv_segsort_config_info(uint32_t k, uint32_t* block, uint32_t* items_per_thread,

@@ -139,6 +139,10 @@ PROTOTYPES = {
     "msv_executor_trim": (_I, [_P]),
     "msv_executor_set_option": (_I, [_P, _I, C.c_int64]),
     "msv_executor_get_option": (_I, [_P, _I, C.POINTER(C.c_int64)]),
+    "msv_executor_begin_capture": (_I, [_P]),
+    "msv_executor_end_capture": (_I, [_P, _PP]),
+    "msv_graph_launch": (_I, [_P]),
+    "msv_graph_destroy": (_I, [_P]),
     "msv_run_jobs": (_I, [_P, C.POINTER(MsvJob), _U32]),
     "msv_server_create": (_I, [C.POINTER(MsvServerConfig), _PP]),
     "msv_server_destroy": (_I, [_P]),

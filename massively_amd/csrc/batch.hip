@@ -411,8 +411,10 @@ msv_status run_jobs(Ctx& c, const msv_job* jobs, uint32_t count) {
       it.s1 = scratch + off + bytes;
       off += 2 * bytes;
     }
-    MSV_HIP(hipMemcpyAsync(d_items, items.data(), sizeof(BatchItem) * items.size(),
-                           hipMemcpyHostToDevice, c.stream()));
+    const void* src = c.staging(items.data(), sizeof(BatchItem) * items.size());
+    if (!src) return MSV_ERR_OUT_OF_MEMORY;
+    MSV_HIP(hipMemcpyAsync(d_items, src, sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice,
+                           c.stream()));
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, c.stream(), queues, 8u, 0u);
     static int per_cu[7] = {0};
     MSV_TRY(launch_family(c, batch_stream_kernel, d_items + start[0], start[1] - start[0],

@@ -94,6 +94,11 @@ struct msv_executor {
   int64_t opt_seg_launch = MSV_SEG_LAUNCH_PERSISTENT;
   int64_t opt_autotune = 1;
   int64_t opt_segsort_config = -1;
+  // HIP graph capture (msv_executor_begin_capture): while capturing, call
+  // scratch and pinned staging are kept for the graph's lifetime.
+  bool capturing = false;
+  std::vector<void*> capture_keep;
+  std::vector<void*> capture_pinned;
   explicit msv_executor(int dev) : allocator(dev) {}
 };
 
@@ -128,6 +133,10 @@ struct View {
   msv_dtype dtype = MSV_U32;
 };
 
+// Fails with MSV_ERR_UNSUPPORTED when the executor is capturing a graph (for
+// calls that need a host synchronisation).
+msv_status no_capture(msv_executor* ex, const char* what);
+
 // Validates ownership/range and resolves the length (synchronising only for a
 // MSV_WHOLE slice of a vector whose length is device-resident).
 msv_status resolve(msv_executor* ex, const msv_slice& s, View* out);
@@ -147,6 +156,9 @@ class Ctx {
   msv_executor* ex() const { return ex_; }
   // nullptr + error on failure.
   void* scratch(size_t bytes);
+  // Host staging for a host-to-device copy: the caller's memory when not
+  // capturing, else a pinned copy kept for the graph's lifetime.
+  const void* staging(const void* host, size_t bytes);
   // Lookback state sized for `words` status words, and a fresh generation
   // for one launch.
   msv_status lookback(size_t words, uint64_t** counter, uint64_t** status, uint32_t* gen);

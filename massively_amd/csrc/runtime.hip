@@ -104,6 +104,15 @@ void executor_release(msv_executor* ex) {
   delete ex;
 }
 
+msv_status no_capture(msv_executor* ex, const char* what) {
+  if (ex && ex->capturing) {
+    set_error(MSV_ERR_UNSUPPORTED, 0, 0, "%s needs a host synchronisation; not allowed while "
+              "capturing a graph", what);
+    return MSV_ERR_UNSUPPORTED;
+  }
+  return MSV_OK;
+}
+
 // ---- views ------------------------------------------------------------------
 msv_status resolve(msv_executor* ex, const msv_slice& s, View* out) {
   if (!ex) {
@@ -122,6 +131,7 @@ msv_status resolve(msv_executor* ex, const msv_slice& s, View* out) {
   }
   uint32_t logical = v->host_len;
   if (v->d_len && s.len == MSV_WHOLE) {
+    MSV_TRY(no_capture(ex, "a whole-length slice of a vector with a device-resident length"));
     MSV_HIP(hipSetDevice(ex->device));
     MSV_HIP(hipMemcpyAsync(&logical, v->d_len, sizeof(uint32_t), hipMemcpyDeviceToHost,
                            ex->stream));
@@ -187,12 +197,43 @@ Ctx::~Ctx() {
 
 void* Ctx::scratch(size_t bytes) {
   void* p = ex_->allocator.alloc(bytes);
-  if (p) scratch_.push_back(p);
+  if (p) (ex_->capturing ? ex_->capture_keep : scratch_).push_back(p);
   return p;
+}
+
+const void* Ctx::staging(const void* host, size_t bytes) {
+  if (!ex_->capturing) return host;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error(MSV_ERR_OUT_OF_MEMORY, bytes, 0, "pinned staging allocation failed");
+    return nullptr;
+  }
+  memcpy(p, host, bytes);
+  ex_->capture_pinned.push_back(p);
+  return p;
+}
+
+static __global__ void zero_u64_kernel(uint64_t* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 msv_status Ctx::lookback(size_t words, uint64_t** counter, uint64_t** status, uint32_t* gen) {
   msv_executor* ex = ex_;
+  if (ex->capturing) {
+    // A captured launch gets its own counter and status words, cleared by a
+    // kernel node before it on every replay, and a fixed generation.
+    uint64_t* p = (uint64_t*)scratch((words + 1) * 8);
+    if (!p) return MSV_ERR_OUT_OF_MEMORY;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)((words + 1 + 255) / 256)), dim3(256), 0,
+                       ex->stream, p, words + 1);
+    MSV_HIP(hipGetLastError());
+    *counter = p;
+    *status = p + 1;
+    *gen = 1;
+    return MSV_OK;
+  }
   // Generations are 29 bits; before one would repeat, every status word is
   // cleared so no stale word can match a live generation.
   if (ex->lookback_gen >= (1u << 29) - 1) {
@@ -293,6 +334,7 @@ msv_status msv_executor_sync(msv_executor* ex) {
     set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null executor");
     return MSV_ERR_INVALID_ARGUMENT;
   }
+  MSV_TRY(no_capture(ex, "Executor::sync"));
   MSV_HIP(hipSetDevice(ex->device));
   MSV_HIP(hipStreamSynchronize(ex->stream));
   return MSV_OK;
@@ -308,9 +350,87 @@ msv_status msv_executor_trim(msv_executor* ex) {
     set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null executor");
     return MSV_ERR_INVALID_ARGUMENT;
   }
+  MSV_TRY(no_capture(ex, "trim"));
   MSV_HIP(hipSetDevice(ex->device));
   MSV_HIP(hipStreamSynchronize(ex->stream));
   ex->allocator.trim();
+  return MSV_OK;
+}
+
+// ---- graph capture ------------------------------------------------------------
+struct msv_graph {
+  msv_executor* ex;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  std::vector<void*> keep;
+  std::vector<void*> pinned;
+};
+
+msv_status msv_executor_begin_capture(msv_executor* ex) {
+  clear_error();
+  if (!ex || ex->capturing) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null executor or already capturing");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  MSV_HIP(hipSetDevice(ex->device));
+  MSV_HIP(hipStreamBeginCapture(ex->stream, hipStreamCaptureModeRelaxed));
+  ex->capturing = true;
+  return MSV_OK;
+}
+
+msv_status msv_executor_end_capture(msv_executor* ex, msv_graph** out) {
+  clear_error();
+  if (!ex || !ex->capturing || !out) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "executor is not capturing or null output");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  ex->capturing = false;
+  auto* g = new msv_graph();
+  g->ex = ex;
+  g->keep.swap(ex->capture_keep);
+  g->pinned.swap(ex->capture_pinned);
+  hipError_t e = hipStreamEndCapture(ex->stream, &g->graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    for (void* p : g->keep) ex->allocator.free(p);
+    for (void* p : g->pinned) (void)hipHostFree(p);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    set_error(MSV_ERR_LAUNCH, 0, 0, "graph capture failed: %s", hipGetErrorString(e));
+    return MSV_ERR_LAUNCH;
+  }
+  executor_retain(ex);
+  *out = g;
+  return MSV_OK;
+}
+
+msv_status msv_graph_launch(msv_graph* g) {
+  clear_error();
+  if (!g) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null graph");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  MSV_HIP(hipSetDevice(g->ex->device));
+  MSV_HIP(hipGraphLaunch(g->exec, g->ex->stream));
+  return MSV_OK;
+}
+
+msv_status msv_graph_destroy(msv_graph* g) {
+  clear_error();
+  if (!g) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null graph");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  msv_executor* ex = g->ex;
+  (void)hipSetDevice(ex->device);
+  (void)hipStreamSynchronize(ex->stream);  // no replay may still use the memory
+  (void)hipGraphExecDestroy(g->exec);
+  (void)hipGraphDestroy(g->graph);
+  for (void* p : g->keep) ex->allocator.free(p);
+  for (void* p : g->pinned) (void)hipHostFree(p);
+  delete g;
+  executor_release(ex);
   return MSV_OK;
 }
 
@@ -366,6 +486,7 @@ msv_status msv_alloc(msv_executor* ex, msv_dtype dtype, uint32_t len, msv_vec** 
 msv_status msv_to_device(msv_executor* ex, msv_dtype dtype, const void* host, uint32_t len,
                          msv_vec** out) {
   clear_error();
+  MSV_TRY(no_capture(ex, "to_device"));
   if (!ex || !dtype_valid(dtype) || (len && !host)) {
     set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null executor/host pointer or bad dtype");
     return MSV_ERR_INVALID_ARGUMENT;
@@ -388,6 +509,7 @@ msv_status msv_to_device(msv_executor* ex, msv_dtype dtype, const void* host, ui
 msv_status msv_to_host(msv_executor* ex, msv_slice src, void* host, uint32_t host_cap,
                        uint32_t* out_len) {
   clear_error();
+  MSV_TRY(no_capture(ex, "to_host"));
   View v;
   MSV_TRY(resolve(ex, src, &v));
   uint32_t len = v.len;

@@ -120,6 +120,7 @@ static msv_status scan_typed(Ctx& c, bool exclusive, const void* in, void* out, 
 template <typename T, int OP>
 static msv_status reduce_typed(Ctx& c, const void* in, uint32_t n, const void* init_host,
                                void* out_host) {
+  MSV_TRY(no_capture(c.ex(), "reduce (host result)"));
   T init;
   memcpy(&init, init_host, sizeof(T));
   T* d_result = (T*)c.scratch(sizeof(T));

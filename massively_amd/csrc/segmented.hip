@@ -473,6 +473,7 @@ extern "C" {
 msv_status msv_segmentation_from_offsets(msv_executor* ex, msv_slice offsets,
                                          msv_segmentation** out) {
   clear_error();
+  MSV_TRY(no_capture(ex, "Segmentation::from_offsets"));
   View o;
   MSV_TRY(resolve(ex, offsets, &o));
   if (!out || o.dtype != MSV_U32) {
@@ -522,6 +523,7 @@ msv_status msv_segmentation_from_offsets(msv_executor* ex, msv_slice offsets,
 msv_status msv_segmentation_from_lengths(msv_executor* ex, msv_slice lengths,
                                          msv_segmentation** out) {
   clear_error();
+  MSV_TRY(no_capture(ex, "Segmentation::from_lengths"));
   View l;
   MSV_TRY(resolve(ex, lengths, &l));
   if (!out || l.dtype != MSV_U32) {

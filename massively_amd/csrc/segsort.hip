@@ -233,14 +233,15 @@ static msv_status segsort_dispatch(Ctx& c, const SegSortArgs& a, uint32_t n, uin
   msv_executor* ex = c.ex();
   if (ex->opt_segsort_config >= 0)
     return segsort_config_go<KW, EW>(c, (int)ex->opt_segsort_config, a, n, nseg, offsets);
-  if (!ex->opt_autotune || n < (1u << 16))
-    return segsort_config_go<KW, EW>(c, kSegSortDefault, a, n, nseg, offsets);
   const uint32_t key = tune_key(KW, EW, ex->opt_seg_launch, n, nseg);
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) return segsort_config_go<KW, EW>(c, it->second, a, n, nseg, offsets);
   }
+  // Timing needs host synchronisation, so a graph capture uses the default.
+  if (!ex->opt_autotune || n < (1u << 16) || ex->capturing)
+    return segsort_config_go<KW, EW>(c, kSegSortDefault, a, n, nseg, offsets);
   // Time every candidate that fits (each run is a complete, correct sort of
   // this input; the last one timed leaves its result in the output).
   hipEvent_t e0, e1;

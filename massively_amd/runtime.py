@@ -58,6 +58,11 @@ class Executor:
     def trim(self) -> None:
         check(self._lib.msv_executor_trim(self.handle))
 
+    def capture(self) -> "GraphCapture":
+        """``with ex.capture() as g: ...`` records the library calls made
+        inside the block into a HIP graph; ``g.graph.launch()`` replays them."""
+        return GraphCapture(self)
+
     # Tuning options of the batching layer (results never depend on them).
     OPT_SEG_LAUNCH, OPT_AUTOTUNE, OPT_SEGSORT_CONFIG = 1, 2, 3
     SEG_LAUNCH_PERSISTENT, SEG_LAUNCH_PER_CLASS = 0, 1
@@ -228,3 +233,46 @@ def segsort_configs():
         check(lib.msv_segsort_config_info(k, C.byref(b), C.byref(i), C.byref(d)))
         out.append((b.value, i.value, d.value))
     return out
+
+
+class Graph:
+    """A captured sequence of library calls (``msv_graph``)."""
+
+    def __init__(self, executor: Executor, handle: C.c_void_p):
+        self.executor = executor
+        self._h = handle
+        self._lib = executor._lib
+
+    def launch(self) -> None:
+        check(self._lib.msv_graph_launch(self._h))
+
+    def close(self) -> None:
+        if self._h is not None:
+            check(self._lib.msv_graph_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GraphCapture:
+    def __init__(self, executor: Executor):
+        self.executor = executor
+        self.graph: Graph | None = None
+
+    def __enter__(self) -> "GraphCapture":
+        check(self.executor._lib.msv_executor_begin_capture(self.executor.handle))
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = C.c_void_p()
+        st = self.executor._lib.msv_executor_end_capture(self.executor.handle, C.byref(h))
+        if exc_type is None:
+            check(st)
+            self.graph = Graph(self.executor, h)
+        elif st == 0:
+            Graph(self.executor, h).close()
+        return False
