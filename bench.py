@@ -249,11 +249,22 @@ def w_seg(c):
     its = seg.SegmentIterator((x, ff), sg)
     # seg::Sort: one persistent launch over all size classes (tile shape chosen
     # by the autotuner on the warm-up call) vs one launch per size class.
+    run_sort = lambda: seg.ForEachSegment(seg.Sort(m.op.Less)).run(c.ex, its)
     for launch, tag in ((c.ex.SEG_LAUNCH_PERSISTENT, ""), (c.ex.SEG_LAUNCH_PER_CLASS, "_per_class")):
         c.ex.set_option(c.ex.OPT_SEG_LAUNCH, launch)
-        ms = timed(c.ex, lambda: seg.ForEachSegment(seg.Sort(m.op.Less)).run(c.ex, its), c.steps, 1)
-        out["sort_u32_f32" + tag] = summarize("seg_sort_u32key_f32val_2^26_in_2^20" + tag, ms,
-                                              16.0 * n + 4.0 * (S + 1), n)
+        ms = timed(c.ex, run_sort, c.steps, 1)
+        d = summarize("seg_sort_u32key_f32val_2^26_in_2^20" + tag, ms, 16.0 * n + 4.0 * (S + 1), n)
+        d["tuned_config"] = m.segsort_configs()[max(0, m.segsort_tuned_config(4, 4, launch, n, S))]
+        # Every candidate tile shape, forced (the autotuner's table).
+        cands = {}
+        for k, cfg in enumerate(m.segsort_configs()):
+            c.ex.set_option(c.ex.OPT_SEGSORT_CONFIG, k)
+            cms = timed(c.ex, run_sort, 3, 1)
+            cands["x".join(map(str, cfg))] = round(statistics.fmean(cms), 4)
+        c.ex.set_option(c.ex.OPT_SEGSORT_CONFIG, -1)
+        d["candidates_ms"] = cands
+        log(f"[bench]   tuned {d['tuned_config']}, candidates (block x ipt x digit bits): {cands}")
+        out["sort_u32_f32" + tag] = d
     c.ex.set_option(c.ex.OPT_SEG_LAUNCH, c.ex.SEG_LAUNCH_PERSISTENT)
     return out
 

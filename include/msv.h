@@ -168,6 +168,10 @@ msv_status msv_graph_destroy(msv_graph* g);
 uint32_t msv_segsort_config_count(void);
 msv_status msv_segsort_config_info(uint32_t k, uint32_t* block, uint32_t* items_per_thread,
                                    uint32_t* digit_bits);
+/* The candidate the autotuner chose for this shape (key/value bytes, launch
+ * form, total elements, segments), or -1 if it has not run for it. */
+int32_t msv_segsort_tuned_config(uint32_t key_bytes, uint32_t value_bytes, int64_t launch,
+                                 uint32_t n, uint32_t nseg);
 
 /* ---- device vectors (runtime.rs:79-177, allocation.rs:852-861) -------- */
 /* Executor::alloc: uninitialised vector of `len` elements. */

@@ -17,7 +17,7 @@ from .error import (
     UnboundColumn,
     Unsupported,
 )
-from .runtime import DeviceSlice, DeviceVec, Executor, segsort_configs
+from .runtime import DeviceSlice, DeviceVec, Executor, segsort_configs, segsort_tuned_config
 
 __all__ = [
     "Executor",
@@ -32,6 +32,7 @@ __all__ = [
     "ForeignExecutor",
     "Unsupported",
     "segsort_configs",
+    "segsort_tuned_config",
     "op",
     "seg",
     "vector",

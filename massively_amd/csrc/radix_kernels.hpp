@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kHistBlock) void radix_hist_kernel(const K* __restr
 //    histogram has a single bin of count n is skipped) and the source and
 //    destination of each remaining pass: caller arrays -> element buffer A0
 //    -> A1 -> A0 ... -> caller arrays.
-template <int P>
+template <int P, bool SOA = false>
 __global__ __launch_bounds__(kRadixBins) void radix_plan_kernel(
     const uint32_t* __restrict__ g_hist, uint32_t n, RadixPass* __restrict__ table,
     const void* in_k, const void* in_v, void* out_k, void* out_v, void* aos0, void* aos1) {
@@ -127,19 +127,24 @@ __global__ __launch_bounds__(kRadixBins) void radix_plan_kernel(
       rp.src_k = rp.src_v = nullptr;
       rp.dst_k = rp.dst_v = nullptr;
       if (rp.active) {
-        rp.in_aos = j > 0;
-        rp.out_aos = j < active - 1;
+        // SOA: an intermediate buffer holds the n keys followed by the values.
+        rp.in_aos = !SOA && j > 0;
+        rp.out_aos = !SOA && j < active - 1;
         if (j == 0) {
           rp.src_k = in_k;
           rp.src_v = in_v;
         } else {
-          rp.src_k = ((j - 1) & 1) ? aos1 : aos0;
+          char* b = (char*)(((j - 1) & 1) ? aos1 : aos0);
+          rp.src_k = b;
+          rp.src_v = SOA ? b + (size_t)n * P : nullptr;
         }
         if (j == active - 1) {
           rp.dst_k = out_k;
           rp.dst_v = out_v;
         } else {
-          rp.dst_k = (j & 1) ? aos1 : aos0;
+          char* b = (char*)((j & 1) ? aos1 : aos0);
+          rp.dst_k = b;
+          rp.dst_v = SOA ? b + (size_t)n * P : nullptr;
         }
         ++j;
       }

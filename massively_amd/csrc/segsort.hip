@@ -310,6 +310,14 @@ extern "C" {
 
 uint32_t msv_segsort_config_count(void) { return msv::kSegSortNumConfigs; }
 
+int32_t msv_segsort_tuned_config(uint32_t key_bytes, uint32_t value_bytes, int64_t launch,
+                                 uint32_t n, uint32_t nseg) {
+  const int kw = (int)key_bytes / 4, ew = kw + (int)value_bytes / 4;
+  std::lock_guard<std::mutex> g(msv::g_tune_mu);
+  auto it = msv::g_tune.find(msv::tune_key(kw, ew, launch, n, nseg));
+  return it == msv::g_tune.end() ? -1 : it->second;
+}
+
 msv_status msv_segsort_config_info(uint32_t k, uint32_t* block, uint32_t* items_per_thread,
                                    uint32_t* digit_bits) {
   msv::clear_error();

@@ -126,6 +126,39 @@ __device__ __forceinline__ void digit_scan_wave0(uint32_t (*h)[BINS], uint32_t* 
   }
 }
 
+// Digit-major exclusive scan of h[wave][digit] by the first BINS threads
+// (thread t owns digit t): afterwards h[w][d] = (elements of digits < d) +
+// (elements of digit d in waves < w). Spreads the work of digit_scan_wave0
+// over BINS / 64 waves at the cost of one more barrier. All threads call it.
+template <int NW, int BINS, int BLOCK>
+__device__ __forceinline__ void digit_scan_block(uint32_t (*h)[BINS], uint32_t* s_wsum) {
+  static_assert(BLOCK >= BINS, "one thread per digit");
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+  constexpr uint32_t DW = (BINS + kWave - 1) / kWave;  // waves holding digits
+  const bool dthread = tid < (uint32_t)BINS;
+  uint32_t c = 0;
+  if (dthread) {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t x = h[w][tid];
+      h[w][tid] = c;
+      c += x;
+    }
+  }
+  uint32_t incl = 0;
+  if (wid < DW) {
+    incl = wave_inclusive_scan<uint32_t, OP_ADD>(c);
+    if (lane == 63) s_wsum[wid] = incl;
+  }
+  __syncthreads();
+  if (dthread) {
+    uint32_t start = incl - c;
+    for (uint32_t k = 0; k < wid; ++k) start += s_wsum[k];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) h[w][tid] += start;
+  }
+}
+
 // One stable counting pass over s.el[0..n) in LDS by the digit at `shift` of
 // the key's order word (or of the slot when `slot_digit`). Items of wave w are
 // [w * 64 * IPT, (w + 1) * 64 * IPT), row-major, so ranking order is input
@@ -162,15 +195,21 @@ __device__ __forceinline__ void tile_pass(TileSmem<EW, BLOCK, IPT, DBITS>& s, ui
   for (int j = 0; j < IPT; ++j) {
     const uint32_t d = dg[j];
     const bool valid = d < (uint32_t)BINS;
-    const uint64_t peers = match_digit<DBITS>(d, valid);
-    const uint32_t before = h[wid][valid ? d : 0u];
-    const uint32_t below = (uint32_t)__builtin_popcountll(peers & lt);
-    if (valid && below == 0) h[wid][d] = before + (uint32_t)__builtin_popcountll(peers);
-    rk[j] = valid ? before + below : 0u;
+    uint32_t r = 0;
+    // Rows past the tile's end are skipped (wave-uniform); no array is
+    // written inside the branch.
+    if (wbase + (uint32_t)j * kWave < n) {
+      const uint64_t peers = match_digit<DBITS>(d, valid);
+      const uint32_t before = h[wid][valid ? d : 0u];
+      const uint32_t below = (uint32_t)__builtin_popcountll(peers & lt);
+      if (valid && below == 0) h[wid][d] = before + (uint32_t)__builtin_popcountll(peers);
+      r = before + below;
+    }
+    rk[j] = valid ? r : 0u;
   }
   __syncthreads();
   for (uint32_t i = tid; i < (uint32_t)(NW * BINS); i += BLOCK) (&s.hist[buf ^ 1][0][0])[i] = 0;
-  if (wid == 0) digit_scan_wave0<NW, BINS>(h, nullptr, nullptr);
+  digit_scan_block<NW, BINS, BLOCK>(h, s.red);
   __syncthreads();
   // Destination of every item, then the items themselves; all reads of the
   // tile finish before the first write.

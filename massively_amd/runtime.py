@@ -276,3 +276,8 @@ class GraphCapture:
         elif st == 0:
             Graph(self.executor, h).close()
         return False
+
+
+def segsort_tuned_config(key_bytes: int, value_bytes: int, launch: int, n: int, nseg: int) -> int:
+    """Index of the seg::Sort candidate the autotuner chose for this shape (-1: not tuned)."""
+    return int(L.lib().msv_segsort_tuned_config(key_bytes, value_bytes, launch, n, nseg))
