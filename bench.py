@@ -182,6 +182,7 @@ def w_sort(c):
     v28 = c.ex.alloc(np.uint32, n28)
     c.mr.iota_u32(c.ex, v28)
     out["u32_2^28"] = _sort_case(c, "sort_by_key_u32_2^28", k28, v28, n28, 4, 4, 4)
+    out["reference_schedule_proxy_u32_2^28"] = refproxy_case(c, k28, v28, n28, reps=2)
     del k28, v28
     n20 = 1 << 20
     k20 = c.ex.alloc(np.uint32, n20)
@@ -190,7 +191,35 @@ def w_sort(c):
     c.mr.iota_u32(c.ex, v20)
     ms = timed(c.ex, lambda: m.vector.sort_by_key(c.ex, k20, v20, m.op.Less), max(c.steps, 20), c.warmup)
     out["u32_iota_2^20"] = summarize("sort_by_key_u32_iota_2^20", ms, 36.0 * n20, n20)
+    ref = m.vector.sort_by_key(c.ex, k20, v20, m.op.Less)
+    out["reference_schedule_proxy_u32_iota_2^20"] = refproxy_case(c, k20, v20, n20, reps=20,
+                                                                  check=c.ex.to_host(ref))
     return out
+
+
+def refproxy_case(c, keys, vals, n, reps, check=None):
+    """The reference's radix schedule (4-bit LSD, 4 launches per pass, key
+    gathers through the permutation, final gathers), timed on the same GPU
+    as a stand-in for its CubeCL-HIP build; see massively_amd/proxy/."""
+    import ctypes
+
+    path = ROOT / "massively_amd" / "proxy" / "libmsv_refproxy.so"
+    if not path.exists():
+        return None
+    lib = ctypes.CDLL(str(path))
+    f = lib.refproxy_sort_by_key_u32_ms
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_int]
+    ok, ov = c.ex.alloc(c.np.uint32, n), c.ex.alloc(c.np.uint32, n)
+    c.ex.sync()
+    ms = f(keys.device_ptr, vals.device_ptr, n, ok.device_ptr, ov.device_ptr, reps)
+    if ms < 0:
+        return None
+    d = summarize(f"reference_schedule_proxy_sort_by_key_u32_{n}", [ms], None, n)
+    if check is not None:
+        d["same_values_as_msv"] = bool(c.ex.to_host(ov).tobytes() == check.tobytes())
+    return d
 
 
 def w_merge_sets(c):
@@ -478,6 +507,7 @@ def main() -> int:
             log(f"[bench] cpu sort: {extra['cpu_sort_baseline']['value']:.4f} Gkeys/s")
 
     sort_g = extra.get("sort", {}).get("u32_2^28", {}).get("gitems_per_s")
+    proxy_g = (extra.get("sort", {}).get("reference_schedule_proxy_u32_2^28") or {}).get("gitems_per_s")
     served = extra.get("served", {})
     line = {
         "metric": METRIC,
@@ -498,6 +528,7 @@ def main() -> int:
                      "frac": scan["frac_hbm"], "traffic": load_traffic("pmc_inclusive_scan_u32_2^28")},
         "cpu_baseline": cpu,
         "sort_gkeys_per_s_u32_2^28": sort_g,
+        "sort_vs_reference_schedule_proxy": round(sort_g / proxy_g, 2) if sort_g and proxy_g else None,
         "served_p50_p99_us": [served.get("poisson_batched", {}).get("p50_us"),
                               served.get("poisson_batched", {}).get("p99_us")] if served else None,
         "extra": extra,
