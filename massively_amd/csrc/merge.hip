@@ -25,8 +25,12 @@
 namespace msv {
 
 constexpr int kMergeBlock = 256;
-constexpr int kMergeIpt = 8;
+constexpr int kMergeIpt = 16;
 constexpr uint32_t kMergeTile = kMergeBlock * kMergeIpt;
+// Set operations emit through a decoupled look-back, so they use larger
+// tiles (fewer tiles in the look-back chain).
+constexpr int kSetIpt = 32;
+constexpr uint32_t kSetTile = kMergeBlock * kSetIpt;
 
 // Keys are compared through their order words (order_bits).
 template <typename K, int MODE>
@@ -126,11 +130,12 @@ __device__ inline void set_split(const K* A, uint32_t na, const K* B, uint32_t n
 template <typename K, int MODE, bool SET>
 __global__ void merge_partition_kernel(const K* __restrict__ A, uint32_t na,
                                        const K* __restrict__ B, uint32_t nb, uint32_t ntiles,
-                                       uint32_t* __restrict__ sa, uint32_t* __restrict__ sb) {
+                                       uint32_t tile, uint32_t* __restrict__ sa,
+                                       uint32_t* __restrict__ sb) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > ntiles) return;
   const uint64_t total = (uint64_t)na + nb;
-  const uint32_t d = (uint32_t)min((uint64_t)t * kMergeTile, total);
+  const uint32_t d = (uint32_t)min((uint64_t)t * tile, total);
   uint32_t i, j;
   if constexpr (SET) {
     set_split<K, MODE>(A, na, B, nb, d, i, j);
@@ -208,6 +213,46 @@ __global__ __launch_bounds__(kMergeBlock) void merge_tile_kernel(
 // ---- ordered set operations ---------------------------------------------------
 enum SetKind : int { SET_UNION = 0, SET_INTERSECTION = 1, SET_DIFFERENCE = 2 };
 
+// Output count of set tile q, computed by one wave from global memory (the
+// self-help path of the lookback, for a predecessor that is not resident):
+// lane l walks the pair-aligned slice [32 l, 32 l + 32) of the tile.
+template <typename K, int MODE, int KIND>
+__device__ uint32_t set_tile_count(const K* __restrict__ A, const K* __restrict__ B,
+                                   const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb,
+                                   uint32_t q) {
+  constexpr uint32_t SLICE = kSetTile / kWave;
+  const uint32_t lane = __lane_id();
+  const uint32_t i0 = sa[q], j0 = sb[q];
+  const uint32_t la = sa[q + 1] - i0, lb = sb[q + 1] - j0, L = la + lb;
+  const K* TA = A + i0;
+  const K* TB = B + j0;
+  uint32_t ia, jb, ia_e, jb_e;
+  set_split<K, MODE>(TA, la, TB, lb, min(lane * SLICE, L), ia, jb);
+  set_split<K, MODE>(TA, la, TB, lb, min(lane * SLICE + SLICE, L), ia_e, jb_e);
+  uint32_t cnt = 0;
+  for (uint32_t q2 = 0; q2 < SLICE + 2 && (ia < ia_e || jb < jb_e); ++q2) {
+    int step;
+    if (ia < ia_e && jb < jb_e) {
+      const auto wa = Ord<K, MODE>::w(TA[ia]), wb = Ord<K, MODE>::w(TB[jb]);
+      step = wa < wb ? 0 : (wb < wa ? 1 : 2);
+    } else {
+      step = ia < ia_e ? 0 : 1;
+    }
+    if (step == 0) {
+      cnt += KIND != SET_INTERSECTION ? 1u : 0u;
+      ++ia;
+    } else if (step == 1) {
+      cnt += KIND == SET_UNION ? 1u : 0u;
+      ++jb;
+    } else {
+      cnt += KIND != SET_DIFFERENCE ? 1u : 0u;
+      ++ia;
+      ++jb;
+    }
+  }
+  return wave_reduce<uint32_t, OP_ADD>(cnt);
+}
+
 template <typename K, int MODE, int KIND>
 __global__ __launch_bounds__(kMergeBlock) void set_tile_kernel(
     const K* __restrict__ A, uint32_t na, const K* __restrict__ B, uint32_t nb,
@@ -215,16 +260,14 @@ __global__ __launch_bounds__(kMergeBlock) void set_tile_kernel(
     uint32_t* __restrict__ out_len, uint64_t* counter, uint64_t* status, uint32_t gen,
     uint32_t ntiles) {
   constexpr int NW = kMergeBlock / kWave;
-  constexpr uint32_t CAP = kMergeTile + 2;
+  constexpr uint32_t CAP = kSetTile + 2;
   __shared__ K s_keys[CAP];
   __shared__ uint32_t s_split[kMergeBlock + 1][2];
   __shared__ uint32_t s_wave[NW];
   __shared__ uint32_t s_prefix;
-  __shared__ uint32_t s_tile;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
-  if (tid == 0) s_tile = draw_tile(counter, ntiles);
-  __syncthreads();
-  const uint32_t b = s_tile;
+  (void)counter;
+  const uint32_t b = blockIdx.x;
   const uint32_t i0 = sa[b], i1 = sa[b + 1], j0 = sb[b], j1 = sb[b + 1];
   const uint32_t la = i1 - i0, lb = j1 - j0, L = la + lb;
   for (uint32_t k = tid; k < L; k += kMergeBlock) s_keys[k] = k < la ? A[i0 + k] : B[j0 + k - la];
@@ -233,7 +276,7 @@ __global__ __launch_bounds__(kMergeBlock) void set_tile_kernel(
   const K* LB = s_keys + la;
   {
     uint32_t ia, jb;
-    set_split<K, MODE>(LA, la, LB, lb, min(tid * (uint32_t)kMergeIpt, L), ia, jb);
+    set_split<K, MODE>(LA, la, LB, lb, min(tid * (uint32_t)kSetIpt, L), ia, jb);
     s_split[tid][0] = ia;
     s_split[tid][1] = jb;
     if (tid == 0) {
@@ -245,33 +288,37 @@ __global__ __launch_bounds__(kMergeBlock) void set_tile_kernel(
   uint32_t ia = s_split[tid][0], jb = s_split[tid][1];
   const uint32_t ia_e = s_split[tid + 1][0], jb_e = s_split[tid + 1][1];
   // The oracle's loop (vector.rs:595-664) on this thread's range.
-  // A thread range holds at most kMergeIpt + 2 steps (both of its ends can
-  // move by one when they are pair-aligned).
-  K ok[kMergeIpt + 2];
-  uint32_t cnt = 0;
+  // A thread range holds at most kSetIpt + 2 steps (both of its ends can
+  // move by one when they are pair-aligned). Step q always writes ok[q] (the
+  // element it visits) and records in `emit` whether it is an output, so the
+  // register array is indexed statically (a running output index would put
+  // it in scratch memory).
+  constexpr int STEPS = kSetIpt + 2;
+  static_assert(STEPS <= 64, "emit mask width");
+  K ok[STEPS];
+  uint64_t emit = 0;
 #pragma unroll
-  for (int q = 0; q < kMergeIpt + 2; ++q) {
-    if (ia < ia_e || jb < jb_e) {
-      int step;  // 0: left only, 1: right only, 2: matched pair
-      if (ia < ia_e && jb < jb_e) {
-        const auto wa = Ord<K, MODE>::w(LA[ia]), wb = Ord<K, MODE>::w(LB[jb]);
-        step = wa < wb ? 0 : (wb < wa ? 1 : 2);
-      } else {
-        step = ia < ia_e ? 0 : 1;
-      }
-      if (step == 0) {
-        if (KIND != SET_INTERSECTION) ok[cnt++] = LA[ia];
-        ++ia;
-      } else if (step == 1) {
-        if (KIND == SET_UNION) ok[cnt++] = LB[jb];
-        ++jb;
-      } else {
-        if (KIND != SET_DIFFERENCE) ok[cnt++] = LA[ia];
-        ++ia;
-        ++jb;
-      }
+  for (int q = 0; q < STEPS; ++q) {
+    const bool ha = ia < ia_e, hb = jb < jb_e;
+    const K a = LA[ha ? ia : 0u];
+    const K b = LB[hb ? jb : 0u];
+    int step;  // 0: left only, 1: right only, 2: matched pair
+    if (ha && hb) {
+      const auto wa = Ord<K, MODE>::w(a), wb = Ord<K, MODE>::w(b);
+      step = wa < wb ? 0 : (wb < wa ? 1 : 2);
+    } else {
+      step = ha ? 0 : 1;
     }
+    const bool act = ha || hb;
+    const bool out = act && (step == 0 ? KIND != SET_INTERSECTION
+                             : step == 1 ? KIND == SET_UNION
+                                         : KIND != SET_DIFFERENCE);
+    ok[q] = step == 1 ? b : a;
+    emit |= (uint64_t)(out ? 1u : 0u) << q;
+    ia += (act && step != 1) ? 1u : 0u;
+    jb += (act && step != 0) ? 1u : 0u;
   }
+  const uint32_t cnt = (uint32_t)__builtin_popcountll(emit);
   // Block scan of emitted counts, then lookback over tiles.
   const uint32_t incl = wave_inclusive_scan<uint32_t, OP_ADD>(cnt);
   if (lane == 63) s_wave[wid] = incl;
@@ -289,7 +336,9 @@ __global__ __launch_bounds__(kMergeBlock) void set_tile_kernel(
       if (lane == 0) st.publish(0, kFlagInc, 0, agg);
     } else {
       if (lane == 0) st.publish(b, kFlagAgg, 0, agg);
-      excl = wave_lookback<uint32_t, OP_ADD>(st, b);
+      excl = wave_lookback_sh<uint32_t, OP_ADD>(st, b, [&](uint32_t q) {
+        return set_tile_count<K, MODE, KIND>(A, B, sa, sb, q);
+      });
       if (lane == 0) st.publish(b, kFlagInc, 0, excl + agg);
     }
     if (lane == 0) {
@@ -298,10 +347,14 @@ __global__ __launch_bounds__(kMergeBlock) void set_tile_kernel(
     }
   }
   __syncthreads();  // everyone is done reading s_keys as input
-  const uint32_t my = woff + incl - cnt;
+  uint32_t my = woff + incl - cnt;
 #pragma unroll
-  for (int q = 0; q < kMergeIpt + 2; ++q)
-    if ((uint32_t)q < cnt) s_keys[my + q] = ok[q];
+  for (int q = 0; q < STEPS; ++q) {
+    if ((emit >> q) & 1u) {
+      s_keys[my] = ok[q];
+      ++my;
+    }
+  }
   __syncthreads();
   const size_t o = s_prefix;
   for (uint32_t k = tid; k < agg; k += kMergeBlock) out[o + k] = s_keys[k];
@@ -312,12 +365,12 @@ __global__ void set_len_zero_kernel(uint32_t* p) { *p = 0; }
 // ---- host ---------------------------------------------------------------------
 template <typename K, int MODE, bool SET>
 static msv_status partition(Ctx& c, const K* A, uint32_t na, const K* B, uint32_t nb,
-                            uint32_t ntiles, uint32_t** sa, uint32_t** sb) {
+                            uint32_t ntiles, uint32_t tile, uint32_t** sa, uint32_t** sb) {
   *sa = (uint32_t*)c.scratch(sizeof(uint32_t) * (ntiles + 1));
   *sb = (uint32_t*)c.scratch(sizeof(uint32_t) * (ntiles + 1));
   if (!*sa || !*sb) return MSV_ERR_OUT_OF_MEMORY;
   hipLaunchKernelGGL((merge_partition_kernel<K, MODE, SET>), dim3((ntiles + 1 + 255) / 256),
-                     dim3(256), 0, c.stream(), A, na, B, nb, ntiles, *sa, *sb);
+                     dim3(256), 0, c.stream(), A, na, B, nb, ntiles, tile, *sa, *sb);
   MSV_HIP(hipGetLastError());
   return MSV_OK;
 }
@@ -329,7 +382,8 @@ static msv_status merge_impl(Ctx& c, const void* A, uint32_t na, const void* B, 
   if (total == 0) return MSV_OK;
   const uint32_t ntiles = (uint32_t)((total + kMergeTile - 1) / kMergeTile);
   uint32_t *sa, *sb;
-  MSV_TRY((partition<K, MODE, false>(c, (const K*)A, na, (const K*)B, nb, ntiles, &sa, &sb)));
+  MSV_TRY((partition<K, MODE, false>(c, (const K*)A, na, (const K*)B, nb, ntiles, kMergeTile, &sa,
+                                     &sb)));
   hipLaunchKernelGGL((merge_tile_kernel<K, MODE, VS>), dim3(ntiles), dim3(kMergeBlock), 0,
                      c.stream(), (const K*)A, na, (const K*)B, nb, Av, Bv, sa, sb, (K*)out_k,
                      out_v);
@@ -346,9 +400,10 @@ static msv_status set_impl(Ctx& c, int kind, const void* A, uint32_t na, const v
     MSV_HIP(hipGetLastError());
     return MSV_OK;
   }
-  const uint32_t ntiles = (uint32_t)((total + kMergeTile - 1) / kMergeTile);
+  const uint32_t ntiles = (uint32_t)((total + kSetTile - 1) / kSetTile);
   uint32_t *sa, *sb;
-  MSV_TRY((partition<K, MODE, true>(c, (const K*)A, na, (const K*)B, nb, ntiles, &sa, &sb)));
+  MSV_TRY((partition<K, MODE, true>(c, (const K*)A, na, (const K*)B, nb, ntiles, kSetTile, &sa,
+                                    &sb)));
   uint64_t *counter, *status;
   uint32_t gen;
   MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));

@@ -18,8 +18,12 @@
 namespace msv {
 
 constexpr int kSelBlock = 256;
-constexpr int kSelRows = 4;
-constexpr uint32_t kSelTile = kSelBlock * kSelRows * 4;
+// Rows of 4-element groups per thread: 4 (4096-element tiles) for short
+// inputs, 8 (8192) from 2^22 elements on, so that fewer tiles take part in
+// the look-back.
+constexpr int kSelRowsSmall = 4;
+constexpr int kSelRowsLarge = 8;
+constexpr uint32_t kSelLargeMin = 1u << 22;
 
 template <typename T>
 __device__ inline void load4(const T* __restrict__ p, size_t i, uint32_t n, bool full, T (&v)[4]) {
@@ -44,7 +48,7 @@ __device__ inline void load4(const T* __restrict__ p, size_t i, uint32_t n, bool
 // FORM 0: u32 stencil array (flag = stencil != 0).
 // FORM 1: predicate on a separate stencil column of type S.
 // FORM 2: predicate on the input itself (S == T, one read).
-template <typename T, typename S, int FORM>
+template <typename T, typename S, int FORM, int ROWS>
 __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__ in,
                                                            const S* __restrict__ stencil,
                                                            Pred<S> pred, bool invert,
@@ -54,8 +58,9 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
                                                            uint64_t* status, uint32_t gen,
                                                            uint32_t ntiles) {
   constexpr int NW = kSelBlock / kWave;
+  constexpr uint32_t kSelTile = (uint32_t)kSelBlock * ROWS * 4;
   __shared__ T s_stage[kSelTile];
-  __shared__ uint32_t s_wave[kSelRows][NW];
+  __shared__ uint32_t s_wave[ROWS][NW];
   __shared__ uint32_t s_prefix;
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
@@ -63,10 +68,10 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
   const size_t base = (size_t)tile * kSelTile;
   const bool full = aligned && base + kSelTile <= n;
 
-  T x[kSelRows][4];
-  uint32_t f[kSelRows][4];
+  T x[ROWS][4];
+  uint32_t f[ROWS][4];
 #pragma unroll
-  for (int r = 0; r < kSelRows; ++r) {
+  for (int r = 0; r < ROWS; ++r) {
     const size_t i = base + ((size_t)r * kSelBlock + tid) * 4;
     load4<T>(in, i, n, full, x[r]);
     if constexpr (FORM == 0) {
@@ -85,18 +90,18 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
     }
   }
 
-  uint32_t ws[kSelRows];
+  uint32_t ws[ROWS];
 #pragma unroll
-  for (int r = 0; r < kSelRows; ++r) {
+  for (int r = 0; r < ROWS; ++r) {
     const uint32_t c = f[r][0] + f[r][1] + f[r][2] + f[r][3];
     ws[r] = wave_inclusive_scan<uint32_t, OP_ADD>(c);
     if (lane == 63) s_wave[r][wid] = ws[r];
   }
   __syncthreads();
-  uint32_t pre[kSelRows];
+  uint32_t pre[ROWS];
   uint32_t agg = 0;
 #pragma unroll
-  for (int r = 0; r < kSelRows; ++r)
+  for (int r = 0; r < ROWS; ++r)
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
       if (w == (int)wid) pre[r] = agg;
@@ -135,7 +140,7 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
 
   // Stage selected elements at their in-tile rank (input order).
 #pragma unroll
-  for (int r = 0; r < kSelRows; ++r) {
+  for (int r = 0; r < ROWS; ++r) {
     uint32_t pos = pre[r] + shift_up1<uint32_t>(ws[r], 0u);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -159,14 +164,21 @@ static msv_status select_launch(Ctx& c, const void* in, const void* st, const ms
     return MSV_OK;
   }
   const Pred<S> pred = make_pred<S>(p);
-  const uint32_t ntiles = (n + kSelTile - 1) / kSelTile;
+  const bool large = n >= kSelLargeMin;
+  const uint32_t tile = (uint32_t)kSelBlock * (large ? kSelRowsLarge : kSelRowsSmall) * 4;
+  const uint32_t ntiles = (n + tile - 1) / tile;
   uint64_t *counter, *status;
   uint32_t gen;
   MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));
   const bool aligned = (((uintptr_t)in | (uintptr_t)st) & 15u) == 0;
-  hipLaunchKernelGGL((select_kernel<T, S, FORM>), dim3(ntiles), dim3(kSelBlock), 0, c.stream(),
-                     (const T*)in, (const S*)st, pred, invert, n, aligned, (T*)out, out_len,
-                     status, gen, ntiles);
+  if (large)
+    hipLaunchKernelGGL((select_kernel<T, S, FORM, kSelRowsLarge>), dim3(ntiles), dim3(kSelBlock), 0,
+                       c.stream(), (const T*)in, (const S*)st, pred, invert, n, aligned, (T*)out,
+                       out_len, status, gen, ntiles);
+  else
+    hipLaunchKernelGGL((select_kernel<T, S, FORM, kSelRowsSmall>), dim3(ntiles), dim3(kSelBlock), 0,
+                       c.stream(), (const T*)in, (const S*)st, pred, invert, n, aligned, (T*)out,
+                       out_len, status, gen, ntiles);
   MSV_HIP(hipGetLastError());
   return MSV_OK;
 }
