@@ -298,10 +298,10 @@ def w_seg(c):
     return out
 
 
-def served_trace(np, jobs=500, seed=10):
+def served_trace(np, jobs=500, seed=10, log2_max=26):
     """The served mixed stream: kinds uniform over sort_by_key / inclusive_scan /
-    reduce / copy_where, sizes log-uniform in 2^10..2^26, key dtypes uniform
-    over u32 / u64 / f32, one seeded input per job."""
+    reduce / copy_where, sizes log-uniform in 2^10..2^log2_max, key dtypes
+    uniform over u32 / u64 / f32, one seeded input per job."""
     from massively_amd import op, serve
 
     rng = np.random.default_rng(seed)
@@ -309,7 +309,7 @@ def served_trace(np, jobs=500, seed=10):
     for i in range(jobs):
         kind = ("sort_by_key", "inclusive_scan", "reduce", "copy_where")[rng.integers(0, 4)]
         dtype = (np.uint32, np.uint64, np.float32)[rng.integers(0, 3)]
-        n = int(round(2 ** rng.uniform(10, 26)))
+        n = int(round(2 ** rng.uniform(10, log2_max)))
         es = np.dtype(dtype).itemsize
         if kind == "sort_by_key":
             r = serve.Request(kind, dtype, n, seed=1000 + i, op=op.Less)
@@ -388,6 +388,18 @@ def w_served(c):
                                       jobs_per_s=round(len(reqs) / (span * 1e-6), 1))
         log(f"[bench] served poisson {mode} @ {rate:.0f} jobs/s: p50 {p50:.0f} us, p99 {p99:.0f} us")
     out["capacity_ratio_batched_over_serial"] = round(cap["batched"] / cap["serial"], 3)
+    # The launch-bound part of the stream on its own: the same mix with sizes
+    # 2^10..2^16 only (a separate, labelled measurement).
+    sreqs, sbytes = served_trace(np, jobs=2000, seed=11, log2_max=16)
+    scap = {}
+    for mode in ("batched", "serial"):
+        span, lat = run_served(np, mode, sreqs)
+        scap[mode] = len(sreqs) / (span * 1e-6)
+        out[f"small_jobs_capacity_{mode}"] = dict(jobs=len(sreqs), makespan_ms=round(span / 1e3, 3),
+                                                  jobs_per_s=round(scap[mode], 1),
+                                                  gbs=round(sum(sbytes) / (span * 1e-6) / 1e9, 2))
+        log(f"[bench] served small-jobs capacity {mode}: {scap[mode]:.0f} jobs/s")
+    out["small_jobs_capacity_ratio"] = round(scap["batched"] / scap["serial"], 3)
     return out
 
 

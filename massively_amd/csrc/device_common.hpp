@@ -459,6 +459,62 @@ __device__ inline void wave_lookback_seg(const TileStatus<T>& st, uint32_t tile,
   carry = prefix;
 }
 
+// Segmented lookback with self-help: as wave_lookback_seg, but a predecessor
+// whose status stays invalid for kSelfHelpPolls polls is computed by the
+// whole wave with `agg_of(q, head, value)` (uniform q), as in
+// wave_lookback_sh, so no tile waits on a tile that is not resident.
+template <typename T, int OP, typename AggFn>
+__device__ inline void wave_lookback_seg_sh(const TileStatus<T>& st, uint32_t tile,
+                                            uint32_t& head_out, T& carry, AggFn agg_of) {
+  using C = Combine<T, OP>;
+  const uint32_t lane = __lane_id();
+  T prefix = C::identity();
+  uint32_t any_head = 0;
+  int64_t base = (int64_t)tile - 1;
+  for (;;) {
+    const int64_t p = base - (int64_t)lane;
+    T v = C::identity();
+    uint32_t head = 0;
+    uint32_t flag = p >= 0 ? 0u : kFlagInc;
+    for (uint32_t polls = 0;; ++polls) {
+      if (flag == 0) flag = st.try_read((uint32_t)p, head, v);
+      const uint64_t pending = ballot(flag == 0);
+      if (!pending) break;
+      if (polls >= kSelfHelpPolls) {
+        for (uint64_t m = pending; m; m &= m - 1) {
+          const uint32_t k = (uint32_t)__builtin_ctzll(m);
+          uint32_t hf = 0;
+          T a = C::identity();
+          agg_of((uint32_t)(base - (int64_t)k), hf, a);
+          if (lane == k) {
+            v = a;
+            head = hf;
+            flag = kFlagAgg;
+          }
+        }
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    const uint64_t stop_mask = ballot(flag == kFlagInc || head != 0);
+    const uint64_t head_mask = ballot(head != 0);
+    if (stop_mask) {
+      const uint32_t first = (uint32_t)__builtin_ctzll(stop_mask);
+      T acc = C::identity();
+      for (int k = (int)first; k >= 0; --k) acc = C::apply(acc, readlane(v, k));
+      any_head |= (head_mask & ((first == 63) ? ~0ull : ((2ull << first) - 1))) ? 1u : 0u;
+      prefix = C::apply(acc, prefix);
+      break;
+    }
+    T acc = C::identity();
+    for (int k = kWave - 1; k >= 0; --k) acc = C::apply(acc, readlane(v, k));
+    prefix = C::apply(acc, prefix);
+    base -= kWave;
+  }
+  head_out = any_head;
+  carry = prefix;
+}
+
 // Order-preserving unsigned encodings for radix ordering (integer order and
 // IEEE total order), see massively core/radix.rs:13-118.
 template <typename K>

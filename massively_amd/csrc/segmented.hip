@@ -28,7 +28,7 @@
 namespace msv {
 
 constexpr int kSegBlock = 256;
-constexpr int kSegRows = 4;
+constexpr int kSegRows = 8;
 
 enum SegOut : int { SEG_INCL = 0, SEG_EXCL = 1, SEG_REDUCE = 2 };
 enum SegMode : int { SEG_LOOKBACK = 0, SEG_AGGREGATE = 1, SEG_PREFIXED = 2 };
@@ -49,6 +49,30 @@ __device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint3
   return lo;
 }
 
+// First index in [lo, hi) with a[i] >= v (hi if none), for a nondecreasing
+// array, searched by one whole wave: each round samples 64 evenly spaced
+// points and keeps the gap that holds the boundary, so a search over 2^20
+// entries takes 4 dependent loads instead of 20.
+__device__ inline uint32_t wave_lower_bound(const uint32_t* a, uint32_t lo, uint32_t hi,
+                                            uint32_t v) {
+  const uint32_t lane = __lane_id();
+  while (hi - lo > (uint32_t)kWave) {
+    const uint32_t step = (hi - lo + kWave - 1) / kWave;
+    const uint32_t idx = lo + lane * step;
+    const bool below = idx < hi && a[idx] < v;
+    const uint32_t cnt = (uint32_t)__builtin_popcountll(ballot(below));
+    // Points lo + k*step for k < cnt are below v; the answer lies after the
+    // last of them and at or before the next sampled point.
+    const uint32_t nlo = cnt ? lo + (cnt - 1) * step + 1 : lo;
+    const uint32_t nhi = min(hi, lo + cnt * step);
+    lo = nlo;
+    hi = nhi;
+  }
+  const uint32_t idx = lo + lane;
+  const bool below = idx < hi && a[idx] < v;
+  return lo + (uint32_t)__builtin_popcountll(ballot(below));
+}
+
 template <typename T, int OP, int MODE, int OUT>
 __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
     const T* __restrict__ in, uint32_t n, const uint32_t* __restrict__ offsets, uint32_t nseg,
@@ -64,26 +88,18 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
   __shared__ uint32_t s_cf;
   __shared__ T s_cv;
   __shared__ uint32_t s_first;
-  __shared__ uint32_t s_tile;
   __shared__ T s_vals[OUT == SEG_REDUCE ? TILE : 1];
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
-  uint32_t tile;
-  if constexpr (MODE == SEG_LOOKBACK) {
-    if (tid == 0) s_tile = draw_tile(counter, ntiles);
-  }
+  (void)counter;
   for (uint32_t i = tid; i < TILE / 32; i += kSegBlock) s_heads[i] = 0;
   __syncthreads();
-  if constexpr (MODE == SEG_LOOKBACK) {
-    tile = s_tile;
-  } else {
-    tile = blockIdx.x;
-  }
+  const uint32_t tile = blockIdx.x;
   const size_t base = (size_t)tile * TILE;
   const uint32_t tile_end = (uint32_t)min((size_t)n, base + TILE);
-  if (tid == 0) {
-    const uint32_t lb = lower_bound_u32(offsets, 0, nseg + 1, (uint32_t)base);
-    s_first = lb > 0 ? lb - 1 : 0;
+  if (wid == 0) {
+    const uint32_t lb = wave_lower_bound(offsets, 0, nseg + 1, (uint32_t)base);
+    if (lane == 0) s_first = lb > 0 ? lb - 1 : 0;
   }
   __syncthreads();
   const uint32_t s0 = s_first;
@@ -176,7 +192,23 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
         if (lane == 0) st.publish(0, kFlagInc, af, av);
       } else {
         if (lane == 0) st.publish(tile, kFlagAgg, af, av);
-        wave_lookback_seg<T, OP>(st, tile, cf, cv);
+        // Self-help: the segmented aggregate of tile q from the input (the
+        // elements from its last segment start, or all of it if none).
+        auto agg_of = [&](uint32_t q, uint32_t& hf, T& a) {
+          const size_t qb = (size_t)q * TILE;
+          const uint32_t qe = (uint32_t)min((size_t)n, qb + TILE);
+          const uint32_t j = lower_bound_u32(offsets, 0, nseg + 1, qe);
+          size_t from = qb;
+          hf = 0;
+          if (j > 0 && offsets[j - 1] >= qb) {
+            from = offsets[j - 1];
+            hf = 1;
+          }
+          T acc = id;
+          for (size_t i = from + lane; i < qe; i += kWave) acc = C::apply(acc, in[i]);
+          a = wave_reduce<T, OP>(acc);
+        };
+        wave_lookback_seg_sh<T, OP>(st, tile, cf, cv, agg_of);
         uint32_t nf = cf;
         T nv = cv;
         seg_combine<T, OP>(nf, nv, af, av);
