@@ -298,6 +298,26 @@ def w_seg(c):
     return out
 
 
+def w_bykey(c):
+    """The by-key family over 2^28 u32 values whose u32 keys form runs of
+    random length (sorted uniform keys in [0, 2^24), mean run 16)."""
+    np, m = c.np, c.m
+    n = 1 << 28
+    k = c.ex.alloc(np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, (1 << 24) - 1, 21, k)
+    ks = m.vector.sort(c.ex, k, m.op.Less)
+    x = c.ex.alloc(np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, 2**32 - 1, 22, x)
+    out = {}
+    ms = timed(c.ex, lambda: m.vector.inclusive_scan_by_key(c.ex, ks, x, m.op.Add), c.steps, c.warmup)
+    out["inclusive_scan_by_key"] = summarize("inclusive_scan_by_key_u32_2^28", ms, 12.0 * n, n)
+    ms = timed(c.ex, lambda: m.vector.reduce_by_key(c.ex, ks, x, 0, m.op.Add), c.steps, c.warmup)
+    out["reduce_by_key"] = summarize("reduce_by_key_u32_2^28", ms, 8.0 * n + 8.0 * n / 16, n)
+    ms = timed(c.ex, lambda: m.vector.unique_by_key(c.ex, ks, x), c.steps, c.warmup)
+    out["unique_by_key"] = summarize("unique_by_key_u32_2^28", ms, 8.0 * n + 4.0 * n / 16, n)
+    return out
+
+
 def served_trace(np, jobs=500, seed=10, log2_max=26):
     """The served mixed stream: kinds uniform over sort_by_key / inclusive_scan /
     reduce / copy_where, sizes log-uniform in 2^10..2^log2_max, key dtypes
@@ -476,6 +496,7 @@ WORKLOADS = {
     "sort": w_sort,
     "merge_sets": w_merge_sets,
     "seg": w_seg,
+    "bykey": w_bykey,
     "served": w_served,
 }
 

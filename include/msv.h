@@ -297,6 +297,25 @@ msv_status msv_random_uniform_u64(msv_executor* ex, uint64_t min, uint64_t max, 
 /* out[i] = start + i (u32). */
 msv_status msv_iota_u32(msv_executor* ex, uint32_t start, msv_slice out);
 
+/* ---- by-key family (massively/src/api/algorithm/by_key.rs:359-811) ----
+ * Keys and values have equal lengths (else MSV_ERR_LENGTH_MISMATCH). The key
+ * equality is the built-in Equal: numeric == in the key dtype (-0.0 == +0.0;
+ * a NaN key equals nothing). A run is a maximal range of equal adjacent keys.
+ *  inclusive_scan_by_key (by_key.rs:399-427): the scan restarts at each run;
+ *  exclusive_scan_by_key (by_key.rs:497-546): each run starts from init;
+ *  reduce_by_key (by_key.rs:618-682): per run, its first key and init (+) its
+ *    values; both outputs carry a device-resident length (the run count);
+ *  unique_by_key (by_key.rs:759-789): the first value of each run, with a
+ *    device-resident length. */
+msv_status msv_inclusive_scan_by_key(msv_executor* ex, msv_slice keys, msv_slice values, msv_op op,
+                                     msv_vec** out);
+msv_status msv_exclusive_scan_by_key(msv_executor* ex, msv_slice keys, msv_slice values,
+                                     const void* init, msv_op op, msv_vec** out);
+msv_status msv_reduce_by_key(msv_executor* ex, msv_slice keys, msv_slice values, const void* init,
+                             msv_op op, msv_vec** out_keys, msv_vec** out_values);
+msv_status msv_unique_by_key(msv_executor* ex, msv_slice keys, msv_slice values,
+                             msv_vec** out_values);
+
 /* ---- batched jobs (no counterpart in the reference) -------------------
  * For callers with many independent jobs (the serving loop below, or any
  * batch of small requests): every job's inputs and outputs are caller-owned

@@ -139,6 +139,10 @@ PROTOTYPES = {
     "msv_executor_trim": (_I, [_P]),
     "msv_executor_set_option": (_I, [_P, _I, C.c_int64]),
     "msv_executor_get_option": (_I, [_P, _I, C.POINTER(C.c_int64)]),
+    "msv_inclusive_scan_by_key": (_I, [_P, _S, _S, _I, _PP]),
+    "msv_exclusive_scan_by_key": (_I, [_P, _S, _S, _P, _I, _PP]),
+    "msv_reduce_by_key": (_I, [_P, _S, _S, _P, _I, _PP, _PP]),
+    "msv_unique_by_key": (_I, [_P, _S, _S, _PP]),
     "msv_executor_begin_capture": (_I, [_P]),
     "msv_executor_end_capture": (_I, [_P, _PP]),
     "msv_graph_launch": (_I, [_P]),

@@ -195,6 +195,10 @@ msv_status run_jobs_plain(Ctx& c, const msv_job* jobs, uint32_t count);
 // mode: 0 total order, 1 Less, 2 Greater; vsize 0 (keys only), 4 or 8.
 msv_status radix_sort_run(Ctx& c, msv_dtype kdt, int mode, size_t vsize, const void* keys,
                           const void* vals, uint32_t n, void* out_k, void* out_v);
+// Segmented inclusive / exclusive scan whose segment heads are the set bits
+// of `hmask` (bit i = element i starts a segment), for the by-key scans.
+msv_status mask_scan_run(Ctx& c, msv_dtype dt, msv_op op, bool exclusive, const void* in,
+                         uint32_t n, const uint32_t* hmask, void* out, const void* init);
 // Enqueues a batch of validated-on-entry jobs (msv_run_jobs).
 msv_status run_jobs(Ctx& c, const msv_job* jobs, uint32_t count);
 // Stable sort of every segment of offsets[0..nseg] on its own (mode 1 Less,

@@ -73,11 +73,15 @@ __device__ inline uint32_t wave_lower_bound(const uint32_t* a, uint32_t lo, uint
   return lo + (uint32_t)__builtin_popcountll(ballot(below));
 }
 
-template <typename T, int OP, int MODE, int OUT>
+// MASK: segment heads come from a bitmask (bit i set = element i starts a
+// segment; the by-key scans build it from adjacent key comparisons) instead
+// of CSR offsets.
+template <typename T, int OP, int MODE, int OUT, bool MASK = false>
 __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
     const T* __restrict__ in, uint32_t n, const uint32_t* __restrict__ offsets, uint32_t nseg,
     T* __restrict__ out, T init, bool aligned, uint64_t* counter, uint64_t* status,
-    uint32_t gen, uint32_t ntiles, uint32_t* __restrict__ tile_f, T* __restrict__ tile_v) {
+    uint32_t gen, uint32_t ntiles, uint32_t* __restrict__ tile_f, T* __restrict__ tile_v,
+    const uint32_t* __restrict__ hmask) {
   using C = Combine<T, OP>;
   constexpr int V = 16 / sizeof(T);
   constexpr int NW = kSegBlock / kWave;
@@ -92,30 +96,36 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
   (void)counter;
-  for (uint32_t i = tid; i < TILE / 32; i += kSegBlock) s_heads[i] = 0;
-  __syncthreads();
   const uint32_t tile = blockIdx.x;
   const size_t base = (size_t)tile * TILE;
   const uint32_t tile_end = (uint32_t)min((size_t)n, base + TILE);
-  if (wid == 0) {
-    const uint32_t lb = wave_lower_bound(offsets, 0, nseg + 1, (uint32_t)base);
-    if (lane == 0) s_first = lb > 0 ? lb - 1 : 0;
-  }
-  __syncthreads();
-  const uint32_t s0 = s_first;
-
-  // Mark the segment starts that fall inside the tile.
-  for (uint32_t c = 0;; c += kSegBlock) {
-    const uint32_t s = s0 + c + tid;
-    bool more = false;
-    if (s < nseg) {
-      const uint32_t st = offsets[s];
-      if (st < tile_end) {
-        more = true;
-        if (st >= base) atomicOr(&s_heads[(st - base) >> 5], 1u << ((st - base) & 31u));
-      }
+  uint32_t s0 = 0;
+  if constexpr (MASK) {
+    // TILE is a multiple of 32, so the tile's heads are whole mask words.
+    const uint32_t w0 = (uint32_t)(base >> 5), nw = (uint32_t)((n + 31) / 32);
+    for (uint32_t i = tid; i < TILE / 32; i += kSegBlock) s_heads[i] = w0 + i < nw ? hmask[w0 + i] : 0u;
+    __syncthreads();
+  } else {
+    for (uint32_t i = tid; i < TILE / 32; i += kSegBlock) s_heads[i] = 0;
+    if (wid == 0) {
+      const uint32_t lb = wave_lower_bound(offsets, 0, nseg + 1, (uint32_t)base);
+      if (lane == 0) s_first = lb > 0 ? lb - 1 : 0;
     }
-    if (!__syncthreads_or(more)) break;
+    __syncthreads();
+    s0 = s_first;
+    // Mark the segment starts that fall inside the tile.
+    for (uint32_t c = 0;; c += kSegBlock) {
+      const uint32_t s = s0 + c + tid;
+      bool more = false;
+      if (s < nseg) {
+        const uint32_t st = offsets[s];
+        if (st < tile_end) {
+          more = true;
+          if (st >= base) atomicOr(&s_heads[(st - base) >> 5], 1u << ((st - base) & 31u));
+        }
+      }
+      if (!__syncthreads_or(more)) break;
+    }
   }
 
   const bool vec_ok = aligned && (base + TILE <= n);
@@ -197,12 +207,29 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
         auto agg_of = [&](uint32_t q, uint32_t& hf, T& a) {
           const size_t qb = (size_t)q * TILE;
           const uint32_t qe = (uint32_t)min((size_t)n, qb + TILE);
-          const uint32_t j = lower_bound_u32(offsets, 0, nseg + 1, qe);
           size_t from = qb;
           hf = 0;
-          if (j > 0 && offsets[j - 1] >= qb) {
-            from = offsets[j - 1];
-            hf = 1;
+          if constexpr (MASK) {
+            // Last set bit of the tile's mask words, searched from the end.
+            const uint32_t w0 = (uint32_t)(qb >> 5), w1 = (qe + 31) / 32;
+            for (int64_t wb = (int64_t)w1 - kWave; wb + kWave > (int64_t)w0; wb -= kWave) {
+              const int64_t w = wb + (int64_t)lane;
+              const uint32_t word = (w >= (int64_t)w0 && w < (int64_t)w1) ? hmask[w] : 0u;
+              const uint64_t any = ballot(word != 0);
+              if (any) {
+                const uint32_t hl = 63u - (uint32_t)__builtin_clzll(any);
+                const uint32_t hw = readlane(word, (int)hl);
+                from = (size_t)(wb + hl) * 32 + (31u - (uint32_t)__builtin_clz(hw));
+                hf = 1;
+                break;
+              }
+            }
+          } else {
+            const uint32_t j = lower_bound_u32(offsets, 0, nseg + 1, qe);
+            if (j > 0 && offsets[j - 1] >= qb) {
+              from = offsets[j - 1];
+              hf = 1;
+            }
           }
           T acc = id;
           for (size_t i = from + lane; i < qe; i += kWave) acc = C::apply(acc, in[i]);
@@ -370,9 +397,10 @@ __global__ void validate_offsets_kernel(const uint32_t* __restrict__ offsets, ui
 }
 
 // ---- host -------------------------------------------------------------------------
-template <typename T, int OP>
+template <typename T, int OP, bool MASK = false>
 static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
-                                const uint32_t* offsets, uint32_t nseg, T* out, T init) {
+                                const uint32_t* offsets, uint32_t nseg, T* out, T init,
+                                const uint32_t* hmask = nullptr) {
   constexpr uint32_t TILE = (uint32_t)kSegBlock * kSegRows * (16 / sizeof(T));
   if (n == 0) {
     if (outk == SEG_REDUCE && nseg) {
@@ -387,12 +415,13 @@ static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
   const uintptr_t vec_ptrs = outk == SEG_REDUCE ? (uintptr_t)in : ((uintptr_t)in | (uintptr_t)out);
   const bool aligned = (vec_ptrs & 15u) == 0;
 #define MSV_SEG_LAUNCH(MODE, OUTK, CNT, ST, TF, TV)                                               \
-  hipLaunchKernelGGL((seg_tile_kernel<T, OP, MODE, OUTK>), dim3(ntiles), dim3(kSegBlock), 0,      \
-                     c.stream(), in, n, offsets, nseg, out, init, aligned, CNT, ST, gen, ntiles, TF, TV)
+  hipLaunchKernelGGL((seg_tile_kernel<T, OP, MODE, OUTK, MASK>), dim3(ntiles), dim3(kSegBlock), 0, \
+                     c.stream(), in, n, offsets, nseg, out, init, aligned, CNT, ST, gen, ntiles, TF, TV, \
+                     hmask)
 #define MSV_SEG_BY_OUT(MODE, CNT, ST, TF, TV)                  \
   if (outk == SEG_INCL) MSV_SEG_LAUNCH(MODE, SEG_INCL, CNT, ST, TF, TV);        \
   else if (outk == SEG_EXCL) MSV_SEG_LAUNCH(MODE, SEG_EXCL, CNT, ST, TF, TV);   \
-  else MSV_SEG_LAUNCH(MODE, SEG_REDUCE, CNT, ST, TF, TV);
+  else if constexpr (!MASK) MSV_SEG_LAUNCH(MODE, SEG_REDUCE, CNT, ST, TF, TV);
   uint32_t gen = 0;
   if constexpr (!IsFloat<T>::value) {
     uint64_t *counter, *status;
@@ -404,9 +433,9 @@ static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
     uint32_t* pf = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
     T* pv = (T*)c.scratch(sizeof(T) * ntiles);
     if (!af || !av || !pf || !pv) return MSV_ERR_OUT_OF_MEMORY;
-    hipLaunchKernelGGL((seg_tile_kernel<T, OP, SEG_AGGREGATE, SEG_INCL>), dim3(ntiles),
+    hipLaunchKernelGGL((seg_tile_kernel<T, OP, SEG_AGGREGATE, SEG_INCL, MASK>), dim3(ntiles),
                        dim3(kSegBlock), 0, c.stream(), in, n, offsets, nseg, out, init, aligned,
-                       (uint64_t*)nullptr, (uint64_t*)nullptr, 0u, ntiles, af, av);
+                       (uint64_t*)nullptr, (uint64_t*)nullptr, 0u, ntiles, af, av, hmask);
     MSV_HIP(hipGetLastError());
     hipLaunchKernelGGL((seg_pairs_exclusive_kernel<T, OP>), dim3(1), dim3(1024), 0, c.stream(),
                        af, av, ntiles, pf, pv);
@@ -447,6 +476,39 @@ msv_status seg_scan_run(Ctx& c, msv_dtype dt, msv_op op, int outk, const void* i
     case MSV_U64: return seg_scan_by_op<uint64_t>(c, op, outk, in, n, offsets, nseg, out, init);
     case MSV_I64: return seg_scan_by_op<int64_t>(c, op, outk, in, n, offsets, nseg, out, init);
     case MSV_F64: return seg_scan_by_op<double>(c, op, outk, in, n, offsets, nseg, out, init);
+  }
+  set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
+  return MSV_ERR_INVALID_ARGUMENT;
+}
+
+template <typename T>
+static msv_status mask_scan_by_op(Ctx& c, msv_op op, int outk, const void* in, uint32_t n,
+                                  const uint32_t* hmask, void* out, const void* init_host) {
+  T init = T();
+  if (init_host) memcpy(&init, init_host, sizeof(T));
+  switch (op) {
+    case MSV_OP_ADD: return seg_scan_impl<T, OP_ADD, true>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, hmask);
+    case MSV_OP_MIN: return seg_scan_impl<T, OP_MIN, true>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, hmask);
+    case MSV_OP_MAX: return seg_scan_impl<T, OP_MAX, true>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, hmask);
+    case MSV_OP_XOR:
+      if constexpr (!IsFloat<T>::value)
+        return seg_scan_impl<T, OP_XOR, true>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, hmask);
+      break;
+  }
+  set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "operator %d is not defined for this dtype", (int)op);
+  return MSV_ERR_INVALID_ARGUMENT;
+}
+
+msv_status mask_scan_run(Ctx& c, msv_dtype dt, msv_op op, bool exclusive, const void* in,
+                         uint32_t n, const uint32_t* hmask, void* out, const void* init) {
+  const int outk = exclusive ? SEG_EXCL : SEG_INCL;
+  switch (dt) {
+    case MSV_U32: return mask_scan_by_op<uint32_t>(c, op, outk, in, n, hmask, out, init);
+    case MSV_I32: return mask_scan_by_op<int32_t>(c, op, outk, in, n, hmask, out, init);
+    case MSV_F32: return mask_scan_by_op<float>(c, op, outk, in, n, hmask, out, init);
+    case MSV_U64: return mask_scan_by_op<uint64_t>(c, op, outk, in, n, hmask, out, init);
+    case MSV_I64: return mask_scan_by_op<int64_t>(c, op, outk, in, n, hmask, out, init);
+    case MSV_F64: return mask_scan_by_op<double>(c, op, outk, in, n, hmask, out, init);
   }
   set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
   return MSV_ERR_INVALID_ARGUMENT;

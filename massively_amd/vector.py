@@ -172,3 +172,41 @@ def set_intersection(exec: Executor, left, right, less: BinaryPredicateOp) -> De
 def set_difference(exec: Executor, left, right, less: BinaryPredicateOp) -> DeviceVec:
     """massively/src/api/algorithm/set.rs:147-195."""
     return _set(2, exec, left, right, less)
+
+
+# ---- by-key family (built-in Equal: numeric == on the key dtype) -------------
+def inclusive_scan_by_key(exec: Executor, keys, values, op: ReductionOp) -> DeviceVec:
+    """massively/src/api/algorithm/by_key.rs:399-427 with the built-in Equal."""
+    h = C.c_void_p()
+    check(_lib().msv_inclusive_scan_by_key(exec.handle, as_slice(keys)._c(), as_slice(values)._c(),
+                                           op.code, C.byref(h)))
+    return _out(exec, h)
+
+
+def exclusive_scan_by_key(exec: Executor, keys, values, init, op: ReductionOp) -> DeviceVec:
+    """massively/src/api/algorithm/by_key.rs:497-546 with the built-in Equal."""
+    v = as_slice(values)
+    i = _scalar(init, v.dtype)
+    h = C.c_void_p()
+    check(_lib().msv_exclusive_scan_by_key(exec.handle, as_slice(keys)._c(), v._c(),
+                                           i.ctypes.data_as(C.c_void_p), op.code, C.byref(h)))
+    return _out(exec, h)
+
+
+def reduce_by_key(exec: Executor, keys, values, init, op: ReductionOp):
+    """massively/src/api/algorithm/by_key.rs:618-682: (run keys, init (+) run values),
+    both with a device-resident length."""
+    v = as_slice(values)
+    i = _scalar(init, v.dtype)
+    hk, hv = C.c_void_p(), C.c_void_p()
+    check(_lib().msv_reduce_by_key(exec.handle, as_slice(keys)._c(), v._c(),
+                                   i.ctypes.data_as(C.c_void_p), op.code, C.byref(hk), C.byref(hv)))
+    return _out(exec, hk), _out(exec, hv)
+
+
+def unique_by_key(exec: Executor, keys, values) -> DeviceVec:
+    """massively/src/api/algorithm/by_key.rs:759-789: the first value of each key run."""
+    h = C.c_void_p()
+    check(_lib().msv_unique_by_key(exec.handle, as_slice(keys)._c(), as_slice(values)._c(),
+                                   C.byref(h)))
+    return _out(exec, h)

@@ -68,6 +68,12 @@ def lib():
         L.msvo_seg_exclusive_scan.argtypes = [I, I, P, P, P, U64, P]
         L.msvo_seg_reduce.argtypes = [I, I, P, P, U64, P, P]
         L.msvo_seg_sort_perm.argtypes = [I, I, P, P, U64, P]
+        L.msvo_inclusive_scan_by_key.argtypes = [I, I, I, P, P, P, U64]
+        L.msvo_exclusive_scan_by_key.argtypes = [I, I, I, P, P, P, U64, P]
+        L.msvo_reduce_by_key.argtypes = [I, I, I, P, P, U64, P, P, P]
+        L.msvo_reduce_by_key.restype = C.c_int64
+        L.msvo_unique_by_key.argtypes = [I, I, P, P, U64, P]
+        L.msvo_unique_by_key.restype = C.c_int64
         L.msvo_pcg_hash32.argtypes = [C.c_uint32]
         L.msvo_pcg_hash32.restype = C.c_uint32
         L.msvo_seed_key.argtypes = [U64]
@@ -263,6 +269,38 @@ def seg_sort_perm(keys, offsets, mode: str = "less") -> np.ndarray:
     perm = np.empty(k.size, dtype=np.uint32)
     lib().msvo_seg_sort_perm(DT[k.dtype], ORD[mode], _p(k), _p(o), o.size - 1, _p(perm))
     return perm
+
+
+def inclusive_scan_by_key(keys, values, op: str) -> np.ndarray:
+    k, v = _c(keys), _c(values)
+    y = np.empty_like(v)
+    lib().msvo_inclusive_scan_by_key(DT[k.dtype], DT[v.dtype], OPS[op], _p(k), _p(v), _p(y), v.size)
+    return y
+
+
+def exclusive_scan_by_key(keys, values, init, op: str) -> np.ndarray:
+    k, v = _c(keys), _c(values)
+    y = np.empty_like(v)
+    i = np.array([init], dtype=v.dtype)
+    lib().msvo_exclusive_scan_by_key(DT[k.dtype], DT[v.dtype], OPS[op], _p(k), _p(v), _p(y), v.size,
+                                     _p(i))
+    return y
+
+
+def reduce_by_key(keys, values, init, op: str):
+    k, v = _c(keys), _c(values)
+    ok, ov = np.empty_like(k), np.empty_like(v)
+    i = np.array([init], dtype=v.dtype)
+    r = lib().msvo_reduce_by_key(DT[k.dtype], DT[v.dtype], OPS[op], _p(k), _p(v), v.size, _p(i),
+                                 _p(ok), _p(ov))
+    return ok[:r].copy(), ov[:r].copy()
+
+
+def unique_by_key(keys, values) -> np.ndarray:
+    k, v = _c(keys), _c(values)
+    ov = np.empty_like(v)
+    r = lib().msvo_unique_by_key(DT[k.dtype], DT[v.dtype], _p(k), _p(v), v.size, _p(ov))
+    return ov[:r].copy()
 
 
 def pcg_hash32(x: int) -> int:

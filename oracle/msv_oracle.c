@@ -364,6 +364,93 @@ int msvo_seg_reduce(int dt, int op, const void* in, const uint32_t* offsets, uin
   return 0;
 }
 
+/* ---- by-key family (verification/oracle/src/vector.rs:92-175,578-594) ----
+ * Key equality is the built-in Equal: numeric == in the key dtype (so
+ * -0.0 == +0.0 and a NaN key equals nothing). */
+static int key_eq(int kdt, const void* keys, uint64_t i, uint64_t j) {
+  const char* k = (const char*)keys;
+  switch (kdt) {
+    case DT_F32: {
+      float a, b;
+      memcpy(&a, k + i * 4, 4);
+      memcpy(&b, k + j * 4, 4);
+      return a == b;
+    }
+    case DT_F64: {
+      double a, b;
+      memcpy(&a, k + i * 8, 8);
+      memcpy(&b, k + j * 8, 8);
+      return a == b;
+    }
+    default: {
+      const size_t es = dt_size(kdt);
+      return memcmp(k + i * es, k + j * es, es) == 0;
+    }
+  }
+}
+
+/* End of the run that starts at b: the scans compare adjacent keys
+ * (vector.rs:107, :131). */
+static uint64_t adjacent_run_end(int kdt, const void* keys, uint64_t b, uint64_t n) {
+  uint64_t e = b + 1;
+  while (e < n && key_eq(kdt, keys, e - 1, e)) ++e;
+  return e;
+}
+
+int msvo_inclusive_scan_by_key(int kdt, int vdt, int op, const void* keys, const void* values,
+                               void* out, uint64_t n) {
+  const size_t es = dt_size(vdt);
+  for (uint64_t b = 0; b < n;) {
+    const uint64_t e = adjacent_run_end(kdt, keys, b, n);
+    if (msvo_inclusive_scan(vdt, op, (const char*)values + b * es, (char*)out + b * es, e - b))
+      return -1;
+    b = e;
+  }
+  return 0;
+}
+
+int msvo_exclusive_scan_by_key(int kdt, int vdt, int op, const void* keys, const void* values,
+                               void* out, uint64_t n, const void* init) {
+  const size_t es = dt_size(vdt);
+  for (uint64_t b = 0; b < n;) {
+    const uint64_t e = adjacent_run_end(kdt, keys, b, n);
+    if (msvo_exclusive_scan(vdt, op, (const char*)values + b * es, (char*)out + b * es, e - b, init))
+      return -1;
+    b = e;
+  }
+  return 0;
+}
+
+/* reduce_by_key (vector.rs:144-175): a run continues while the key equals
+ * the run's first key; the run's first element always belongs to it. Returns
+ * the number of runs. */
+int64_t msvo_reduce_by_key(int kdt, int vdt, int op, const void* keys, const void* values,
+                           uint64_t n, const void* init, void* out_keys, void* out_values) {
+  const size_t ks = dt_size(kdt), vs = dt_size(vdt);
+  int64_t r = 0;
+  for (uint64_t b = 0; b < n;) {
+    uint64_t e = b + 1;
+    while (e < n && key_eq(kdt, keys, b, e)) ++e;
+    memcpy((char*)out_keys + r * ks, (const char*)keys + b * ks, ks);
+    if (msvo_reduce(vdt, op, (const char*)values + b * vs, e - b, init, (char*)out_values + r * vs))
+      return -1;
+    ++r;
+    b = e;
+  }
+  return r;
+}
+
+/* unique_by_key (vector.rs:578-594): the value of each run's first element. */
+int64_t msvo_unique_by_key(int kdt, int vdt, const void* keys, const void* values, uint64_t n,
+                           void* out_values) {
+  const size_t vs = dt_size(vdt);
+  int64_t r = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    if (i == 0 || !key_eq(kdt, keys, i - 1, i))
+      memcpy((char*)out_values + (r++) * vs, (const char*)values + i * vs, vs);
+  return r;
+}
+
 /* Per-segment stable sorting permutation (global indices). */
 void msvo_seg_sort_perm(int dt, int mode, const void* keys, const uint32_t* offsets,
                         uint64_t nseg, uint32_t* perm) {

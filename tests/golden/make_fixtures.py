@@ -105,6 +105,24 @@ def literal_vectors() -> list[dict]:
     v.append(dict(name="seg_exclusive_from_reduce", src="massively/tests/seg/api.rs:326-348",
                   op="seg_exclusive_scan", dtype="u32", reduce="add", init=0, values=[1, 2, 4],
                   offsets=[0, 2, 3], expected=[0, 1, 0], expected_reduce=[3, 4]))
+    # by-key family: doc examples and the owned-API test.
+    v.append(dict(name="inclusive_scan_by_key_doc", src="massively/src/api/algorithm/by_key.rs:366-397",
+                  op="inclusive_scan_by_key", dtype="u32", reduce="add", keys=[1, 1, 2, 2],
+                  values=[10, 20, 30, 40], expected=[10, 30, 30, 70]))
+    v.append(dict(name="exclusive_scan_by_key_doc", src="massively/src/api/algorithm/by_key.rs:470-495",
+                  op="exclusive_scan_by_key", dtype="u32", reduce="add", init=0, keys=[1, 1, 2, 2],
+                  values=[10, 20, 30, 40], expected=[0, 10, 0, 30]))
+    v.append(dict(name="reduce_by_key_doc", src="massively/src/api/algorithm/by_key.rs:586-617",
+                  op="reduce_by_key", dtype="u32", reduce="add", init=0, keys=[1, 1, 2, 2],
+                  values=[10, 20, 30, 40], expected_keys=[1, 2], expected=[30, 70]))
+    v.append(dict(name="unique_by_key_doc", src="massively/src/api/algorithm/by_key.rs:730-757",
+                  op="unique_by_key", dtype="u32", keys=[1, 1, 2, 2, 3], values=[10, 11, 20, 21, 30],
+                  expected=[10, 20, 30]))
+    v.append(dict(name="by_key_owned", src="massively/tests/vector/api/owned.rs:186-228",
+                  op="by_key_owned", dtype="u32", reduce="add", init=0, keys=[1, 1, 2, 3],
+                  values=[10, 11, 20, 30], expected_inclusive=[10, 21, 20, 30],
+                  expected_exclusive=[0, 10, 0, 0], expected_reduce_keys=[1, 2, 3],
+                  expected_reduce=[21, 20, 30], expected_unique=[10, 20, 30]))
     v.append(dict(name="segmentation_from_lengths", src="massively/src/seg/segmentation.rs:105-128",
                   op="segmentation", lengths=[1, 2, 3], expected_offsets=[0, 1, 3, 6],
                   expected_segment_ids=[0, 1, 1, 2, 2, 2]))

@@ -285,3 +285,66 @@ def test_generators_follow_reference_properties():
     # pcg_hash32 is a bijection: no collisions over a window of 2^20 inputs.
     h = oracle.uniform_u32(0, 2**32 - 1, 0, 1 << 20)
     assert np.unique(h).size == h.size
+
+
+# ---- by-key family --------------------------------------------------------
+def _np_runs(keys):
+    """Run starts under the built-in Equal (numeric ==; NaN equals nothing)."""
+    k = np.asarray(keys)
+    if k.size == 0:
+        return np.zeros(0, np.int64)
+    return np.flatnonzero(np.r_[True, k[1:] != k[:-1]])
+
+
+def test_by_key_golden():
+    v = golden("inclusive_scan_by_key_doc")
+    k, x = np.array(v["keys"], np.uint32), np.array(v["values"], np.uint32)
+    assert oracle.inclusive_scan_by_key(k, x, "add").tolist() == v["expected"]
+    v = golden("exclusive_scan_by_key_doc")
+    assert oracle.exclusive_scan_by_key(k, x, v["init"], "add").tolist() == v["expected"]
+    v = golden("reduce_by_key_doc")
+    rk, rv = oracle.reduce_by_key(k, x, v["init"], "add")
+    assert rk.tolist() == v["expected_keys"] and rv.tolist() == v["expected"]
+    v = golden("unique_by_key_doc")
+    assert oracle.unique_by_key(np.array(v["keys"], np.uint32),
+                                np.array(v["values"], np.uint32)).tolist() == v["expected"]
+    v = golden("by_key_owned")
+    k, x = np.array(v["keys"], np.uint32), np.array(v["values"], np.uint32)
+    assert oracle.inclusive_scan_by_key(k, x, "add").tolist() == v["expected_inclusive"]
+    assert oracle.exclusive_scan_by_key(k, x, v["init"], "add").tolist() == v["expected_exclusive"]
+    rk, rv = oracle.reduce_by_key(k, x, v["init"], "add")
+    assert rk.tolist() == v["expected_reduce_keys"] and rv.tolist() == v["expected_reduce"]
+    assert oracle.unique_by_key(k, x).tolist() == v["expected_unique"]
+
+
+@pytest.mark.parametrize("kdtype", [np.uint32, np.int64, np.float32])
+@pytest.mark.parametrize("op", ["add", "max", "xor"])
+def test_by_key_vs_numpy_runs(kdtype, op):
+    """Against numpy: per-run ufunc accumulate / reduceat on run boundaries
+    from `k[1:] != k[:-1]`, the first value of each run for unique."""
+    rng = np.random.default_rng(9)
+    for n in (0, 1, 2, 7, 300, 5000):
+        if np.dtype(kdtype).kind == "f":
+            k = np.sort(rng.integers(-20, 20, n)).astype(kdtype)
+            k[k == 0] = np.where(rng.random((k == 0).sum()) < 0.5, -0.0, 0.0).astype(kdtype)
+        else:
+            k = np.sort(rng.integers(0, max(1, n // 7), n)).astype(kdtype)
+        x = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+        starts = _np_runs(k)
+        uf = {"add": np.add, "max": np.maximum, "xor": np.bitwise_xor}[op]
+        inc = np.empty_like(x)
+        exc = np.empty_like(x)
+        red = []
+        bounds = list(starts) + [n]
+        for b, e in zip(bounds[:-1], bounds[1:]):
+            inc[b:e] = uf.accumulate(x[b:e], dtype=np.uint32)
+            ex = np.concatenate([[np.uint32(5)], x[b:e - 1]]).astype(np.uint32)
+            exc[b:e] = uf.accumulate(ex, dtype=np.uint32)
+            red.append(uf.reduce(np.concatenate([[np.uint32(5)], x[b:e]]).astype(np.uint32),
+                                 dtype=np.uint32))
+        assert np.array_equal(oracle.inclusive_scan_by_key(k, x, op), inc)
+        assert np.array_equal(oracle.exclusive_scan_by_key(k, x, 5, op), exc)
+        rk, rv = oracle.reduce_by_key(k, x, 5, op)
+        assert rk.tobytes() == k[starts].tobytes()
+        assert rv.tolist() == [int(r) for r in red]
+        assert np.array_equal(oracle.unique_by_key(k, x), x[starts])
