@@ -9,12 +9,12 @@
 // -0.0 and +0.0 compare equal, so they are ordered as equal keys (stably).
 //
 // Pipeline (keys K of 4 or 8 bytes, optional 4/8-byte values):
-//  1. histogram: one read of the keys builds the 256-bin histograms of every
-//     8-bit digit at once (LDS histograms, wave-uniform digits added once);
-//  2. plan (one block): exclusive scan of each histogram into digit starts;
-//     a digit position whose histogram has a single bin of count n is
-//     skipped; the ping-pong source/destination of every remaining pass is
-//     written into a device-side table, so no host round trip is needed;
+//  1. pass 0's upsweep reads the caller's keys and also records, per tile,
+//     the OR and the AND of the order words;
+//  2. plan (one block): a digit position on which OR and AND agree over all
+//     keys holds one value and is skipped; the ping-pong source/destination
+//     of every remaining pass is written into a device-side table, so no
+//     host round trip is needed;
 //  3. per remaining digit, reduce-then-scan: an upsweep counts the digits of
 //     every tile (512 threads x 16 or 8 keys) into a digit-major
 //     count matrix, one exclusive scan of that matrix (the library's lookback
@@ -56,26 +56,26 @@ static msv_status radix_sort_impl(Ctx& c, const void* keys, const void* vals, ui
   const size_t elem_bytes = (size_t)n * EW * 4;
   void* aos0 = c.scratch(elem_bytes);
   void* aos1 = c.scratch(elem_bytes);
-  uint32_t* hist = (uint32_t*)c.scratch(sizeof(uint32_t) * P * kRadixBins);
+  uint64_t* orand = (uint64_t*)c.scratch(sizeof(uint64_t) * 2 * ntiles);
   RadixPass* table = (RadixPass*)c.scratch(sizeof(RadixPass) * P);
   uint32_t* counts = (uint32_t*)c.scratch(sizeof(uint32_t) * kRadixBins * (size_t)ntiles);
   uint32_t* offs = (uint32_t*)c.scratch(sizeof(uint32_t) * kRadixBins * (size_t)ntiles);
-  if (!aos0 || !aos1 || !hist || !table || !counts || !offs) return MSV_ERR_OUT_OF_MEMORY;
+  if (!aos0 || !aos1 || !orand || !table || !counts || !offs) return MSV_ERR_OUT_OF_MEMORY;
 
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(P), dim3(kRadixBins), 0, c.stream(), hist,
-                     (uint32_t)(P * kRadixBins), 0u);
-  const uint32_t hblocks = (uint32_t)std::min<size_t>(
-      (n + kHistBlock * 16 - 1) / (kHistBlock * 16), (size_t)c.num_cus() * 8);
-  hipLaunchKernelGGL((radix_hist_kernel<K, MODE>), dim3(std::max(1u, hblocks)), dim3(kHistBlock), 0,
-                     c.stream(), (const K*)keys, n, hist);
-  hipLaunchKernelGGL((radix_plan_kernel<P>), dim3(1), dim3(kRadixBins), 0, c.stream(), hist, n,
-                     table, keys, vals, out_k, out_v, aos0, aos1);
+  // Pass 0's counts and the per-tile OR / AND of the order words, then the
+  // plan (which passes run, and where each reads and writes).
+  hipLaunchKernelGGL((radix_upsweep0_kernel<K, MODE, BLOCK, IPT>), dim3(ntiles), dim3(BLOCK), 0,
+                     c.stream(), (const K*)keys, n, ntiles, counts, orand);
+  hipLaunchKernelGGL((radix_plan_kernel<P>), dim3(1), dim3(kRadixBins), 0, c.stream(), orand,
+                     ntiles, n, table, keys, vals, out_k, out_v, aos0, aos1);
   MSV_HIP(hipGetLastError());
   const uint32_t zero = 0;
   for (int p = 0; p < P; ++p) {
-    hipLaunchKernelGGL((radix_upsweep_kernel<K, MODE, EW, BLOCK, IPT>), dim3(ntiles), dim3(BLOCK), 0,
-                       c.stream(), table, p, n, ntiles, counts);
-    MSV_HIP(hipGetLastError());
+    if (p > 0) {
+      hipLaunchKernelGGL((radix_upsweep_kernel<K, MODE, EW, BLOCK, IPT>), dim3(ntiles), dim3(BLOCK),
+                         0, c.stream(), table, p, n, ntiles, counts);
+      MSV_HIP(hipGetLastError());
+    }
     MSV_TRY(scan_run(c, MSV_U32, MSV_OP_ADD, true, counts, offs, kRadixBins * ntiles, &zero));
     hipLaunchKernelGGL((radix_downsweep_kernel<K, MODE, EW, BLOCK, IPT>), dim3(ntiles), dim3(BLOCK), 0,
                        c.stream(), table, p, offs, n, ntiles);

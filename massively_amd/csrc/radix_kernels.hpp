@@ -8,7 +8,6 @@ namespace msv {
 
 constexpr int kRadixIpt = 16;     // keys per thread in a pass tile
 constexpr int kRadixBins = 256;   // 8-bit digits
-constexpr int kHistBlock = 256;
 
 // Element moved by the passes: the key (1 or 2 dwords) followed by the value
 // (0, 1 or 2 dwords). Between passes elements are kept interleaved (AoS), so
@@ -58,58 +57,102 @@ static __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
-// 1. All-digit histogram: each block counts 16 consecutive groups of 256 keys
-//    (loads issued before the LDS updates), per-wave LDS counters.
-template <typename K, int MODE>
-__global__ __launch_bounds__(kHistBlock) void radix_hist_kernel(const K* __restrict__ keys,
-                                                                uint32_t n,
-                                                                uint32_t* __restrict__ g_hist) {
-  constexpr int P = sizeof(K);
-  constexpr int NW = kHistBlock / kWave;
-  constexpr int J = 16;
-  __shared__ uint32_t s_hist[NW][P][kRadixBins];
-  const uint32_t tid = threadIdx.x, wid = tid / 64u;
-  for (uint32_t i = tid; i < NW * P * kRadixBins; i += kHistBlock) (&s_hist[0][0][0])[i] = 0;
+// 1. First upsweep (pass 0's digit counts, see 3a) from the caller's keys,
+//    which also records per tile the OR and the AND of the keys' order
+//    words: a digit position on which OR ^ AND is zero holds one value for
+//    every key, and its pass is skipped. (An all-digit histogram would need
+//    one LDS atomic per key per digit; two bitwise reductions per tile do.)
+template <typename K, int MODE, int BLOCK, int IPT = kRadixIpt>
+__global__ __launch_bounds__(BLOCK) void radix_upsweep0_kernel(const K* __restrict__ keys,
+                                                              uint32_t n, uint32_t ntiles,
+                                                              uint32_t* __restrict__ counts,
+                                                              uint64_t* __restrict__ orand) {
+  using U = typename Bits<K>::U;
+  constexpr int NW = BLOCK / kWave;
+  constexpr uint32_t TILE = (uint32_t)BLOCK * IPT;
+  __shared__ uint32_t s_cnt[NW][kRadixBins];
+  __shared__ uint64_t s_or[NW], s_and[NW];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
+  for (uint32_t i = tid; i < NW * kRadixBins; i += BLOCK) (&s_cnt[0][0])[i] = 0;
   __syncthreads();
-  for (size_t base = (size_t)blockIdx.x * kHistBlock * J; base < n;
-       base += (size_t)gridDim.x * kHistBlock * J) {
-    typename Bits<K>::U e[J];
-    bool ok[J];
+  const size_t base = (size_t)blockIdx.x * TILE;
+  uint64_t o = 0, a = ~0ull;
+  uint32_t dig[IPT];
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const size_t i = base + (size_t)j * kHistBlock + tid;
-      ok[j] = i < n;
-      e[j] = ok[j] ? order_bits<K, MODE>(keys[i]) : 0;
-    }
+  for (int j = 0; j < IPT; ++j) {
+    const size_t i = base + (size_t)j * BLOCK + tid;
+    const U w = order_bits<K, MODE>(keys[i < n ? i : (size_t)n - 1]);
+    const bool ok = i < n;
+    o |= ok ? (uint64_t)w : 0ull;
+    a &= ok ? (uint64_t)w : ~0ull;
+    dig[j] = ok ? ((uint32_t)w & 0xffu) : 0x100u;
+  }
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-      if (!ok[j]) continue;
+  for (int j = 0; j < IPT; ++j)
+    if (dig[j] < 0x100u) atomicAdd(&s_cnt[wid][dig[j]], 1u);
 #pragma unroll
-      for (int p = 0; p < P; ++p) atomicAdd(&s_hist[wid][p][(uint32_t)(e[j] >> (8 * p)) & 0xffu], 1u);
-    }
+  for (int m = 1; m < kWave; m <<= 1) {
+    o |= __shfl_xor(o, m);
+    a &= __shfl_xor(a, m);
+  }
+  if (lane == 0) {
+    s_or[wid] = o;
+    s_and[wid] = a;
   }
   __syncthreads();
-  for (uint32_t i = tid; i < P * kRadixBins; i += kHistBlock) {
+  for (uint32_t d = tid; d < (uint32_t)kRadixBins; d += BLOCK) {
     uint32_t c = 0;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) c += (&s_hist[w][0][0])[i];
-    if (c) atomicAdd(&g_hist[i], c);
+    for (int w = 0; w < NW; ++w) c += s_cnt[w][d];
+    counts[(size_t)d * ntiles + blockIdx.x] = c;
+  }
+  if (tid == 0) {
+    uint64_t ro = 0, ra = ~0ull;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      ro |= s_or[w];
+      ra &= s_and[w];
+    }
+    orand[2 * (size_t)blockIdx.x] = ro;
+    orand[2 * (size_t)blockIdx.x + 1] = ra;
   }
 }
 
-// 2. Pass plan: which digit positions need a pass (a position whose
-//    histogram has a single bin of count n is skipped) and the source and
+// 2. Pass plan: which digit positions need a pass (a position on which the
+//    OR and the AND of all order words agree is skipped) and the source and
 //    destination of each remaining pass: caller arrays -> element buffer A0
-//    -> A1 -> A0 ... -> caller arrays.
+//    -> A1 -> A0 ... -> caller arrays. `orand` holds (OR, AND) per tile of
+//    the first upsweep.
 template <int P, bool SOA = false>
 __global__ __launch_bounds__(kRadixBins) void radix_plan_kernel(
-    const uint32_t* __restrict__ g_hist, uint32_t n, RadixPass* __restrict__ table,
+    const uint64_t* __restrict__ orand, uint32_t ntiles, uint32_t n, RadixPass* __restrict__ table,
     const void* in_k, const void* in_v, void* out_k, void* out_v, void* aos0, void* aos1) {
   __shared__ uint32_t s_trivial[P];
-  const uint32_t t = threadIdx.x;
-  for (int p = 0; p < P; ++p) {
-    const int triv = __syncthreads_or(g_hist[p * kRadixBins + t] == n);
-    if (t == 0) s_trivial[p] = triv ? 1u : 0u;
+  __shared__ uint64_t s_or[kRadixBins / kWave], s_and[kRadixBins / kWave];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
+  uint64_t o = 0, a = ~0ull;
+  for (uint32_t q = t; q < ntiles; q += kRadixBins) {
+    o |= orand[2 * (size_t)q];
+    a &= orand[2 * (size_t)q + 1];
+  }
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) {
+    o |= __shfl_xor(o, m);
+    a &= __shfl_xor(a, m);
+  }
+  if (lane == 0) {
+    s_or[wid] = o;
+    s_and[wid] = a;
+  }
+  __syncthreads();
+  if (t == 0) {
+    uint64_t ro = 0, ra = ~0ull;
+    for (int w = 0; w < kRadixBins / kWave; ++w) {
+      ro |= s_or[w];
+      ra &= s_and[w];
+    }
+    const uint64_t varying = ro ^ ra;
+    for (int p = 0; p < P; ++p) s_trivial[p] = ((varying >> (8 * p)) & 0xffu) == 0 ? 1u : 0u;
   }
   __syncthreads();
   if (t == 0) {
