@@ -242,6 +242,8 @@ __global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const RadixPass* _
                                                              uint32_t* __restrict__ counts) {
   constexpr int NW = BLOCK / kWave;
   constexpr uint32_t TILE = (uint32_t)BLOCK * IPT;
+  constexpr int KW = sizeof(K) / 4;
+  static_assert(IPT % 4 == 0, "groups of 4 elements");
   __shared__ uint32_t s_cnt[NW][kRadixBins];
   const RadixPass rp = table[pass];
   if (!rp.active) return;
@@ -250,12 +252,48 @@ __global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const RadixPass* _
   __syncthreads();
   const size_t base = (size_t)blockIdx.x * TILE;
   const uint32_t shift = rp.shift;
+  const bool full = base + TILE <= n;
+  // A full tile whose source is 16-byte aligned is read as groups of 4 whole
+  // elements (EW dwordx4 loads for an element array, KW for a key array);
+  // counting does not depend on the order in which keys are seen.
+  const bool vec = full && (((uintptr_t)rp.src_k & 15u) == 0);
   uint32_t dig[IPT];
+  if (vec) {
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const size_t i = base + (size_t)j * BLOCK + tid;
-    dig[j] = i < n ? ((uint32_t)(order_bits<K, MODE>(load_key<K, EW>(rp, i)) >> shift) & 0xffu)
-                   : 0x100u;
+    for (int r = 0; r < IPT / 4; ++r) {
+      const size_t g = base / 4 + (size_t)r * BLOCK + tid;  // group of elements 4g .. 4g+3
+      uint32_t w[4 * (EW > KW ? EW : KW)];
+      const int QW = rp.in_aos ? EW : KW;  // dwordx4 loads per group
+      const uint4* q = (const uint4*)rp.src_k + g * (size_t)QW;
+#pragma unroll
+      for (int t = 0; t < (EW > KW ? EW : KW); ++t) {
+        if (t < QW) {
+          const uint4 v = q[t];
+          w[4 * t] = v.x;
+          w[4 * t + 1] = v.y;
+          w[4 * t + 2] = v.z;
+          w[4 * t + 3] = v.w;
+        }
+      }
+      const int stride = rp.in_aos ? EW : KW;  // dwords per element in the group
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        K k;
+        if constexpr (KW == 1) {
+          k = bit_cast<K>(w[m * stride]);
+        } else {
+          k = bit_cast<K>(((uint64_t)w[m * stride + 1] << 32) | w[m * stride]);
+        }
+        dig[4 * r + m] = (uint32_t)(order_bits<K, MODE>(k) >> shift) & 0xffu;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const size_t i = base + (size_t)j * BLOCK + tid;
+      dig[j] = i < n ? ((uint32_t)(order_bits<K, MODE>(load_key<K, EW>(rp, i)) >> shift) & 0xffu)
+                     : 0x100u;
+    }
   }
 #pragma unroll
   for (int j = 0; j < IPT; ++j)
