@@ -24,8 +24,8 @@ ARCH = "gfx950"
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
-# -packed-fp32-ops: no packed FP32 arithmetic is generated (forms of it have
-# produced silently wrong results on this hardware pool).
+# -packed-fp32-ops: no packed FP32 arithmetic is generated (some of its forms
+# have given silently wrong results on gfx950).
 COMMON = [
     "-O3",
     "-std=c++17",

@@ -10,6 +10,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 
 namespace msv {
@@ -182,6 +183,30 @@ __device__ inline T shift_up1(T v, T fill) {
     out = bit_cast<T>(((uint64_t)hi << 32) | lo);
   }
   return lane == 0 ? fill : out;
+}
+
+// Lane l receives the value of lane l+1; lane 63 receives `fill`.
+template <typename T>
+__device__ inline T shift_down1(T v, T fill) {
+  const int lane = (int)__lane_id();
+  const int src = (lane == 63 ? 63 : lane + 1) << 2;
+  T out;
+  if constexpr (sizeof(T) == 4) {
+    out = bit_cast<T>((uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)bit_cast<uint32_t>(v)));
+  } else {
+    uint64_t u = bit_cast<uint64_t>(v);
+    uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)u);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)(u >> 32));
+    out = bit_cast<T>(((uint64_t)hi << 32) | lo);
+  }
+  return lane == 63 ? fill : out;
+}
+
+// The by-key family's built-in Equal: numeric == on the key dtype (bitwise
+// for integers; for floats -0.0 equals +0.0 and a NaN equals nothing).
+template <typename K>
+__device__ inline bool key_equal(K a, K b) {
+  return a == b;
 }
 
 // Inclusive wave64 scan (Hillis-Steele over DPP). Result lane l holds

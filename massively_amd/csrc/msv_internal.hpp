@@ -195,10 +195,18 @@ msv_status run_jobs_plain(Ctx& c, const msv_job* jobs, uint32_t count);
 // mode: 0 total order, 1 Less, 2 Greater; vsize 0 (keys only), 4 or 8.
 msv_status radix_sort_run(Ctx& c, msv_dtype kdt, int mode, size_t vsize, const void* keys,
                           const void* vals, uint32_t n, void* out_k, void* out_v);
-// Segmented inclusive / exclusive scan whose segment heads are the set bits
-// of `hmask` (bit i = element i starts a segment), for the by-key scans.
-msv_status mask_scan_run(Ctx& c, msv_dtype dt, msv_op op, bool exclusive, const void* in,
-                         uint32_t n, const uint32_t* hmask, void* out, const void* init);
+// The by-key scans over runs of equal keys (a run head is an element whose
+// key differs from its predecessor's): RUNS_INCLUSIVE / RUNS_EXCLUSIVE write
+// one value per element; RUNS_REDUCE writes each run's first key to out_keys
+// and init (+) the run's values to out, and the run count to both device
+// lengths.
+enum RunsKind : int { RUNS_INCLUSIVE = 0, RUNS_EXCLUSIVE = 1, RUNS_REDUCE = 2 };
+msv_status runs_scan_run(Ctx& c, msv_dtype kdt, const void* keys, msv_dtype vdt, msv_op op,
+                         int kind, const void* in, uint32_t n, void* out, const void* init,
+                         void* out_keys, uint32_t* len_keys, uint32_t* len_vals);
+// unique_by_key: the values at run heads, compacted; the count goes to *out_len.
+msv_status unique_runs_run(Ctx& c, msv_dtype kdt, const void* keys, size_t vsize,
+                           const void* vals, uint32_t n, void* out, uint32_t* out_len);
 // Enqueues a batch of validated-on-entry jobs (msv_run_jobs).
 msv_status run_jobs(Ctx& c, const msv_job* jobs, uint32_t count);
 // Stable sort of every segment of offsets[0..nseg] on its own (mode 1 Less,

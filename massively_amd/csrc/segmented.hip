@@ -7,7 +7,7 @@
 // seg/segmentation.rs:10-52, 187-264 (validation, offsets from lengths).
 //
 // Scans and reductions run over the flat value array in tiles of 256 threads
-// x 4 rows x one 16-byte vector (4096 u32 / 2048 u64). Each tile finds its
+// x 8 rows x one 16-byte vector (8192 u32 / 4096 u64). Each tile finds its
 // segment heads directly from the CSR offsets: one binary search for the
 // first offset at or after the tile start, then the threads mark the heads
 // that fall in the tile in an LDS bitmap (empty segments share their start
@@ -28,9 +28,12 @@
 namespace msv {
 
 constexpr int kSegBlock = 256;
-constexpr int kSegRows = 8;
 
-enum SegOut : int { SEG_INCL = 0, SEG_EXCL = 1, SEG_REDUCE = 2 };
+enum SegOut : int { SEG_INCL = 0, SEG_EXCL = 1, SEG_REDUCE = 2, SEG_RUNS = 3 };
+
+// Rows of 16-byte vectors per thread: 8 (8192 u32 / 4096 u64 per tile); 4
+// for SEG_RUNS, whose LDS staging would otherwise limit occupancy.
+__host__ __device__ constexpr int seg_rows(int out) { return out == SEG_RUNS ? 4 : 8; }
 enum SegMode : int { SEG_LOOKBACK = 0, SEG_AGGREGATE = 1, SEG_PREFIXED = 2 };
 
 template <typename T, int OP>
@@ -73,39 +76,76 @@ __device__ inline uint32_t wave_lower_bound(const uint32_t* a, uint32_t lo, uint
   return lo + (uint32_t)__builtin_popcountll(ballot(below));
 }
 
-// MASK: segment heads come from a bitmask (bit i set = element i starts a
-// segment; the by-key scans build it from adjacent key comparisons) instead
-// of CSR offsets.
-template <typename T, int OP, int MODE, int OUT, bool MASK = false>
+// Extra arguments of the by-key forms. HK (the key type) != void: segment
+// heads are run heads, elements whose key differs (key_equal) from the
+// previous element's, found from the keys inside each tile. SEG_RUNS
+// (reduce_by_key) also counts the heads: each run head writes its key to
+// out_keys[run], each run end (an element followed by a head, or the last
+// element) writes init (+) its segmented inclusive value to out[run], and the
+// last tile stores the run count in both device lengths.
+struct RunsArgs {
+  const void* keys = nullptr;
+  void* out_keys = nullptr;
+  uint32_t* len_keys = nullptr;
+  uint32_t* len_vals = nullptr;
+  uint64_t* cstatus = nullptr;  // look-back form: run-count tile status
+  uint32_t* tile_c = nullptr;   // fixed-shape form: tile run counts, then their exclusive prefixes
+};
+
+// V consecutive elements from p + i (16-byte aligned when full).
+template <typename K, int V>
+__device__ __forceinline__ void load_run(const K* __restrict__ p, size_t i, uint32_t n, bool full,
+                                         K (&v)[V]) {
+  if (full) {
+    struct alignas(16) Q { K v[V]; };
+    const Q q = *reinterpret_cast<const Q*>(p + i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[k] = q.v[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[k] = (i + k < n) ? p[i + k] : K();
+  }
+}
+
+template <typename T, int OP, int MODE, int OUT, typename HK = void>
 __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
     const T* __restrict__ in, uint32_t n, const uint32_t* __restrict__ offsets, uint32_t nseg,
-    T* __restrict__ out, T init, bool aligned, uint64_t* counter, uint64_t* status,
-    uint32_t gen, uint32_t ntiles, uint32_t* __restrict__ tile_f, T* __restrict__ tile_v,
-    const uint32_t* __restrict__ hmask) {
+    T* __restrict__ out, T init, bool aligned, uint64_t* status, uint32_t gen, uint32_t ntiles,
+    uint32_t* __restrict__ tile_f, T* __restrict__ tile_v, RunsArgs ra) {
   using C = Combine<T, OP>;
+  constexpr bool KEYS = !std::is_same<HK, void>::value;
+  constexpr bool RUNS = OUT == SEG_RUNS;
+  static_assert(!RUNS || KEYS, "runs need keys");
+  static_assert(!(KEYS && OUT == SEG_REDUCE), "per-segment reduce takes offsets");
+  using KT = typename std::conditional<KEYS, HK, uint32_t>::type;
   constexpr int V = 16 / sizeof(T);
   constexpr int NW = kSegBlock / kWave;
+  constexpr int kSegRows = seg_rows(OUT);
   constexpr uint32_t TILE = (uint32_t)kSegBlock * kSegRows * V;
-  __shared__ uint32_t s_heads[TILE / 32];
+  __shared__ uint32_t s_heads[KEYS ? 1 : TILE / 32];
   __shared__ uint32_t s_wf[kSegRows][NW];
   __shared__ T s_wv[kSegRows][NW];
+  __shared__ uint32_t s_wc[RUNS ? kSegRows : 1][NW];
   __shared__ uint32_t s_cf;
   __shared__ T s_cv;
+  __shared__ uint32_t s_cc;
   __shared__ uint32_t s_first;
   __shared__ T s_vals[OUT == SEG_REDUCE ? TILE : 1];
+  // SEG_RUNS: each thread parks its own keys in s_keys. Each wave's part of
+  // a row (a chunk of 64 vectors) later compacts its run keys in place in
+  // its part of s_keys and its run values in the same part of s_vals, in run
+  // order, and the chunks are then stored contiguously.
+  __shared__ KT s_keys[RUNS ? TILE : 1];
+  __shared__ T s_rvals[RUNS ? TILE : 1];
+  __shared__ uint4 s_chunk[RUNS ? kSegRows : 1][RUNS ? NW : 1];  // run base, heads, h0, ends
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
-  (void)counter;
   const uint32_t tile = blockIdx.x;
   const size_t base = (size_t)tile * TILE;
   const uint32_t tile_end = (uint32_t)min((size_t)n, base + TILE);
+  const KT* __restrict__ hk = (const KT*)ra.keys;
   uint32_t s0 = 0;
-  if constexpr (MASK) {
-    // TILE is a multiple of 32, so the tile's heads are whole mask words.
-    const uint32_t w0 = (uint32_t)(base >> 5), nw = (uint32_t)((n + 31) / 32);
-    for (uint32_t i = tid; i < TILE / 32; i += kSegBlock) s_heads[i] = w0 + i < nw ? hmask[w0 + i] : 0u;
-    __syncthreads();
-  } else {
+  if constexpr (!KEYS) {
     for (uint32_t i = tid; i < TILE / 32; i += kSegBlock) s_heads[i] = 0;
     if (wid == 0) {
       const uint32_t lb = wave_lower_bound(offsets, 0, nseg + 1, (uint32_t)base);
@@ -131,37 +171,55 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
   const bool vec_ok = aligned && (base + TILE <= n);
   const T id = C::identity();
   T x[kSegRows][V];
-  uint32_t h[kSegRows][V];
+  uint32_t hb[kSegRows];    // bit k: element k of the thread's vector in row r starts a segment
+  uint32_t ends[kSegRows];  // SEG_RUNS: bit k = element k of the vector ends a run
 #pragma unroll
   for (int r = 0; r < kSegRows; ++r) {
     const uint32_t li = ((uint32_t)r * kSegBlock + tid) * V;
-    if (vec_ok) {
-      struct alignas(16) Q { T v[V]; };
-      const Q q = *reinterpret_cast<const Q*>(in + base + li);
+    const size_t gi = base + li;
+    load_run<T, V>(in, gi, n, vec_ok, x[r]);
+    if constexpr (KEYS) {
+      KT kv[V];
+      load_run<KT, V>(hk, gi, n, vec_ok, kv);
+      // The key before the vector: the previous lane's last key (lane 0
+      // reads it; consecutive threads hold consecutive vectors).
+      const KT before = shift_up1<KT>(kv[V - 1], (lane == 0 && gi > 0 && gi <= n) ? hk[gi - 1] : KT());
+      uint32_t b = 0;
 #pragma unroll
-      for (int k = 0; k < V; ++k) x[r][k] = q.v[k];
+      for (int k = 0; k < V; ++k) {
+        const KT pk = k == 0 ? before : kv[k - 1];
+        b |= ((gi + k < n) && (gi + k == 0 || !key_equal(pk, kv[k])) ? 1u : 0u) << k;
+      }
+      hb[r] = b;
+      if constexpr (RUNS) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) s_keys[li + k] = kv[k];
+        const KT after = shift_down1<KT>(kv[0], (lane == 63 && gi + V < n) ? hk[gi + V] : KT());
+        uint32_t e = 0;
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+          const KT nk = k == V - 1 ? after : kv[k + 1];
+          const bool end = (gi + k < n) && (gi + k + 1 == n || !key_equal(kv[k], nk));
+          e |= (end ? 1u : 0u) << k;
+        }
+        ends[r] = e;
+      }
     } else {
-#pragma unroll
-      for (int k = 0; k < V; ++k) x[r][k] = (base + li + k < n) ? in[base + li + k] : id;
+      // li is a multiple of V and V divides 32: the vector's bits share a word.
+      hb[r] = (s_heads[li >> 5] >> (li & 31u)) & ((1u << V) - 1u);
     }
-#pragma unroll
-    for (int k = 0; k < V; ++k) h[r][k] = (s_heads[(li + k) >> 5] >> ((li + k) & 31u)) & 1u;
   }
 
-  // Vector-local segmented inclusive scan and wave scans of vector totals.
-  T inc[kSegRows][V];
+  // Vector totals and wave scans of them.
   uint32_t wf[kSegRows];
   T wv[kSegRows];
+  uint32_t wc[kSegRows];
 #pragma unroll
   for (int r = 0; r < kSegRows; ++r) {
-    uint32_t f = h[r][0];
+    uint32_t f = hb[r] & 1u;
     T v = x[r][0];
-    inc[r][0] = v;
 #pragma unroll
-    for (int k = 1; k < V; ++k) {
-      seg_combine<T, OP>(f, v, h[r][k], x[r][k]);
-      inc[r][k] = v;
-    }
+    for (int k = 1; k < V; ++k) seg_combine<T, OP>(f, v, (hb[r] >> k) & 1u, x[r][k]);
     wf[r] = f;
     wv[r] = v;
     wave_inclusive_seg_scan<T, OP>(wf[r], wv[r]);
@@ -169,28 +227,82 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
       s_wf[r][wid] = wf[r];
       s_wv[r][wid] = wv[r];
     }
+    if constexpr (RUNS) {
+      wc[r] = wave_inclusive_scan<uint32_t, OP_ADD>((uint32_t)__builtin_popcount(hb[r]));
+      if (lane == 63) s_wc[r][wid] = wc[r];
+    }
   }
   __syncthreads();
-  // Exclusive carry of (row r, wave wid) and the tile aggregate.
+  // Exclusive carry of (row r, wave wid) and the tile aggregate: every wave
+  // scans the kSegRows x NW partials in (row, wave) order, one per lane, and
+  // takes its own with readlane (wave-uniform values).
+  static_assert(kSegRows * NW <= kWave, "one partial per lane");
+  constexpr int NP = kSegRows * NW;
+  uint32_t qf = 0, qc = 0;
+  T qv = id;
+  if (lane < (uint32_t)NP) {
+    qf = s_wf[lane / NW][lane % NW];
+    qv = s_wv[lane / NW][lane % NW];
+    if constexpr (RUNS) qc = s_wc[lane / NW][lane % NW];
+  }
+  wave_inclusive_seg_scan<T, OP>(qf, qv);
+  const uint32_t af = readlane(qf, NP - 1);
+  const T av = readlane(qv, NP - 1);
+  const uint32_t xf = shift_up1<uint32_t>(qf, 0u);
+  const T xv = shift_up1<T>(qv, id);
+  uint32_t ac = 0, xc = 0;
+  if constexpr (RUNS) {
+    qc = wave_inclusive_scan<uint32_t, OP_ADD>(qc);
+    ac = readlane(qc, NP - 1);
+    xc = shift_up1<uint32_t>(qc, 0u);
+  }
+  const int w0 = __builtin_amdgcn_readfirstlane((int)wid);
   uint32_t pf[kSegRows];
   T pv[kSegRows];
-  uint32_t af = 0;
-  T av = id;
+  uint32_t pc[kSegRows];
 #pragma unroll
-  for (int r = 0; r < kSegRows; ++r)
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      if (w == (int)wid) {
-        pf[r] = af;
-        pv[r] = av;
+  for (int r = 0; r < kSegRows; ++r) {
+    pf[r] = readlane(xf, r * NW + w0);
+    pv[r] = readlane(xv, r * NW + w0);
+    pc[r] = RUNS ? readlane(xc, r * NW + w0) : 0u;
+  }
+
+  // Segmented aggregate of tile q from the input (the elements from its last
+  // segment start, or all of it if none): look-back self-help.
+  auto agg_of = [&](uint32_t q, uint32_t& hf, T& a) {
+    const size_t qb = (size_t)q * TILE;
+    const uint32_t qe = (uint32_t)min((size_t)n, qb + TILE);
+    size_t from = qb;
+    hf = 0;
+    if constexpr (KEYS) {
+      // Last run head of the tile, searched from the end.
+      for (int64_t wb = (int64_t)qe - kWave; wb + kWave > (int64_t)qb; wb -= kWave) {
+        const int64_t i = wb + (int64_t)lane;
+        const bool hd = i >= (int64_t)qb && (i == 0 || !key_equal(hk[i - 1], hk[i]));
+        const uint64_t any = ballot(hd);
+        if (any) {
+          from = (size_t)(wb + 63 - (int64_t)__builtin_clzll(any));
+          hf = 1;
+          break;
+        }
       }
-      seg_combine<T, OP>(af, av, s_wf[r][w], s_wv[r][w]);
+    } else {
+      const uint32_t j = lower_bound_u32(offsets, 0, nseg + 1, qe);
+      if (j > 0 && offsets[j - 1] >= qb) {
+        from = offsets[j - 1];
+        hf = 1;
+      }
     }
+    T acc = id;
+    for (size_t i = from + lane; i < qe; i += kWave) acc = C::apply(acc, in[i]);
+    a = wave_reduce<T, OP>(acc);
+  };
 
   if constexpr (MODE == SEG_AGGREGATE) {
     if (tid == 0) {
       tile_f[tile] = af;
       tile_v[tile] = av;
+      if constexpr (RUNS) ra.tile_c[tile] = ac;
     }
     return;
   } else if constexpr (MODE == SEG_LOOKBACK) {
@@ -202,39 +314,6 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
         if (lane == 0) st.publish(0, kFlagInc, af, av);
       } else {
         if (lane == 0) st.publish(tile, kFlagAgg, af, av);
-        // Self-help: the segmented aggregate of tile q from the input (the
-        // elements from its last segment start, or all of it if none).
-        auto agg_of = [&](uint32_t q, uint32_t& hf, T& a) {
-          const size_t qb = (size_t)q * TILE;
-          const uint32_t qe = (uint32_t)min((size_t)n, qb + TILE);
-          size_t from = qb;
-          hf = 0;
-          if constexpr (MASK) {
-            // Last set bit of the tile's mask words, searched from the end.
-            const uint32_t w0 = (uint32_t)(qb >> 5), w1 = (qe + 31) / 32;
-            for (int64_t wb = (int64_t)w1 - kWave; wb + kWave > (int64_t)w0; wb -= kWave) {
-              const int64_t w = wb + (int64_t)lane;
-              const uint32_t word = (w >= (int64_t)w0 && w < (int64_t)w1) ? hmask[w] : 0u;
-              const uint64_t any = ballot(word != 0);
-              if (any) {
-                const uint32_t hl = 63u - (uint32_t)__builtin_clzll(any);
-                const uint32_t hw = readlane(word, (int)hl);
-                from = (size_t)(wb + hl) * 32 + (31u - (uint32_t)__builtin_clz(hw));
-                hf = 1;
-                break;
-              }
-            }
-          } else {
-            const uint32_t j = lower_bound_u32(offsets, 0, nseg + 1, qe);
-            if (j > 0 && offsets[j - 1] >= qb) {
-              from = offsets[j - 1];
-              hf = 1;
-            }
-          }
-          T acc = id;
-          for (size_t i = from + lane; i < qe; i += kWave) acc = C::apply(acc, in[i]);
-          a = wave_reduce<T, OP>(acc);
-        };
         wave_lookback_seg_sh<T, OP>(st, tile, cf, cv, agg_of);
         uint32_t nf = cf;
         T nv = cv;
@@ -246,13 +325,41 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
         s_cv = cv;
       }
     }
+    if (RUNS && wid == 1) {
+      // The run count before this tile, on its own status words.
+      TileStatus<uint32_t> cs{ra.cstatus, gen};
+      uint32_t excl = 0;
+      if (tile == 0) {
+        if (lane == 0) cs.publish(0, kFlagInc, 0, ac);
+      } else {
+        if (lane == 0) cs.publish(tile, kFlagAgg, 0, ac);
+        auto count_of = [&](uint32_t q) -> uint32_t {
+          const size_t qb = (size_t)q * TILE;
+          const size_t qe = min((size_t)n, qb + TILE);
+          uint32_t cnt = 0;
+          for (size_t i = qb + lane; i < qe; i += kWave)
+            cnt += (i == 0 || !key_equal(hk[i - 1], hk[i])) ? 1u : 0u;
+          return wave_reduce<uint32_t, OP_ADD>(cnt);
+        };
+        excl = wave_lookback_sh<uint32_t, OP_ADD>(cs, tile, count_of);
+        if (lane == 0) cs.publish(tile, kFlagInc, 0, excl + ac);
+      }
+      if (lane == 0) s_cc = excl;
+    }
   } else {
     if (tid == 0) {
       s_cf = tile > 0 ? tile_f[tile] : 0u;
       s_cv = tile > 0 ? tile_v[tile] : id;
+      if constexpr (RUNS) s_cc = ra.tile_c[tile];
     }
   }
   __syncthreads();
+  if constexpr (RUNS) {
+    if (tid == 0 && tile == ntiles - 1) {
+      *ra.len_keys = s_cc + ac;
+      *ra.len_vals = s_cc + ac;
+    }
+  }
 
 #pragma unroll
   for (int r = 0; r < kSegRows; ++r) {
@@ -264,27 +371,50 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
     const T ev = shift_up1<T>(wv[r], id);
     seg_combine<T, OP>(cf, cv, ef, ev);
     const uint32_t li = ((uint32_t)r * kSegBlock + tid) * V;
+    // The segmented inclusive values continue from the carry (the value of
+    // the element before the vector).
     T y[V];
-    if constexpr (OUT == SEG_EXCL) {
-      T run = cv;  // segmented inclusive value of the element before
+    T run = cv;
 #pragma unroll
-      for (int k = 0; k < V; ++k) {
-        y[k] = h[r][k] ? init : C::apply(init, run);
-        run = h[r][k] ? x[r][k] : C::apply(run, x[r][k]);
-      }
-    } else {
-      // Elements before the vector's first head continue the carried segment.
-      uint32_t seen = 0;
-#pragma unroll
-      for (int k = 0; k < V; ++k) {
-        seen |= h[r][k];
-        y[k] = seen ? inc[r][k] : C::apply(cv, inc[r][k]);
-      }
+    for (int k = 0; k < V; ++k) {
+      const bool head = (hb[r] >> k) & 1u;
+      if constexpr (OUT == SEG_EXCL) y[k] = head ? init : C::apply(init, run);
+      run = head ? x[r][k] : C::apply(run, x[r][k]);
+      if constexpr (OUT != SEG_EXCL) y[k] = run;
     }
     if constexpr (OUT == SEG_REDUCE) {
 #pragma unroll
       for (int k = 0; k < V; ++k) s_vals[li + k] = y[k];
-    } else {
+    } else if constexpr (RUNS) {
+      // Runs started before this vector, then one per head; consecutive lanes
+      // write consecutive runs.
+      // This wave's chunk of the row: its heads take runs [wb, wb + wh) and
+      // its ends the runs from wb - 1 + h0 on (h0 = 1 if the chunk starts a
+      // run). Keys are read before any lane writes, so compacting them in
+      // place (rank <= position) is safe.
+      const uint32_t cbase = ((uint32_t)r * kSegBlock + wid * kWave) * V;
+      uint32_t rank = shift_up1<uint32_t>(wc[r], 0u);
+      const uint32_t h0 = readlane(hb[r], 0) & 1u;
+      KT kk[V];
+#pragma unroll
+      for (int k = 0; k < V; ++k) kk[k] = s_keys[li + k];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        const uint32_t head = (hb[r] >> k) & 1u;
+        if (head) s_keys[cbase + rank] = kk[k];
+        rank += head;
+        if ((ends[r] >> k) & 1u) s_rvals[cbase + rank - h0] = C::apply(init, y[k]);
+      }
+      // Runs that meet the chunk: wh + 1 - h0; all but the last end in it
+      // unless its last element ends a run (always so for the chunk that
+      // holds element n - 1; none meet a chunk past n).
+      const size_t cstart = base + cbase;
+      const size_t cend = cstart + (size_t)kWave * V;
+      const uint32_t wh = readlane(wc[r], kWave - 1);
+      const uint32_t elast = readlane(ends[r], kWave - 1) >> (V - 1);
+      const uint32_t last = cend >= n ? (cstart < n ? 1u : 0u) : elast;
+      if (lane == 0) s_chunk[r][wid] = make_uint4(s_cc + pc[r], wh, h0, wh - h0 + last);    } else {
       if (vec_ok) {
         struct alignas(16) Q { T v[V]; };
         Q q;
@@ -296,6 +426,19 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
         for (int k = 0; k < V; ++k)
           if (base + li + k < n) out[base + li + k] = y[k];
       }
+    }
+  }
+
+  if constexpr (RUNS) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSegRows; ++r) {
+      const uint4 ch = s_chunk[r][wid];
+      const uint32_t cbase = ((uint32_t)r * kSegBlock + wid * kWave) * V;
+      KT* ok = (KT*)ra.out_keys + ch.x;
+      for (uint32_t i = lane; i < ch.y; i += kWave) ok[i] = s_keys[cbase + i];
+      T* ov = out + (ch.x + ch.z - 1u);
+      for (uint32_t i = lane; i < ch.w; i += kWave) ov[i] = s_rvals[cbase + i];
     }
   }
 
@@ -328,22 +471,24 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
   }
 }
 
-// One-block exclusive segmented scan of tile aggregates (float path).
+// One-block exclusive segmented scan of tile aggregates (float path), with
+// an optional exclusive sum of per-tile run counts beside it.
 template <typename T, int OP>
-__global__ __launch_bounds__(1024) void seg_pairs_exclusive_kernel(const uint32_t* __restrict__ f_in,
-                                                                   const T* __restrict__ v_in,
-                                                                   uint32_t m,
-                                                                   uint32_t* __restrict__ f_out,
-                                                                   T* __restrict__ v_out) {
+__global__ __launch_bounds__(1024) void seg_pairs_exclusive_kernel(
+    const uint32_t* __restrict__ f_in, const T* __restrict__ v_in, uint32_t m,
+    uint32_t* __restrict__ f_out, T* __restrict__ v_out, const uint32_t* __restrict__ c_in,
+    uint32_t* __restrict__ c_out) {
   using C = Combine<T, OP>;
   constexpr int NW = 1024 / kWave;
   __shared__ uint32_t s_f[NW];
   __shared__ T s_v[NW];
-  __shared__ uint32_t s_cf;
+  __shared__ uint32_t s_c[NW];
+  __shared__ uint32_t s_cf, s_cc;
   __shared__ T s_cv;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
   if (tid == 0) {
     s_cf = 0;
+    s_cc = 0;
     s_cv = C::identity();
   }
   __syncthreads();
@@ -351,15 +496,21 @@ __global__ __launch_bounds__(1024) void seg_pairs_exclusive_kernel(const uint32_
     const uint32_t i = base + tid;
     uint32_t f = i < m ? f_in[i] : 0u;
     T v = i < m ? v_in[i] : C::identity();
+    const uint32_t cnt = (c_in && i < m) ? c_in[i] : 0u;
     wave_inclusive_seg_scan<T, OP>(f, v);
+    const uint32_t ci = wave_inclusive_scan<uint32_t, OP_ADD>(cnt);
     if (lane == 63) {
       s_f[wid] = f;
       s_v[wid] = v;
+      s_c[wid] = ci;
     }
     __syncthreads();
-    uint32_t cf = s_cf;
+    uint32_t cf = s_cf, cc = s_cc;
     T cv = s_cv;
-    for (uint32_t w = 0; w < wid; ++w) seg_combine<T, OP>(cf, cv, s_f[w], s_v[w]);
+    for (uint32_t w = 0; w < wid; ++w) {
+      seg_combine<T, OP>(cf, cv, s_f[w], s_v[w]);
+      cc += s_c[w];
+    }
     const uint32_t ef = shift_up1<uint32_t>(f, 0u);
     const T ev = shift_up1<T>(v, C::identity());
     uint32_t xf = cf;
@@ -368,12 +519,14 @@ __global__ __launch_bounds__(1024) void seg_pairs_exclusive_kernel(const uint32_
     if (i < m) {
       f_out[i] = xf;
       v_out[i] = xv;
+      if (c_out) c_out[i] = cc + ci - cnt;
     }
     __syncthreads();
     if (tid == 1023) {
       seg_combine<T, OP>(cf, cv, f, v);
       s_cf = cf;
       s_cv = cv;
+      s_cc = cc + ci;
     }
     __syncthreads();
   }
@@ -397,11 +550,14 @@ __global__ void validate_offsets_kernel(const uint32_t* __restrict__ offsets, ui
 }
 
 // ---- host -------------------------------------------------------------------------
-template <typename T, int OP, bool MASK = false>
+// Segmented scans / reduce over CSR offsets (HK = void) or over runs of equal
+// keys (HK = the key type, ra.keys set).
+template <typename T, int OP, typename HK = void>
 static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
                                 const uint32_t* offsets, uint32_t nseg, T* out, T init,
-                                const uint32_t* hmask = nullptr) {
-  constexpr uint32_t TILE = (uint32_t)kSegBlock * kSegRows * (16 / sizeof(T));
+                                RunsArgs ra = RunsArgs()) {
+  constexpr bool KEYS = !std::is_same<HK, void>::value;
+  const uint32_t TILE = (uint32_t)kSegBlock * seg_rows(outk) * (16 / sizeof(T));
   if (n == 0) {
     if (outk == SEG_REDUCE && nseg) {
       hipLaunchKernelGGL(fill_kernel<T>, dim3((nseg + 255) / 256), dim3(256), 0, c.stream(), out,
@@ -411,36 +567,59 @@ static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
     return MSV_OK;
   }
   const uint32_t ntiles = (n + TILE - 1) / TILE;
-  // Reduce writes per segment, so only the input needs 16-byte alignment.
-  const uintptr_t vec_ptrs = outk == SEG_REDUCE ? (uintptr_t)in : ((uintptr_t)in | (uintptr_t)out);
+  // Per-element outputs are stored as vectors; the others only need the
+  // inputs aligned.
+  const bool elem_out = outk == SEG_INCL || outk == SEG_EXCL;
+  const uintptr_t vec_ptrs = (uintptr_t)in | (elem_out ? (uintptr_t)out : 0) | (uintptr_t)ra.keys;
   const bool aligned = (vec_ptrs & 15u) == 0;
-#define MSV_SEG_LAUNCH(MODE, OUTK, CNT, ST, TF, TV)                                               \
-  hipLaunchKernelGGL((seg_tile_kernel<T, OP, MODE, OUTK, MASK>), dim3(ntiles), dim3(kSegBlock), 0, \
-                     c.stream(), in, n, offsets, nseg, out, init, aligned, CNT, ST, gen, ntiles, TF, TV, \
-                     hmask)
-#define MSV_SEG_BY_OUT(MODE, CNT, ST, TF, TV)                  \
-  if (outk == SEG_INCL) MSV_SEG_LAUNCH(MODE, SEG_INCL, CNT, ST, TF, TV);        \
-  else if (outk == SEG_EXCL) MSV_SEG_LAUNCH(MODE, SEG_EXCL, CNT, ST, TF, TV);   \
-  else if constexpr (!MASK) MSV_SEG_LAUNCH(MODE, SEG_REDUCE, CNT, ST, TF, TV);
+#define MSV_SEG_LAUNCH(MODE, OUTK, ST, TF, TV)                                                   \
+  hipLaunchKernelGGL((seg_tile_kernel<T, OP, MODE, OUTK, HK>), dim3(ntiles), dim3(kSegBlock), 0, \
+                     c.stream(), in, n, offsets, nseg, out, init, aligned, ST, gen, ntiles, TF, TV, ra)
+#define MSV_SEG_BY_OUT(MODE, ST, TF, TV)                                  \
+  if (outk == SEG_INCL) MSV_SEG_LAUNCH(MODE, SEG_INCL, ST, TF, TV);        \
+  else if (outk == SEG_EXCL) MSV_SEG_LAUNCH(MODE, SEG_EXCL, ST, TF, TV);   \
+  else if constexpr (KEYS) MSV_SEG_LAUNCH(MODE, SEG_RUNS, ST, TF, TV);     \
+  else MSV_SEG_LAUNCH(MODE, SEG_REDUCE, ST, TF, TV);
   uint32_t gen = 0;
   if constexpr (!IsFloat<T>::value) {
     uint64_t *counter, *status;
-    MSV_TRY(c.lookback((size_t)ntiles * TileStatus<T>::kWords, &counter, &status, &gen));
-    MSV_SEG_BY_OUT(SEG_LOOKBACK, counter, status, (uint32_t*)nullptr, (T*)nullptr)
+    const size_t words = (size_t)ntiles * TileStatus<T>::kWords;
+    const size_t cwords = outk == SEG_RUNS ? (size_t)ntiles : 0;
+    MSV_TRY(c.lookback(words + cwords, &counter, &status, &gen));
+    if (cwords) ra.cstatus = status + words;
+    MSV_SEG_BY_OUT(SEG_LOOKBACK, status, (uint32_t*)nullptr, (T*)nullptr)
   } else {
     uint32_t* af = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
     T* av = (T*)c.scratch(sizeof(T) * ntiles);
     uint32_t* pf = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
     T* pv = (T*)c.scratch(sizeof(T) * ntiles);
+    uint32_t *acnt = nullptr, *pcnt = nullptr;
+    if (outk == SEG_RUNS) {
+      acnt = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
+      pcnt = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
+      if (!acnt || !pcnt) return MSV_ERR_OUT_OF_MEMORY;
+    }
     if (!af || !av || !pf || !pv) return MSV_ERR_OUT_OF_MEMORY;
-    hipLaunchKernelGGL((seg_tile_kernel<T, OP, SEG_AGGREGATE, SEG_INCL, MASK>), dim3(ntiles),
-                       dim3(kSegBlock), 0, c.stream(), in, n, offsets, nseg, out, init, aligned,
-                       (uint64_t*)nullptr, (uint64_t*)nullptr, 0u, ntiles, af, av, hmask);
+    RunsArgs agg = ra;
+    agg.tile_c = acnt;
+    // The aggregate pass computes the plain segmented aggregates (its output
+    // kind only matters for the run counts).
+    if (outk == SEG_RUNS) {
+      if constexpr (KEYS)
+        hipLaunchKernelGGL((seg_tile_kernel<T, OP, SEG_AGGREGATE, SEG_RUNS, HK>), dim3(ntiles),
+                           dim3(kSegBlock), 0, c.stream(), in, n, offsets, nseg, out, init, aligned,
+                           (uint64_t*)nullptr, 0u, ntiles, af, av, agg);
+    } else {
+      hipLaunchKernelGGL((seg_tile_kernel<T, OP, SEG_AGGREGATE, SEG_INCL, HK>), dim3(ntiles),
+                         dim3(kSegBlock), 0, c.stream(), in, n, offsets, nseg, out, init, aligned,
+                         (uint64_t*)nullptr, 0u, ntiles, af, av, agg);
+    }
     MSV_HIP(hipGetLastError());
     hipLaunchKernelGGL((seg_pairs_exclusive_kernel<T, OP>), dim3(1), dim3(1024), 0, c.stream(),
-                       af, av, ntiles, pf, pv);
+                       af, av, ntiles, pf, pv, acnt, pcnt);
     MSV_HIP(hipGetLastError());
-    MSV_SEG_BY_OUT(SEG_PREFIXED, (uint64_t*)nullptr, (uint64_t*)nullptr, pf, pv)
+    ra.tile_c = pcnt;
+    MSV_SEG_BY_OUT(SEG_PREFIXED, (uint64_t*)nullptr, pf, pv)
   }
 #undef MSV_SEG_BY_OUT
 #undef MSV_SEG_LAUNCH
@@ -481,34 +660,55 @@ msv_status seg_scan_run(Ctx& c, msv_dtype dt, msv_op op, int outk, const void* i
   return MSV_ERR_INVALID_ARGUMENT;
 }
 
-template <typename T>
-static msv_status mask_scan_by_op(Ctx& c, msv_op op, int outk, const void* in, uint32_t n,
-                                  const uint32_t* hmask, void* out, const void* init_host) {
+template <typename T, typename K>
+static msv_status runs_scan_by_op(Ctx& c, msv_op op, int outk, const void* in, uint32_t n,
+                                  void* out, const void* init_host, const RunsArgs& ra) {
   T init = T();
   if (init_host) memcpy(&init, init_host, sizeof(T));
   switch (op) {
-    case MSV_OP_ADD: return seg_scan_impl<T, OP_ADD, true>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, hmask);
-    case MSV_OP_MIN: return seg_scan_impl<T, OP_MIN, true>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, hmask);
-    case MSV_OP_MAX: return seg_scan_impl<T, OP_MAX, true>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, hmask);
+    case MSV_OP_ADD: return seg_scan_impl<T, OP_ADD, K>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, ra);
+    case MSV_OP_MIN: return seg_scan_impl<T, OP_MIN, K>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, ra);
+    case MSV_OP_MAX: return seg_scan_impl<T, OP_MAX, K>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, ra);
     case MSV_OP_XOR:
       if constexpr (!IsFloat<T>::value)
-        return seg_scan_impl<T, OP_XOR, true>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, hmask);
+        return seg_scan_impl<T, OP_XOR, K>(c, outk, (const T*)in, n, nullptr, 0, (T*)out, init, ra);
       break;
   }
   set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "operator %d is not defined for this dtype", (int)op);
   return MSV_ERR_INVALID_ARGUMENT;
 }
 
-msv_status mask_scan_run(Ctx& c, msv_dtype dt, msv_op op, bool exclusive, const void* in,
-                         uint32_t n, const uint32_t* hmask, void* out, const void* init) {
-  const int outk = exclusive ? SEG_EXCL : SEG_INCL;
-  switch (dt) {
-    case MSV_U32: return mask_scan_by_op<uint32_t>(c, op, outk, in, n, hmask, out, init);
-    case MSV_I32: return mask_scan_by_op<int32_t>(c, op, outk, in, n, hmask, out, init);
-    case MSV_F32: return mask_scan_by_op<float>(c, op, outk, in, n, hmask, out, init);
-    case MSV_U64: return mask_scan_by_op<uint64_t>(c, op, outk, in, n, hmask, out, init);
-    case MSV_I64: return mask_scan_by_op<int64_t>(c, op, outk, in, n, hmask, out, init);
-    case MSV_F64: return mask_scan_by_op<double>(c, op, outk, in, n, hmask, out, init);
+template <typename K>
+static msv_status runs_scan_values(Ctx& c, msv_dtype vdt, msv_op op, int outk, const void* in,
+                                   uint32_t n, void* out, const void* init, const RunsArgs& ra) {
+  switch (vdt) {
+    case MSV_U32: return runs_scan_by_op<uint32_t, K>(c, op, outk, in, n, out, init, ra);
+    case MSV_I32: return runs_scan_by_op<int32_t, K>(c, op, outk, in, n, out, init, ra);
+    case MSV_F32: return runs_scan_by_op<float, K>(c, op, outk, in, n, out, init, ra);
+    case MSV_U64: return runs_scan_by_op<uint64_t, K>(c, op, outk, in, n, out, init, ra);
+    case MSV_I64: return runs_scan_by_op<int64_t, K>(c, op, outk, in, n, out, init, ra);
+    case MSV_F64: return runs_scan_by_op<double, K>(c, op, outk, in, n, out, init, ra);
+  }
+  set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
+  return MSV_ERR_INVALID_ARGUMENT;
+}
+
+msv_status runs_scan_run(Ctx& c, msv_dtype kdt, const void* keys, msv_dtype vdt, msv_op op,
+                         int kind, const void* in, uint32_t n, void* out, const void* init,
+                         void* out_keys, uint32_t* len_keys, uint32_t* len_vals) {
+  const int outk = kind == RUNS_INCLUSIVE ? SEG_INCL : kind == RUNS_EXCLUSIVE ? SEG_EXCL : SEG_RUNS;
+  RunsArgs ra;
+  ra.keys = keys;
+  ra.out_keys = out_keys;
+  ra.len_keys = len_keys;
+  ra.len_vals = len_vals;
+  switch (kdt) {
+    case MSV_U32: return runs_scan_values<uint32_t>(c, vdt, op, outk, in, n, out, init, ra);
+    case MSV_I32: return runs_scan_values<int32_t>(c, vdt, op, outk, in, n, out, init, ra);
+    case MSV_F32: return runs_scan_values<float>(c, vdt, op, outk, in, n, out, init, ra);
+    case MSV_U64: return runs_scan_values<uint64_t>(c, vdt, op, outk, in, n, out, init, ra);
+    case MSV_I64: return runs_scan_values<int64_t>(c, vdt, op, outk, in, n, out, init, ra);
+    case MSV_F64: return runs_scan_values<double>(c, vdt, op, outk, in, n, out, init, ra);
   }
   set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
   return MSV_ERR_INVALID_ARGUMENT;

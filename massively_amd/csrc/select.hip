@@ -48,6 +48,8 @@ __device__ inline void load4(const T* __restrict__ p, size_t i, uint32_t n, bool
 // FORM 0: u32 stencil array (flag = stencil != 0).
 // FORM 1: predicate on a separate stencil column of type S.
 // FORM 2: predicate on the input itself (S == T, one read).
+// FORM 3: run heads of a key column of type S (unique_by_key): an element is
+//         kept when it is the first or its key differs from the previous key.
 template <typename T, typename S, int FORM, int ROWS>
 __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__ in,
                                                            const S* __restrict__ stencil,
@@ -84,9 +86,20 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
       load4<S>(stencil, i, n, full, s);
 #pragma unroll
       for (int k = 0; k < 4; ++k) f[r][k] = (pred(s[k]) != invert) && (i + k < n);
-    } else {
+    } else if constexpr (FORM == 2) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) f[r][k] = (pred(bit_cast<S>(x[r][k])) != invert) && (i + k < n);
+    } else {
+      S s[4];
+      load4<S>(stencil, i, n, full, s);
+      // The key before the group: the previous lane's last key (lane 0
+      // reads it; consecutive lanes hold consecutive groups).
+      const S before = shift_up1<S>(s[3], (lane == 0 && i > 0 && i <= n) ? stencil[i - 1] : S());
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const S pk = k == 0 ? before : s[k - 1];
+        f[r][k] = (i + k < n) && (i + k == 0 || !key_equal(pk, s[k]));
+      }
     }
   }
 
@@ -124,7 +137,8 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
           bool f;
           if constexpr (FORM == 0) f = stencil[i] != 0;
           else if constexpr (FORM == 1) f = pred(stencil[i]);
-          else f = pred(bit_cast<S>(in[i]));
+          else if constexpr (FORM == 2) f = pred(bit_cast<S>(in[i]));
+          else f = i == 0 || !key_equal(stencil[i - 1], stencil[i]);
           c += (f != invert) ? 1u : 0u;
         }
         return wave_reduce<uint32_t, OP_ADD>(c);
@@ -207,6 +221,27 @@ msv_status select_pred_run(Ctx& c, msv_dtype dt, const void* in, const msv_predi
     case MSV_U64: return select_with_stencil_type<uint64_t>(c, vs, true, in, in, p, invert, n, out, out_len);
     case MSV_I64: return select_with_stencil_type<int64_t>(c, vs, true, in, in, p, invert, n, out, out_len);
     case MSV_F64: return select_with_stencil_type<double>(c, vs, true, in, in, p, invert, n, out, out_len);
+  }
+  set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
+  return MSV_ERR_INVALID_ARGUMENT;
+}
+
+template <typename S>
+static msv_status unique_with_key_type(Ctx& c, size_t vsize, const void* keys, const void* vals,
+                                       uint32_t n, void* out, uint32_t* out_len) {
+  if (vsize == 4) return select_launch<uint32_t, S, 3>(c, vals, keys, nullptr, false, n, out, out_len);
+  return select_launch<uint64_t, S, 3>(c, vals, keys, nullptr, false, n, out, out_len);
+}
+
+msv_status unique_runs_run(Ctx& c, msv_dtype kdt, const void* keys, size_t vsize,
+                           const void* vals, uint32_t n, void* out, uint32_t* out_len) {
+  switch (kdt) {
+    case MSV_U32: return unique_with_key_type<uint32_t>(c, vsize, keys, vals, n, out, out_len);
+    case MSV_I32: return unique_with_key_type<int32_t>(c, vsize, keys, vals, n, out, out_len);
+    case MSV_F32: return unique_with_key_type<float>(c, vsize, keys, vals, n, out, out_len);
+    case MSV_U64: return unique_with_key_type<uint64_t>(c, vsize, keys, vals, n, out, out_len);
+    case MSV_I64: return unique_with_key_type<int64_t>(c, vsize, keys, vals, n, out, out_len);
+    case MSV_F64: return unique_with_key_type<double>(c, vsize, keys, vals, n, out, out_len);
   }
   set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
   return MSV_ERR_INVALID_ARGUMENT;

@@ -111,3 +111,28 @@ def test_by_key_length_mismatch(exec_):
         m.vector.inclusive_scan_by_key(exec_, k, x, m.op.Add)
     with pytest.raises(m.LengthMismatch):
         m.vector.unique_by_key(exec_, k, x)
+
+
+@pytest.mark.parametrize("kdtype,vdtype", [(np.uint64, np.uint64), (np.float32, np.int64),
+                                           (np.float64, np.uint32), (np.int32, np.uint64)])
+def test_by_key_mixed_widths_and_unaligned_views(exec_, kdtype, vdtype):
+    """Key and value widths differ (the keys are read per value vector), and
+    views that start one element in (not 16-byte aligned) take the scalar
+    loads; both give the oracle's bytes."""
+    for n in (5, 2049, 4100, 20000, 300001):
+        k = _keys(kdtype, n + 1, max(1, n // 7))
+        x = RNG.integers(0, 2**63, n + 1, dtype=np.uint64).astype(vdtype)
+        dk, dx = exec_.to_device(k), exec_.to_device(x)
+        for lo in (0, 1):
+            kk, xx = k[lo:lo + n], x[lo:lo + n]
+            sk, sx = dk.slice(lo, lo + n), dx.slice(lo, lo + n)
+            got = exec_.to_host(m.vector.inclusive_scan_by_key(exec_, sk, sx, m.op.Add))
+            assert got.tobytes() == oracle.inclusive_scan_by_key(kk, xx, "add").tobytes(), (n, lo)
+            got = exec_.to_host(m.vector.exclusive_scan_by_key(exec_, sk, sx, 5, m.op.Min))
+            assert got.tobytes() == oracle.exclusive_scan_by_key(kk, xx, 5, "min").tobytes(), (n, lo)
+            rk, rv = m.vector.reduce_by_key(exec_, sk, sx, 9, m.op.Xor)
+            wk, wv = oracle.reduce_by_key(kk, xx, 9, "xor")
+            assert exec_.to_host(rk).tobytes() == wk.tobytes(), (n, lo)
+            assert exec_.to_host(rv).tobytes() == wv.tobytes(), (n, lo)
+            u = exec_.to_host(m.vector.unique_by_key(exec_, sk, sx))
+            assert u.tobytes() == oracle.unique_by_key(kk, xx).tobytes(), (n, lo)
