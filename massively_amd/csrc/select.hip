@@ -111,15 +111,19 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
     if (lane == 63) s_wave[r][wid] = ws[r];
   }
   __syncthreads();
+  // Exclusive count before (row r, wave wid) and the tile total: every wave
+  // scans the ROWS x NW partials in (row, wave) order, one per lane, and takes
+  // its own with readlane (wave-uniform values, no per-thread copies).
+  static_assert(ROWS * NW <= kWave, "one partial per lane");
+  constexpr int NP = ROWS * NW;
+  uint32_t q = lane < (uint32_t)NP ? s_wave[lane / NW][lane % NW] : 0u;
+  q = wave_inclusive_scan<uint32_t, OP_ADD>(q);
+  const uint32_t agg = readlane(q, NP - 1);
+  const uint32_t xq = shift_up1<uint32_t>(q, 0u);
+  const int w0 = __builtin_amdgcn_readfirstlane((int)wid);
   uint32_t pre[ROWS];
-  uint32_t agg = 0;
 #pragma unroll
-  for (int r = 0; r < ROWS; ++r)
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      if (w == (int)wid) pre[r] = agg;
-      agg += s_wave[r][w];
-    }
+  for (int r = 0; r < ROWS; ++r) pre[r] = readlane(xq, r * NW + w0);
 
   if (wid == 0) {
     TileStatus<uint32_t> st{status, gen};
