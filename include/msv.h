@@ -220,12 +220,24 @@ msv_status msv_remove_where(msv_executor* ex, msv_slice input, msv_slice stencil
  * may be the input itself). `invert` != 0 gives remove_where semantics. */
 msv_status msv_copy_where_pred(msv_executor* ex, msv_slice input, msv_slice stencil,
                                const msv_predicate* pred, int32_t invert, msv_vec** out);
+/* vector::partition (api/algorithm/selection.rs:245-305; core/selection.rs:414-457):
+ * stable partition, passing elements first, then failing ones, each in input
+ * order. `*boundary` receives the number of passing elements (read back, as
+ * the reference does). */
+msv_status msv_partition(msv_executor* ex, msv_slice input, const msv_predicate* pred,
+                         msv_vec** out, uint32_t* boundary);
 /* vector::gather (api/algorithm/indexed.rs:157-208): out[i] = values[indices[i]]. */
 msv_status msv_gather(msv_executor* ex, msv_slice values, msv_slice indices, msv_vec** out);
 /* vector::scatter (api/algorithm/scatter.rs:225-265): output[indices[i]] = values[i].
  * Indices are unchecked by the reference; this build rejects none but never
  * writes outside `output` (out-of-range indices are dropped). */
 msv_status msv_scatter(msv_executor* ex, msv_slice values, msv_slice indices, msv_slice output);
+/* vector::scatter_reduce (api/algorithm/scatter.rs:337-420; core/scatter_reduce.rs):
+ * output[d] = op(output[d], init (+) every values[i] with indices[i] == d), for
+ * the destinations present in `indices`; others are unchanged. `init` must be
+ * op's identity (the reference's contract). Out-of-range indices are dropped. */
+msv_status msv_scatter_reduce(msv_executor* ex, msv_slice values, msv_slice indices,
+                              const void* init, msv_op op, msv_slice output);
 
 /* ---- vector:: ordering ---------------------------------------------- */
 /* vector::radix_sort_by_key (api/algorithm/by_key.rs:252-325; core/radix.rs):
@@ -286,6 +298,21 @@ msv_status msv_seg_reduce(msv_executor* ex, msv_slice values, const msv_segmenta
 msv_status msv_seg_sort(msv_executor* ex, msv_slice keys, msv_slice values,
                         const msv_segmentation* seg, msv_cmp cmp, msv_vec** out_keys,
                         msv_vec** out_values);
+/* The compacting segment operations (seg/mod.rs:597-640 Unique, :810-846
+ * Filter, :847-880 Take, :1245-1310 owned results; seg/control.rs:555-618).
+ * Each returns the kept values (an exact-length vector; the kept count is
+ * read back, as the reference does) and a new segmentation with the same
+ * segment count whose offsets index them. Filter keeps the elements passing
+ * `pred`; Unique keeps each segment's first element and every element whose
+ * value differs (numeric ==) from the previous one; Take keeps the first
+ * `count` elements of every segment. */
+msv_status msv_seg_filter(msv_executor* ex, msv_slice values, const msv_segmentation* seg,
+                          const msv_predicate* pred, msv_vec** out_values,
+                          msv_segmentation** out_seg);
+msv_status msv_seg_unique(msv_executor* ex, msv_slice values, const msv_segmentation* seg,
+                          msv_vec** out_values, msv_segmentation** out_seg);
+msv_status msv_seg_take(msv_executor* ex, msv_slice values, const msv_segmentation* seg,
+                        uint32_t count, msv_vec** out_values, msv_segmentation** out_seg);
 
 /* ---- seeded synthetic inputs (massively/src/api/util/random.rs:12-22,331-396) */
 /* uniform_u32(min, max, seed): pcg_hash32(i ^ seed_key(seed) ^ stream) mapped

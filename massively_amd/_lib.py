@@ -178,6 +178,8 @@ PROTOTYPES = {
     "msv_copy_where_pred": (_I, [_P, _S, _S, C.POINTER(MsvPredicate), _I, _PP]),
     "msv_gather": (_I, [_P, _S, _S, _PP]),
     "msv_scatter": (_I, [_P, _S, _S, _S]),
+    "msv_scatter_reduce": (_I, [_P, _S, _S, _P, _I, _S]),
+    "msv_partition": (_I, [_P, _S, _P, _PP, C.POINTER(C.c_uint32)]),
     "msv_radix_sort_by_key": (_I, [_P, _S, _S, _PP, _PP]),
     "msv_sort_by_key": (_I, [_P, _S, _S, _I, _PP, _PP]),
     "msv_sort": (_I, [_P, _S, _I, _PP]),
@@ -194,6 +196,9 @@ PROTOTYPES = {
     "msv_seg_exclusive_scan": (_I, [_P, _S, _P, _P, _I, _PP]),
     "msv_seg_reduce": (_I, [_P, _S, _P, _P, _I, _PP]),
     "msv_seg_sort": (_I, [_P, _S, _S, _P, _I, _PP, _PP]),
+    "msv_seg_filter": (_I, [_P, _S, _P, _P, _PP, _PP]),
+    "msv_seg_unique": (_I, [_P, _S, _P, _PP, _PP]),
+    "msv_seg_take": (_I, [_P, _S, _P, C.c_uint32, _PP, _PP]),
     "msv_random_uniform_u32": (_I, [_P, _U32, _U32, _U64, _S]),
     "msv_random_uniform_u64": (_I, [_P, _U64, _U64, _U64, _S]),
     "msv_iota_u32": (_I, [_P, _U32, _S]),

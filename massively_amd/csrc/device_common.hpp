@@ -202,6 +202,29 @@ __device__ inline T shift_down1(T v, T fill) {
   return lane == 63 ? fill : out;
 }
 
+// Four consecutive elements from p + i: one or two 16-byte loads when `full`
+// (the caller has checked alignment and range), else guarded scalar loads
+// (0 past n).
+template <typename T>
+__device__ inline void load4(const T* __restrict__ p, size_t i, uint32_t n, bool full, T (&v)[4]) {
+  if (full) {
+    if constexpr (sizeof(T) == 4) {
+      const uint4 q = *reinterpret_cast<const uint4*>(p + i);
+      v[0] = bit_cast<T>(q.x); v[1] = bit_cast<T>(q.y); v[2] = bit_cast<T>(q.z); v[3] = bit_cast<T>(q.w);
+    } else {
+      const uint4 q0 = *reinterpret_cast<const uint4*>(p + i);
+      const uint4 q1 = *reinterpret_cast<const uint4*>(p + i + 2);
+      v[0] = bit_cast<T>(((uint64_t)q0.y << 32) | q0.x);
+      v[1] = bit_cast<T>(((uint64_t)q0.w << 32) | q0.z);
+      v[2] = bit_cast<T>(((uint64_t)q1.y << 32) | q1.x);
+      v[3] = bit_cast<T>(((uint64_t)q1.w << 32) | q1.z);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = (i + k < n) ? p[i + k] : (T)0;
+  }
+}
+
 // The by-key family's built-in Equal: numeric == on the key dtype (bitwise
 // for integers; for floats -0.0 equals +0.0 and a NaN equals nothing).
 template <typename K>
@@ -576,6 +599,30 @@ __device__ inline typename Bits<K>::U order_bits(K k) {
   }
   if constexpr (MODE == ORD_GREATER) e = (U)~e;
   return e;
+}
+
+// First index in [lo, hi) with a[i] >= v (hi if none), for a nondecreasing
+// array, searched by one whole wave: each round samples 64 evenly spaced
+// points and keeps the gap that holds the boundary, so a search over 2^20
+// entries takes 4 dependent loads instead of 20.
+__device__ inline uint32_t wave_lower_bound(const uint32_t* a, uint32_t lo, uint32_t hi,
+                                            uint32_t v) {
+  const uint32_t lane = __lane_id();
+  while (hi - lo > (uint32_t)kWave) {
+    const uint32_t step = (hi - lo + kWave - 1) / kWave;
+    const uint32_t idx = lo + lane * step;
+    const bool below = idx < hi && a[idx] < v;
+    const uint32_t cnt = (uint32_t)__builtin_popcountll(ballot(below));
+    // Points lo + k*step for k < cnt are below v; the answer lies after the
+    // last of them and at or before the next sampled point.
+    const uint32_t nlo = cnt ? lo + (cnt - 1) * step + 1 : lo;
+    const uint32_t nhi = min(hi, lo + cnt * step);
+    lo = nlo;
+    hi = nhi;
+  }
+  const uint32_t idx = lo + lane;
+  const bool below = idx < hi && a[idx] < v;
+  return lo + (uint32_t)__builtin_popcountll(ballot(below));
 }
 
 }  // namespace msv

@@ -183,6 +183,116 @@ msv_status iota_run(Ctx& c, uint32_t* out, uint32_t n, uint32_t start) {
   return MSV_OK;
 }
 
+
+// ---- vector::scatter_reduce ----------------------------------------------------
+// out[d] = op(out[d], init (+) the values whose index is d), for destinations
+// present in `indices` (api/algorithm/scatter.rs:337-420; the reference sorts
+// the proposals by index, reduces each run and combines each result with its
+// destination, core/scatter_reduce.rs:19-150). `init` must be op's identity,
+// as the reference requires. Integers: one pass of device-scope atomics, exact
+// whatever the order. Floats: the reference's shape (stable radix sort of
+// (index, value), reduce_by_key, one combine per destination), so results are
+// reproducible bit for bit.
+template <typename T, int OP>
+__device__ inline void atomic_combine(T* p, T v) {
+  if constexpr (OP == OP_ADD) {
+    if constexpr (sizeof(T) == 4) atomicAdd((unsigned int*)p, (unsigned int)v);
+    else atomicAdd((unsigned long long*)p, (unsigned long long)v);
+  } else if constexpr (OP == OP_XOR) {
+    if constexpr (sizeof(T) == 4) atomicXor((unsigned int*)p, (unsigned int)v);
+    else atomicXor((unsigned long long*)p, (unsigned long long)v);
+  } else if constexpr (OP == OP_MIN) {
+    if constexpr (std::is_same<T, int64_t>::value) atomicMin((long long*)p, (long long)v);
+    else if constexpr (std::is_same<T, uint64_t>::value) atomicMin((unsigned long long*)p, (unsigned long long)v);
+    else atomicMin(p, v);
+  } else {
+    if constexpr (std::is_same<T, int64_t>::value) atomicMax((long long*)p, (long long)v);
+    else if constexpr (std::is_same<T, uint64_t>::value) atomicMax((unsigned long long*)p, (unsigned long long)v);
+    else atomicMax(p, v);
+  }
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void scatter_reduce_atomic_kernel(const T* __restrict__ vals,
+                                                                    const uint32_t* __restrict__ idx,
+                                                                    uint32_t n, T* out,
+                                                                    uint32_t out_len, bool aligned) {
+  const size_t groups = ((size_t)n + 3) / 4;
+  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups;
+       g += (size_t)gridDim.x * blockDim.x) {
+    const bool full = aligned && g * 4 + 4 <= n;
+    uint32_t d[4];
+    T v[4];
+    load4<uint32_t>(idx, g * 4, n, full, d);
+    load4<T>(vals, g * 4, n, full, v);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (g * 4 + k < n && d[k] < out_len) atomic_combine<T, OP>(out + d[k], v[k]);
+  }
+}
+
+// One proposal per destination (keys distinct): out[d] = op(out[d], proposal).
+template <typename T, int OP>
+__global__ void scatter_combine_kernel(const uint32_t* __restrict__ keys, const T* __restrict__ vals,
+                                       const uint32_t* __restrict__ count, T* __restrict__ out,
+                                       uint32_t out_len) {
+  const uint32_t m = *count;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x) {
+    const uint32_t d = keys[j];
+    if (d < out_len) out[d] = Combine<T, OP>::apply(out[d], vals[j]);
+  }
+}
+
+template <typename T, int OP>
+static msv_status scatter_reduce_typed(Ctx& c, msv_dtype dt, msv_op op, const void* vals,
+                                       const uint32_t* idx, uint32_t n, const void* init,
+                                       void* out, uint32_t out_len) {
+  if (n == 0 || out_len == 0) return MSV_OK;
+  if constexpr (!IsFloat<T>::value) {
+    (void)dt;
+    (void)op;
+    (void)init;
+    const bool aligned = (((uintptr_t)vals | (uintptr_t)idx) & 15u) == 0;
+    const uint32_t groups = (n + 3) / 4;
+    const uint32_t blocks = min((groups + 255) / 256, 8192u);
+    hipLaunchKernelGGL((scatter_reduce_atomic_kernel<T, OP>), dim3(blocks), dim3(256), 0,
+                       c.stream(), (const T*)vals, idx, n, (T*)out, out_len, aligned);
+    MSV_HIP(hipGetLastError());
+    return MSV_OK;
+  } else {
+    uint32_t* sk = (uint32_t*)c.scratch((size_t)n * 4);
+    T* sv = (T*)c.scratch((size_t)n * sizeof(T));
+    uint32_t* rk = (uint32_t*)c.scratch((size_t)n * 4);
+    T* rv = (T*)c.scratch((size_t)n * sizeof(T));
+    uint32_t* cnt = (uint32_t*)c.scratch(8);
+    if (!sk || !sv || !rk || !rv || !cnt) return MSV_ERR_OUT_OF_MEMORY;
+    MSV_TRY(radix_sort_run(c, MSV_U32, 0, sizeof(T), idx, vals, n, sk, sv));
+    MSV_TRY(runs_scan_run(c, MSV_U32, sk, dt, op, RUNS_REDUCE, sv, n, rv, init, rk, cnt, cnt + 1));
+    const uint32_t blocks = min((n + 255) / 256, 8192u);
+    hipLaunchKernelGGL((scatter_combine_kernel<T, OP>), dim3(blocks), dim3(256), 0, c.stream(), rk,
+                       rv, cnt, (T*)out, out_len);
+    MSV_HIP(hipGetLastError());
+    return MSV_OK;
+  }
+}
+
+template <typename T>
+static msv_status scatter_reduce_by_op(Ctx& c, msv_dtype dt, msv_op op, const void* vals,
+                                       const uint32_t* idx, uint32_t n, const void* init,
+                                       void* out, uint32_t out_len) {
+  switch (op) {
+    case MSV_OP_ADD: return scatter_reduce_typed<T, OP_ADD>(c, dt, op, vals, idx, n, init, out, out_len);
+    case MSV_OP_MIN: return scatter_reduce_typed<T, OP_MIN>(c, dt, op, vals, idx, n, init, out, out_len);
+    case MSV_OP_MAX: return scatter_reduce_typed<T, OP_MAX>(c, dt, op, vals, idx, n, init, out, out_len);
+    case MSV_OP_XOR:
+      if constexpr (!IsFloat<T>::value)
+        return scatter_reduce_typed<T, OP_XOR>(c, dt, op, vals, idx, n, init, out, out_len);
+      break;
+  }
+  set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "operator %d is not defined for this dtype", (int)op);
+  return MSV_ERR_INVALID_ARGUMENT;
+}
+
 }  // namespace msv
 
 using namespace msv;
@@ -231,6 +341,37 @@ msv_status msv_scatter(msv_executor* ex, msv_slice values, msv_slice indices, ms
   }
   Ctx c(ex);
   return scatter_run(c, v.dtype, v.ptr, (const uint32_t*)ix.ptr, v.len, o.ptr, o.len);
+}
+
+msv_status msv_scatter_reduce(msv_executor* ex, msv_slice values, msv_slice indices,
+                              const void* init, msv_op op, msv_slice output) {
+  clear_error();
+  View v, ix, o;
+  MSV_TRY(resolve(ex, values, &v));
+  MSV_TRY(resolve(ex, indices, &ix));
+  MSV_TRY(resolve(ex, output, &o));
+  MSV_TRY(check_index_slice(ix));
+  if (v.len != ix.len) {
+    set_error(MSV_ERR_LENGTH_MISMATCH, v.len, ix.len, "input lengths differ: %u != %u", v.len,
+              ix.len);
+    return MSV_ERR_LENGTH_MISMATCH;
+  }
+  if (o.dtype != v.dtype || !init) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "output dtype differs from values, or null init");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  Ctx c(ex);
+  const uint32_t* idx = (const uint32_t*)ix.ptr;
+  switch (v.dtype) {
+    case MSV_U32: return scatter_reduce_by_op<uint32_t>(c, v.dtype, op, v.ptr, idx, v.len, init, o.ptr, o.len);
+    case MSV_I32: return scatter_reduce_by_op<int32_t>(c, v.dtype, op, v.ptr, idx, v.len, init, o.ptr, o.len);
+    case MSV_F32: return scatter_reduce_by_op<float>(c, v.dtype, op, v.ptr, idx, v.len, init, o.ptr, o.len);
+    case MSV_U64: return scatter_reduce_by_op<uint64_t>(c, v.dtype, op, v.ptr, idx, v.len, init, o.ptr, o.len);
+    case MSV_I64: return scatter_reduce_by_op<int64_t>(c, v.dtype, op, v.ptr, idx, v.len, init, o.ptr, o.len);
+    case MSV_F64: return scatter_reduce_by_op<double>(c, v.dtype, op, v.ptr, idx, v.len, init, o.ptr, o.len);
+  }
+  set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
+  return MSV_ERR_INVALID_ARGUMENT;
 }
 
 msv_status msv_random_uniform_u32(msv_executor* ex, uint32_t min, uint32_t max, uint64_t seed,

@@ -52,30 +52,6 @@ __device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint3
   return lo;
 }
 
-// First index in [lo, hi) with a[i] >= v (hi if none), for a nondecreasing
-// array, searched by one whole wave: each round samples 64 evenly spaced
-// points and keeps the gap that holds the boundary, so a search over 2^20
-// entries takes 4 dependent loads instead of 20.
-__device__ inline uint32_t wave_lower_bound(const uint32_t* a, uint32_t lo, uint32_t hi,
-                                            uint32_t v) {
-  const uint32_t lane = __lane_id();
-  while (hi - lo > (uint32_t)kWave) {
-    const uint32_t step = (hi - lo + kWave - 1) / kWave;
-    const uint32_t idx = lo + lane * step;
-    const bool below = idx < hi && a[idx] < v;
-    const uint32_t cnt = (uint32_t)__builtin_popcountll(ballot(below));
-    // Points lo + k*step for k < cnt are below v; the answer lies after the
-    // last of them and at or before the next sampled point.
-    const uint32_t nlo = cnt ? lo + (cnt - 1) * step + 1 : lo;
-    const uint32_t nhi = min(hi, lo + cnt * step);
-    lo = nlo;
-    hi = nhi;
-  }
-  const uint32_t idx = lo + lane;
-  const bool below = idx < hi && a[idx] < v;
-  return lo + (uint32_t)__builtin_popcountll(ballot(below));
-}
-
 // Extra arguments of the by-key forms. HK (the key type) != void: segment
 // heads are run heads, elements whose key differs (key_equal) from the
 // previous element's, found from the keys inside each tile. SEG_RUNS

@@ -25,32 +25,15 @@ constexpr int kSelRowsSmall = 4;
 constexpr int kSelRowsLarge = 8;
 constexpr uint32_t kSelLargeMin = 1u << 22;
 
-template <typename T>
-__device__ inline void load4(const T* __restrict__ p, size_t i, uint32_t n, bool full, T (&v)[4]) {
-  if (full) {
-    if constexpr (sizeof(T) == 4) {
-      const uint4 q = *reinterpret_cast<const uint4*>(p + i);
-      v[0] = bit_cast<T>(q.x); v[1] = bit_cast<T>(q.y); v[2] = bit_cast<T>(q.z); v[3] = bit_cast<T>(q.w);
-    } else {
-      const uint4 q0 = *reinterpret_cast<const uint4*>(p + i);
-      const uint4 q1 = *reinterpret_cast<const uint4*>(p + i + 2);
-      v[0] = bit_cast<T>(((uint64_t)q0.y << 32) | q0.x);
-      v[1] = bit_cast<T>(((uint64_t)q0.w << 32) | q0.z);
-      v[2] = bit_cast<T>(((uint64_t)q1.y << 32) | q1.x);
-      v[3] = bit_cast<T>(((uint64_t)q1.w << 32) | q1.z);
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = (i + k < n) ? p[i + k] : (T)0;
-  }
-}
-
 // FORM 0: u32 stencil array (flag = stencil != 0).
 // FORM 1: predicate on a separate stencil column of type S.
 // FORM 2: predicate on the input itself (S == T, one read).
 // FORM 3: run heads of a key column of type S (unique_by_key): an element is
 //         kept when it is the first or its key differs from the previous key.
-template <typename T, typename S, int FORM, int ROWS>
+// PART (partition): failing elements are written too, after all passing
+// ones: `part_total` holds the number of passing elements, and a failing
+// element goes to part_total + (failing elements before it).
+template <typename T, typename S, int FORM, int ROWS, bool PART = false>
 __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__ in,
                                                            const S* __restrict__ stencil,
                                                            Pred<S> pred, bool invert,
@@ -58,7 +41,8 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
                                                            T* __restrict__ out,
                                                            uint32_t* __restrict__ out_len,
                                                            uint64_t* status, uint32_t gen,
-                                                           uint32_t ntiles) {
+                                                           uint32_t ntiles,
+                                                           const uint32_t* __restrict__ part_total) {
   constexpr int NW = kSelBlock / kWave;
   constexpr uint32_t kSelTile = (uint32_t)kSelBlock * ROWS * 4;
   __shared__ T s_stage[kSelTile];
@@ -152,7 +136,7 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
     }
     if (lane == 0) {
       s_prefix = excl;
-      if (tile == ntiles - 1) *out_len = excl + agg;
+      if (out_len && tile == ntiles - 1) *out_len = excl + agg;
     }
   }
 
@@ -160,15 +144,43 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
     uint32_t pos = pre[r] + shift_up1<uint32_t>(ws[r], 0u);
+    const uint32_t j0 = ((uint32_t)r * kSelBlock + tid) * 4;  // in-tile index of the group
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (f[r][k]) s_stage[pos] = x[r][k];
+      // A failing element's rank among the tile's failing ones: the elements
+      // before it minus the passing ones.
+      if (PART && !f[r][k] && base + j0 + k < n) s_stage[agg + (j0 + k - pos)] = x[r][k];
       pos += f[r][k];
     }
   }
   __syncthreads();
   const size_t o = s_prefix;
   for (uint32_t i = tid; i < agg; i += kSelBlock) out[o + i] = s_stage[i];
+  if constexpr (PART) {
+    const uint32_t cnt = (uint32_t)min((size_t)kSelTile, (size_t)n - base);
+    const size_t fo = (size_t)*part_total + (base - o);  // failing elements before the tile
+    for (uint32_t i = agg + tid; i < cnt; i += kSelBlock)
+      if (fo + (i - agg) < n) out[fo + (i - agg)] = s_stage[i];  // always true when counts agree
+  }
+}
+
+// Number of elements passing the predicate, added to *total (zeroed before).
+template <typename S>
+__global__ __launch_bounds__(256) void pred_count_kernel(const S* __restrict__ in, uint32_t n,
+                                                         Pred<S> pred, bool aligned,
+                                                         uint32_t* __restrict__ total) {
+  const size_t groups = ((size_t)n + 3) / 4;
+  uint32_t c = 0;
+  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups;
+       g += (size_t)gridDim.x * blockDim.x) {
+    S v[4];
+    load4<S>(in, g * 4, n, aligned && g * 4 + 4 <= n, v);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c += (g * 4 + k < n && pred(v[k])) ? 1u : 0u;
+  }
+  c = wave_reduce<uint32_t, OP_ADD>(c);
+  if ((threadIdx.x & 63u) == 0 && c) atomicAdd(total, c);
 }
 
 __global__ void store_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
@@ -192,11 +204,11 @@ static msv_status select_launch(Ctx& c, const void* in, const void* st, const ms
   if (large)
     hipLaunchKernelGGL((select_kernel<T, S, FORM, kSelRowsLarge>), dim3(ntiles), dim3(kSelBlock), 0,
                        c.stream(), (const T*)in, (const S*)st, pred, invert, n, aligned, (T*)out,
-                       out_len, status, gen, ntiles);
+                       out_len, status, gen, ntiles, (const uint32_t*)nullptr);
   else
     hipLaunchKernelGGL((select_kernel<T, S, FORM, kSelRowsSmall>), dim3(ntiles), dim3(kSelBlock), 0,
                        c.stream(), (const T*)in, (const S*)st, pred, invert, n, aligned, (T*)out,
-                       out_len, status, gen, ntiles);
+                       out_len, status, gen, ntiles, (const uint32_t*)nullptr);
   MSV_HIP(hipGetLastError());
   return MSV_OK;
 }
@@ -228,6 +240,40 @@ msv_status select_pred_run(Ctx& c, msv_dtype dt, const void* in, const msv_predi
   }
   set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad dtype");
   return MSV_ERR_INVALID_ARGUMENT;
+}
+
+// vector::partition: the passing count first, then one pass that writes the
+// passing elements from 0 and the failing ones from that count on.
+template <typename S>
+static msv_status partition_launch(Ctx& c, const void* in, uint32_t n, const msv_predicate* p,
+                                   void* out, uint32_t* d_total) {
+  using U = typename Bits<S>::U;
+  hipLaunchKernelGGL(store_u32_kernel, dim3(1), dim3(1), 0, c.stream(), d_total, 0u);
+  MSV_HIP(hipGetLastError());
+  if (n == 0) return MSV_OK;
+  const Pred<S> pred = make_pred<S>(p);
+  const bool aligned = ((uintptr_t)in & 15u) == 0;
+  const uint32_t groups = (n + 3) / 4;
+  const uint32_t blocks = min((groups + 255) / 256, 4096u);
+  hipLaunchKernelGGL(pred_count_kernel<S>, dim3(blocks), dim3(256), 0, c.stream(), (const S*)in, n,
+                     pred, aligned, d_total);
+  MSV_HIP(hipGetLastError());
+  const bool large = n >= kSelLargeMin;
+  const uint32_t tile = (uint32_t)kSelBlock * (large ? kSelRowsLarge : kSelRowsSmall) * 4;
+  const uint32_t ntiles = (n + tile - 1) / tile;
+  uint64_t *counter, *status;
+  uint32_t gen;
+  MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));
+  if (large)
+    hipLaunchKernelGGL((select_kernel<U, S, 2, kSelRowsLarge, true>), dim3(ntiles), dim3(kSelBlock),
+                       0, c.stream(), (const U*)in, (const S*)in, pred, false, n, aligned, (U*)out,
+                       (uint32_t*)nullptr, status, gen, ntiles, (const uint32_t*)d_total);
+  else
+    hipLaunchKernelGGL((select_kernel<U, S, 2, kSelRowsSmall, true>), dim3(ntiles), dim3(kSelBlock),
+                       0, c.stream(), (const U*)in, (const S*)in, pred, false, n, aligned, (U*)out,
+                       (uint32_t*)nullptr, status, gen, ntiles, (const uint32_t*)d_total);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
 }
 
 template <typename S>
@@ -324,3 +370,46 @@ msv_status msv_copy_where_pred(msv_executor* ex, msv_slice input, msv_slice sten
 }
 
 }  // extern "C"
+
+extern "C" msv_status msv_partition(msv_executor* ex, msv_slice input, const msv_predicate* pred,
+                                    msv_vec** out, uint32_t* boundary) {
+  clear_error();
+  MSV_TRY(no_capture(ex, "vector::partition"));
+  View in;
+  MSV_TRY(resolve(ex, input, &in));
+  if (!pred || !out || !boundary) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null predicate or output");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  if (pred->kind > PRED_BITS_EQ) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, pred->kind, 0, "unknown predicate kind %u", pred->kind);
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  MSV_TRY(new_vec(ex, in.dtype, in.len, false, out));
+  Ctx c(ex);
+  uint32_t* d_total = (uint32_t*)c.scratch(sizeof(uint32_t));
+  msv_status s = d_total ? MSV_OK : MSV_ERR_OUT_OF_MEMORY;
+  if (s == MSV_OK) {
+    switch (in.dtype) {
+      case MSV_U32: s = partition_launch<uint32_t>(c, in.ptr, in.len, pred, (*out)->ptr, d_total); break;
+      case MSV_I32: s = partition_launch<int32_t>(c, in.ptr, in.len, pred, (*out)->ptr, d_total); break;
+      case MSV_F32: s = partition_launch<float>(c, in.ptr, in.len, pred, (*out)->ptr, d_total); break;
+      case MSV_U64: s = partition_launch<uint64_t>(c, in.ptr, in.len, pred, (*out)->ptr, d_total); break;
+      case MSV_I64: s = partition_launch<int64_t>(c, in.ptr, in.len, pred, (*out)->ptr, d_total); break;
+      default: s = partition_launch<double>(c, in.ptr, in.len, pred, (*out)->ptr, d_total); break;
+    }
+  }
+  if (s == MSV_OK) {
+    hipError_t e = hipMemcpyAsync(boundary, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, ex->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ex->stream);
+    if (e != hipSuccess) {
+      set_error(MSV_ERR_LAUNCH, 0, 0, "partition failed: %s", hipGetErrorString(e));
+      s = MSV_ERR_LAUNCH;
+    }
+  }
+  if (s != MSV_OK) {
+    msv_vec_release(*out);
+    *out = nullptr;
+  }
+  return s;
+}

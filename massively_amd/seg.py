@@ -4,9 +4,11 @@
 massively/src/seg/mod.rs:315-330. Offsets are validated into a
 :class:`Segmentation` (seg/segmentation.rs:187-264) before any kernel runs.
 
-Served algorithms: ``InclusiveScan``, ``ExclusiveScan``, ``Reduce`` and
-``Sort``. ``Sort`` accepts either a key column or a ``(keys, values)`` pair, in
-which case rows are ordered by key and the values move with them.
+Served algorithms: ``InclusiveScan``, ``ExclusiveScan``, ``Reduce``, ``Sort``
+and the compacting ``Filter``, ``Unique`` and ``Take`` (seg/mod.rs:597-880),
+which return the kept values with a new :class:`Segmentation`. ``Sort``
+accepts either a key column or a ``(keys, values)`` pair, in which case rows
+are ordered by key and the values move with them.
 """
 from __future__ import annotations
 
@@ -17,7 +19,7 @@ import numpy as np
 
 from . import _lib as L
 from .error import check
-from .op import BinaryPredicateOp, ReductionOp
+from .op import BinaryPredicateOp, PredicateOp, ReductionOp
 from .runtime import DeviceSlice, DeviceVec, Executor, NULL_SLICE, as_slice
 
 _lib = L.lib
@@ -109,6 +111,23 @@ class Sort:
     less: BinaryPredicateOp
 
 
+@dataclass(frozen=True)
+class Filter:
+    pred: PredicateOp
+
+
+@dataclass(frozen=True)
+class Unique:
+    """Unique(Equal): the built-in numeric equality."""
+
+    equal: object = None
+
+
+@dataclass(frozen=True)
+class Take:
+    count: int
+
+
 class ForEachSegment:
     def __init__(self, algorithm):
         self.algorithm = algorithm
@@ -147,4 +166,17 @@ class ForEachSegment:
                                         init.ctypes.data_as(C.c_void_p), alg.op.code,
                                         C.byref(h)))
             return DeviceVec(exec, h)
+        if isinstance(alg, (Filter, Unique, Take)):
+            hs = C.c_void_p()
+            if isinstance(alg, Filter):
+                p = alg.pred.c_struct(s.dtype)
+                check(_lib().msv_seg_filter(exec.handle, s._c(), seg.handle, C.byref(p),
+                                            C.byref(h), C.byref(hs)))
+            elif isinstance(alg, Unique):
+                check(_lib().msv_seg_unique(exec.handle, s._c(), seg.handle, C.byref(h),
+                                            C.byref(hs)))
+            else:
+                check(_lib().msv_seg_take(exec.handle, s._c(), seg.handle, int(alg.count),
+                                          C.byref(h), C.byref(hs)))
+            return SegmentIterator(DeviceVec(exec, h), Segmentation(exec, hs))
         raise TypeError(f"unsupported segment algorithm {alg!r}")

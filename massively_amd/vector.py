@@ -91,6 +91,17 @@ def remove_where_if(exec: Executor, input, stencil, pred: PredicateOp):
     return copy_where_if(exec, input, stencil, pred, invert=True)
 
 
+def partition(exec: Executor, input, pred: PredicateOp):
+    """massively/src/api/algorithm/selection.rs:245-288: stable partition,
+    passing items first. Returns (output, boundary)."""
+    src = as_slice(input)
+    p = pred.c_struct(src.dtype)
+    h = C.c_void_p()
+    boundary = C.c_uint32()
+    check(_lib().msv_partition(exec.handle, src._c(), C.byref(p), C.byref(h), C.byref(boundary)))
+    return _out(exec, h), int(boundary.value)
+
+
 def gather(exec: Executor, values, indices) -> DeviceVec:
     """massively/src/api/algorithm/indexed.rs:157-208: out[i] = values[indices[i]]."""
     h = C.c_void_p()
@@ -103,6 +114,15 @@ def scatter(exec: Executor, values, indices, output) -> None:
     """massively/src/api/algorithm/scatter.rs:248-265: output[indices[i]] = values[i]."""
     check(_lib().msv_scatter(exec.handle, as_slice(values)._c(), as_slice(indices)._c(),
                              as_slice(output)._c()))
+
+
+def scatter_reduce(exec: Executor, values, indices, init, op: ReductionOp, output) -> None:
+    """massively/src/api/algorithm/scatter.rs:337-420: output[d] = op(output[d],
+    init (+) the values aimed at d); ``init`` must be op's identity."""
+    v = as_slice(values)
+    i = np.array([init], dtype=v.dtype)
+    check(_lib().msv_scatter_reduce(exec.handle, v._c(), as_slice(indices)._c(),
+                                    i.ctypes.data_as(C.c_void_p), op.code, as_slice(output)._c()))
 
 
 def radix_sort_by_key(exec: Executor, keys, values, return_keys: bool = False):

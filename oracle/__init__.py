@@ -74,6 +74,11 @@ def lib():
         L.msvo_reduce_by_key.restype = C.c_int64
         L.msvo_unique_by_key.argtypes = [I, I, P, P, U64, P]
         L.msvo_unique_by_key.restype = C.c_int64
+        L.msvo_partition.argtypes = [I, P, U64, I, U64, U64, P]
+        L.msvo_partition.restype = U64
+        L.msvo_scatter_reduce.argtypes = [I, I, P, P, U64, P, P, U64]
+        L.msvo_seg_compact.argtypes = [I, I, P, P, U64, I, U64, U64, U64, P, P]
+        L.msvo_seg_compact.restype = U64
         L.msvo_pcg_hash32.argtypes = [C.c_uint32]
         L.msvo_pcg_hash32.restype = C.c_uint32
         L.msvo_seed_key.argtypes = [U64]
@@ -321,3 +326,47 @@ def uniform_u64(lo: int, hi: int, seed: int, n: int, start: int = 0) -> np.ndarr
     out = np.empty(n, dtype=np.uint64)
     lib().msvo_uniform_u64(lo, hi, seed, start, n, _p(out))
     return out
+
+
+def partition(x, kind: int, a_bits: int = 0, b_bits: int = 0):
+    """(out, boundary): matching elements first, then failing ones."""
+    x = _c(x)
+    out = np.empty_like(x)
+    k = lib().msvo_partition(DT[x.dtype], _p(x), x.size, kind, a_bits, b_bits, _p(out))
+    return out, int(k)
+
+
+def scatter_reduce(values, indices, init, op: str, output) -> np.ndarray:
+    v = _c(values)
+    idx = np.ascontiguousarray(indices, dtype=np.uint32)
+    out = np.array(output, dtype=v.dtype, copy=True)
+    i = np.array([init], dtype=v.dtype)
+    r = lib().msvo_scatter_reduce(DT[v.dtype], OPS[op], _p(v), _p(idx), v.size, _p(i), _p(out),
+                                  out.size)
+    assert r == 0
+    return out
+
+
+def _seg_compact(x, offsets, form: int, kind: int = 0, a_bits: int = 0, b_bits: int = 0,
+                 take: int = 0):
+    x, o = _c(x), _offs(offsets)
+    out = np.empty_like(x)
+    oo = np.empty(o.size, dtype=np.uint32)
+    k = lib().msvo_seg_compact(DT[x.dtype], form, _p(x), _p(o), o.size - 1, kind, a_bits, b_bits,
+                               take, _p(out), _p(oo))
+    return out[:k].copy(), oo
+
+
+def seg_filter(x, offsets, kind: int, a_bits: int = 0, b_bits: int = 0):
+    """(values, offsets) of ForEachSegment(Filter(pred))."""
+    return _seg_compact(x, offsets, 0, kind, a_bits, b_bits)
+
+
+def seg_unique(x, offsets):
+    """(values, offsets) of ForEachSegment(Unique(Equal))."""
+    return _seg_compact(x, offsets, 1)
+
+
+def seg_take(x, offsets, k: int):
+    """(values, offsets) of ForEachSegment(Take(k))."""
+    return _seg_compact(x, offsets, 2, take=k)

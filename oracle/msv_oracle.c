@@ -451,6 +451,91 @@ int64_t msvo_unique_by_key(int kdt, int vdt, const void* keys, const void* value
   return r;
 }
 
+/* ---- partition (vector.rs:220-235) -----------------------------------------
+ * Matching elements in input order, then failing ones in input order. Returns
+ * the number of matching elements (the boundary). */
+uint64_t msvo_partition(int dt, const void* in, uint64_t n, int kind, uint64_t a, uint64_t b,
+                        void* out) {
+  const size_t es = dt_size(dt);
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    if (pred_eval(dt, (const char*)in + i * es, kind, a, b))
+      memcpy((char*)out + k++ * es, (const char*)in + i * es, es);
+  uint64_t f = k;
+  for (uint64_t i = 0; i < n; ++i)
+    if (!pred_eval(dt, (const char*)in + i * es, kind, a, b))
+      memcpy((char*)out + f++ * es, (const char*)in + i * es, es);
+  return k;
+}
+
+/* ---- scatter_reduce (api/algorithm/scatter.rs:337-420; core/scatter_reduce.rs:
+ * 130-145 combines as op(previous, proposal)) -------------------------------
+ * The proposal for destination d is init (+) the values with index d folded
+ * in input order; out[d] = op(out[d], proposal) for every d present. Indices
+ * >= m are dropped (the GPU build never writes outside the output). */
+#define SCATTER_REDUCE(T, SUF)                                                     \
+  {                                                                                \
+    const T* v = (const T*)values;                                                 \
+    T* o = (T*)out;                                                                \
+    T* acc = (T*)malloc((m ? m : 1) * sizeof(T));                                  \
+    unsigned char* seen = (unsigned char*)calloc(m ? m : 1, 1);                    \
+    if (!acc || !seen) { free(acc); free(seen); return -1; }                       \
+    T iv;                                                                          \
+    memcpy(&iv, init, sizeof(T));                                                  \
+    for (uint64_t i = 0; i < n; ++i) {                                             \
+      const uint32_t d = idx[i];                                                   \
+      if (d >= m) continue;                                                        \
+      if (!seen[d]) { acc[d] = iv; seen[d] = 1; }                                  \
+      acc[d] = apply_##SUF(op, acc[d], v[i]);                                      \
+    }                                                                              \
+    for (uint64_t d = 0; d < m; ++d)                                               \
+      if (seen[d]) o[d] = apply_##SUF(op, o[d], acc[d]);                           \
+    free(acc);                                                                     \
+    free(seen);                                                                    \
+  }
+int msvo_scatter_reduce(int dt, int op, const void* values, const uint32_t* idx, uint64_t n,
+                        const void* init, void* out, uint64_t m) {
+  if ((dt == DT_F32 || dt == DT_F64) && op == OP_XOR) return -1;
+  DISPATCH(dt, SCATTER_REDUCE(uint32_t, u32), SCATTER_REDUCE(int32_t, i32),
+           SCATTER_REDUCE(float, f32), SCATTER_REDUCE(uint64_t, u64),
+           SCATTER_REDUCE(int64_t, i64), SCATTER_REDUCE(double, f64));
+  return 0;
+}
+
+/* ---- compacting segment operations (seg.rs:109-153) ------------------------
+ * form 0 Filter (keep the elements passing the predicate), 1 Unique (keep an
+ * element unless it equals, numeric ==, the last element kept from its
+ * segment), 2 Take (the first `take` elements of each segment). Writes the
+ * kept values and the new offsets (nseg + 1: kept elements before each
+ * segment); returns the number kept. */
+static int num_equal(int dt, const void* a, const void* b) {
+  switch (dt) {
+    case DT_F32: { float x, y; memcpy(&x, a, 4); memcpy(&y, b, 4); return x == y; }
+    case DT_F64: { double x, y; memcpy(&x, a, 8); memcpy(&y, b, 8); return x == y; }
+    default: return memcmp(a, b, dt_size(dt)) == 0;
+  }
+}
+uint64_t msvo_seg_compact(int dt, int form, const void* in, const uint32_t* offsets,
+                          uint64_t nseg, int kind, uint64_t a, uint64_t b, uint64_t take,
+                          void* out, uint32_t* out_offsets) {
+  const size_t es = dt_size(dt);
+  uint64_t k = 0;
+  for (uint64_t s = 0; s < nseg; ++s) {
+    out_offsets[s] = (uint32_t)k;
+    const uint64_t first = k;
+    for (uint64_t i = offsets[s]; i < offsets[s + 1]; ++i) {
+      const char* x = (const char*)in + i * es;
+      int keep;
+      if (form == 0) keep = pred_eval(dt, x, kind, a, b);
+      else if (form == 1) keep = k == first || !num_equal(dt, (char*)out + (k - 1) * es, x);
+      else keep = i - offsets[s] < take;
+      if (keep) memcpy((char*)out + k++ * es, x, es);
+    }
+  }
+  out_offsets[nseg] = (uint32_t)k;
+  return k;
+}
+
 /* Per-segment stable sorting permutation (global indices). */
 void msvo_seg_sort_perm(int dt, int mode, const void* keys, const uint32_t* offsets,
                         uint64_t nseg, uint32_t* perm) {

@@ -123,6 +123,23 @@ def literal_vectors() -> list[dict]:
                   values=[10, 11, 20, 30], expected_inclusive=[10, 21, 20, 30],
                   expected_exclusive=[0, 10, 0, 0], expected_reduce_keys=[1, 2, 3],
                   expected_reduce=[21, 20, 30], expected_unique=[10, 20, 30]))
+    v.append(dict(name="partition_even_doc", src="massively/src/api/algorithm/selection.rs:245-271",
+                  op="partition", dtype="u32", input=[1, 2, 3, 4], pred="even",
+                  expected=[2, 4, 1, 3], expected_boundary=2))
+    v.append(dict(name="scatter_reduce_doc", src="massively/src/api/algorithm/scatter.rs:337-376",
+                  op="scatter_reduce", dtype="u32", reduce="add", init=0, values=[2, 3, 5, 7],
+                  indices=[1, 0, 1, 1], output_before=[10, 20, 30], expected=[13, 34, 30]))
+    v.append(dict(name="scatter_reduce_owned", src="massively/tests/vector/api/owned.rs:283-295",
+                  op="scatter_reduce", dtype="u32", reduce="add", init=0, values=[1, 2],
+                  indices=[0, 0], output_before=[10, 20], expected=[13, 20]))
+    v.append(dict(name="seg_unique_owned", src="massively/tests/seg/api.rs:285-313",
+                  op="seg_unique", dtype="u32", input=[1, 1, 2, 3, 3], offsets=[0, 3, 5],
+                  expected=[1, 2, 3], expected_offsets=[0, 2, 3]))
+    v.append(dict(name="seg_take_empty_segments", src="massively/tests/seg/api.rs:766-796",
+                  op="seg_take", dtype="u32", input=[2, 4, 6, 1, 1, 3, 5, 7, 8, 8],
+                  offsets=[0, 0, 3, 6, 7, 10], take=2, expected=[2, 4, 1, 1, 5, 7, 8],
+                  expected_offsets=[0, 0, 2, 4, 5, 7], expected_take0=[],
+                  expected_offsets_take0=[0, 0, 0, 0, 0, 0]))
     v.append(dict(name="segmentation_from_lengths", src="massively/src/seg/segmentation.rs:105-128",
                   op="segmentation", lengths=[1, 2, 3], expected_offsets=[0, 1, 3, 6],
                   expected_segment_ids=[0, 1, 1, 2, 2, 2]))

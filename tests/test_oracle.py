@@ -348,3 +348,116 @@ def test_by_key_vs_numpy_runs(kdtype, op):
         assert rk.tobytes() == k[starts].tobytes()
         assert rv.tolist() == [int(r) for r in red]
         assert np.array_equal(oracle.unique_by_key(k, x), x[starts])
+
+
+# ---- partition, scatter_reduce, compacting segment operations ---------------
+
+def test_partition_golden_and_masks():
+    g = golden("partition_even_doc")
+    out, b = oracle.partition(np.array(g["input"], np.uint32), 7, 1, 0)  # (x & 1) == 0
+    assert out.tolist() == g["expected"] and b == g["expected_boundary"]
+    for dt in (np.uint32, np.int64, np.float32):
+        x = rand(dt, 3001)
+        if np.dtype(dt).kind == "f":
+            x[::97] = np.nan  # NaN < t is false: NaNs go to the failing side
+            t = np.float32(0.1)
+            kind, a = 2, int(np.array([t]).view(np.uint32)[0])
+            mask = x < t
+        else:
+            kind, a = 6, 0x10
+            mask = (x.view(np.uint32 if x.itemsize == 4 else np.uint64) & 0x10) != 0
+        out, b = oracle.partition(x, kind, a)
+        exp = np.concatenate([x[mask], x[~mask]])
+        assert b == int(mask.sum())
+        assert out.tobytes() == exp.tobytes()
+
+
+def test_scatter_reduce_golden():
+    for name in ("scatter_reduce_doc", "scatter_reduce_owned"):
+        g = golden(name)
+        out = oracle.scatter_reduce(np.array(g["values"], np.uint32), g["indices"], g["init"], "add",
+                                    np.array(g["output_before"], np.uint32))
+        assert out.tolist() == g["expected"], name
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.int32, np.uint64, np.int64])
+def test_scatter_reduce_matches_ufunc_at(dtype):
+    """Integer results do not depend on the order of combination, so numpy's
+    unbuffered ufunc.at (sequential in index order) is the same answer."""
+    n, m = 5000, 700
+    v = rand(dtype, n)
+    idx = RNG.integers(0, m + 50, n).astype(np.uint32)  # some out of range: dropped
+    base = rand(dtype, m)
+    keep = idx < m
+    for op, uf, init in (("add", np.add, 0), ("min", np.minimum, np.iinfo(dtype).max),
+                         ("max", np.maximum, np.iinfo(dtype).min), ("xor", np.bitwise_xor, 0)):
+        exp = base.copy()
+        uf.at(exp, idx[keep], v[keep])
+        got = oracle.scatter_reduce(v, idx, init, op, base)
+        assert got.tobytes() == exp.tobytes(), op
+
+
+def test_scatter_reduce_float_folds_in_input_order():
+    """op(out[d], init (+) v1 (+) v2 ...), left to right: with values whose sums
+    are exact the order does not matter; with 2^24 + 1 + 1 it does (a fold that
+    started from out[d] would round twice)."""
+    v = np.array([1.0, 1.0, 3.0], np.float32)
+    out = oracle.scatter_reduce(v, [0, 0, 1], 0.0, "add", np.array([2.0**24, 5.0], np.float32))
+    # proposal for 0 is 0 + 1 + 1 = 2, and 2^24 + 2 is exact in f32.
+    assert out.tolist() == [2.0**24 + 2.0, 8.0]
+    x = rand(np.float32, 3000)
+    idx = RNG.integers(0, 400, 3000).astype(np.uint32)
+    base = rand(np.float32, 400)
+    exp = base.copy()
+    np.minimum.at(exp, idx, x)
+    assert oracle.scatter_reduce(x, idx, np.inf, "min", base).tobytes() == exp.tobytes()
+
+
+def _segs(lengths, x):
+    off = _offsets(lengths)
+    return off, [x[off[i]:off[i + 1]] for i in range(len(lengths))]
+
+
+def _np_compact(parts):
+    lens = [len(p) for p in parts]
+    vals = np.concatenate(parts) if parts else np.array([])
+    return vals, _offsets(lens)
+
+
+def test_seg_compact_golden():
+    g = golden("seg_unique_owned")
+    vals, off = oracle.seg_unique(np.array(g["input"], np.uint32), g["offsets"])
+    assert vals.tolist() == g["expected"] and off.tolist() == g["expected_offsets"]
+    g = golden("seg_take_empty_segments")
+    x = np.array(g["input"], np.uint32)
+    vals, off = oracle.seg_take(x, g["offsets"], g["take"])
+    assert vals.tolist() == g["expected"] and off.tolist() == g["expected_offsets"]
+    vals, off = oracle.seg_take(x, g["offsets"], 0)
+    assert vals.tolist() == g["expected_take0"] and off.tolist() == g["expected_offsets_take0"]
+
+
+def test_seg_compact_matches_per_segment_numpy():
+    lengths = [0, 1, 2, 31, 32, 33, 0, 0, 127, 128, 129, 255, 256, 257, 0]
+    x = rand(np.uint32, sum(lengths), 0, 4)
+    off, segs = _segs(lengths, x)
+    vals, o = oracle.seg_filter(x, off, 2, 2)  # x < 2
+    ev, eo = _np_compact([s[s < 2] for s in segs])
+    assert vals.tobytes() == ev.astype(np.uint32).tobytes() and np.array_equal(o, eo)
+    vals, o = oracle.seg_unique(x, off)
+    ev, eo = _np_compact([s[np.r_[True, s[1:] != s[:-1]]] if len(s) else s for s in segs])
+    assert vals.tobytes() == ev.astype(np.uint32).tobytes() and np.array_equal(o, eo)
+    for k in (0, 1, 3, 32, 300):
+        vals, o = oracle.seg_take(x, off, k)
+        ev, eo = _np_compact([s[:k] for s in segs])
+        assert vals.tobytes() == ev.astype(np.uint32).tobytes() and np.array_equal(o, eo), k
+
+
+def test_seg_unique_uses_numeric_equality():
+    """-0.0 equals +0.0 (dropped), a NaN equals nothing (kept), and a segment
+    start is always kept even when it equals the previous segment's end."""
+    x = np.array([0.0, -0.0, 0.0, np.nan, np.nan, 1.0, 1.0, 1.0], np.float32)
+    vals, off = oracle.seg_unique(x, [0, 3, 6, 8])
+    # [0, -0, 0] -> [0]; [nan, nan, 1] -> all kept; [1, 1] -> [1] (its start is kept)
+    assert vals.size == 5 and not np.signbit(vals).any()
+    assert vals[0] == 0.0 and np.isnan(vals[1:3]).all() and vals[3:].tolist() == [1.0, 1.0]
+    assert off.tolist() == [0, 1, 4, 5]
