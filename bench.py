@@ -318,6 +318,58 @@ def w_bykey(c):
     return out
 
 
+def w_compact(c):
+    """seg::{Filter, Unique, Take} over the seg workload's segments (2^26 u32
+    values in [0, 4), 2^20 log-normal segments), partition of 2^28 u32 at 50%,
+    and scatter_reduce add of 2^28 u32 (and 2^26 f32) values into 2^24 uniform
+    random destinations."""
+    np, m = c.np, c.m
+    from massively_amd import seg
+
+    off = lognormal_offsets(np)
+    n, S = int(off[-1]), off.size - 1
+    sg = seg.Segmentation.from_offsets(c.ex, c.ex.to_device(off))
+    x = c.ex.alloc(np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, 3, 31, x)
+    it = seg.SegmentIterator(x, sg)
+    out = {}
+    for name, alg in (("filter_lt2", seg.Filter(m.op.Lt(2))), ("unique", seg.Unique()),
+                      ("take16", seg.Take(16))):
+        kept = [0]
+
+        def run():
+            r = seg.ForEachSegment(alg).run(c.ex, it)
+            kept[0] = r.offsets.value_count()
+
+        ms = timed(c.ex, run, c.steps, c.warmup)
+        # read the values and offsets, write the kept values and new offsets
+        out["seg_" + name] = summarize(f"seg_{name}_u32_2^26_in_2^20", ms,
+                                       4.0 * n + 4.0 * kept[0] + 8.0 * (S + 1), n, kept=kept[0])
+    N = 1 << 28
+    y = c.ex.alloc(np.uint32, N)
+    c.mr.uniform_u32(c.ex, 0, 2**32 - 1, 32, y)
+    ms = timed(c.ex, lambda: m.vector.partition(c.ex, y, m.op.Lt(1 << 31)), c.steps, c.warmup)
+    # count pass (read) + partition pass (read + write)
+    out["partition"] = summarize("partition_u32_50pct_2^28", ms, 12.0 * N, N)
+    D = 1 << 24
+    idx = c.ex.alloc(np.uint32, N)
+    c.mr.uniform_u32(c.ex, 0, D - 1, 33, idx)
+    dst = c.ex.alloc(np.uint32, D)
+    ms = timed(c.ex, lambda: m.vector.scatter_reduce(c.ex, y, idx, 0, m.op.Add, dst), c.steps, c.warmup)
+    out["scatter_reduce_u32"] = summarize("scatter_reduce_add_u32_2^28_into_2^24", ms,
+                                          8.0 * N + 8.0 * D, N)
+    Nf = 1 << 26
+    fv = c.ex.alloc(np.uint32, Nf)
+    c.mr.uniform_u32(c.ex, 0x3F000000, 0x3F7FFFFF, 34, fv)
+    fvf = c.ex.wrap(fv.device_ptr, np.float32, Nf)
+    fi = c.ex.wrap(idx.device_ptr, np.uint32, Nf)
+    fdst = c.ex.alloc(np.float32, D)
+    ms = timed(c.ex, lambda: m.vector.scatter_reduce(c.ex, fvf, fi, 0.0, m.op.Add, fdst), c.steps, c.warmup)
+    out["scatter_reduce_f32"] = summarize("scatter_reduce_add_f32_2^26_into_2^24_sorted", ms,
+                                          8.0 * Nf + 8.0 * D, Nf)
+    return out
+
+
 def served_trace(np, jobs=500, seed=10, log2_max=26):
     """The served mixed stream: kinds uniform over sort_by_key / inclusive_scan /
     reduce / copy_where, sizes log-uniform in 2^10..2^log2_max, key dtypes
@@ -497,6 +549,7 @@ WORKLOADS = {
     "merge_sets": w_merge_sets,
     "seg": w_seg,
     "bykey": w_bykey,
+    "compact": w_compact,
     "served": w_served,
 }
 
