@@ -245,6 +245,14 @@ msv_status msv_scatter_reduce(msv_executor* ex, msv_slice values, msv_slice indi
  * values; `out_keys` (nullable) also receives the sorted keys. */
 msv_status msv_radix_sort_by_key(msv_executor* ex, msv_slice keys, msv_slice values,
                                  msv_vec** out_values, msv_vec** out_keys);
+/* vector::radix_sort_by_key over zipped columns (core/radix.rs:499-531;
+ * api/iter.rs:380-388; tests/vector/api/radix.rs:118-242): 1..3 key columns in
+ * lexicographic order (leftmost primary; each column in integer / IEEE total
+ * order), stable; 1..12 value columns, all reordered by the one permutation.
+ * `out_values` receives nvalues new vectors. */
+msv_status msv_radix_sort_by_key_columns(msv_executor* ex, const msv_slice* keys, uint32_t nkeys,
+                                         const msv_slice* values, uint32_t nvalues,
+                                         msv_vec** out_values);
 /* vector::sort_by_key (api/algorithm/by_key.rs:193-250): stable sort by the
  * built-in comparison; float keys compare with IEEE `<` (-0 == +0). */
 msv_status msv_sort_by_key(msv_executor* ex, msv_slice keys, msv_slice values, msv_cmp cmp,

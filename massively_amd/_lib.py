@@ -180,6 +180,8 @@ PROTOTYPES = {
     "msv_scatter": (_I, [_P, _S, _S, _S]),
     "msv_scatter_reduce": (_I, [_P, _S, _S, _P, _I, _S]),
     "msv_partition": (_I, [_P, _S, _P, _PP, C.POINTER(C.c_uint32)]),
+    "msv_radix_sort_by_key_columns": (_I, [_P, C.POINTER(MsvSlice), C.c_uint32,
+                                           C.POINTER(MsvSlice), C.c_uint32, _PP]),
     "msv_radix_sort_by_key": (_I, [_P, _S, _S, _PP, _PP]),
     "msv_sort_by_key": (_I, [_P, _S, _S, _I, _PP, _PP]),
     "msv_sort": (_I, [_P, _S, _I, _PP]),

@@ -116,6 +116,85 @@ msv_status radix_sort_run(Ctx& c, msv_dtype kdt, int mode, size_t vsize, const v
   return MSV_ERR_INVALID_ARGUMENT;
 }
 
+
+// ---- compound keys (up to 3 columns, lexicographic) ------------------------------
+// The reference sorts a zip of key columns with one stable radix pass set per
+// column, rightmost column first (core/radix.rs:499-531; api/iter.rs:380-388:
+// up to three columns, leftmost primary). Here the order-preserving words of
+// adjacent columns are packed into one u64 whenever they fit (two u32
+// columns: one 64-bit sort instead of two 32-bit ones), so a key needs one
+// stable sort per 64-bit group, rightmost group first, each carrying the row
+// permutation; value columns are then gathered once.
+struct KeyColumn {
+  const void* p;
+  int dt;    // msv_dtype
+  int bits;  // 32 or 64
+};
+
+template <typename K>
+__device__ inline uint64_t order_bits(const void* p, uint32_t i) {
+  return (uint64_t)radix_encode<K>(((const K*)p)[i]);
+}
+
+__device__ inline uint64_t column_order_word(const KeyColumn& c, uint32_t i) {
+  switch (c.dt) {
+    case MSV_U32: return order_bits<uint32_t>(c.p, i);
+    case MSV_I32: return order_bits<int32_t>(c.p, i);
+    case MSV_F32: return order_bits<float>(c.p, i);
+    case MSV_U64: return order_bits<uint64_t>(c.p, i);
+    case MSV_I64: return order_bits<int64_t>(c.p, i);
+    default: return order_bits<double>(c.p, i);
+  }
+}
+
+// out[i] = the packed order words of row perm[i] (or i) over columns [0, ncols).
+__global__ __launch_bounds__(256) void pack_key_group_kernel(KeyColumn c0, KeyColumn c1, int ncols,
+                                                             const uint32_t* __restrict__ perm,
+                                                             uint32_t n, uint64_t* __restrict__ out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t r = perm ? perm[i] : i;
+    uint64_t w = column_order_word(c0, r);
+    if (ncols == 2) w = (w << c1.bits) | column_order_word(c1, r);
+    out[i] = w;
+  }
+}
+
+// Row permutation of a stable lexicographic sort by up to 3 key columns.
+static msv_status compound_key_perm(Ctx& c, const KeyColumn* cols, int ncols, uint32_t n,
+                                    uint32_t* perm) {
+  // Groups of adjacent columns of at most 64 bits, formed from the right.
+  int gstart[3], gcount[3], ng = 0;
+  for (int j = ncols - 1; j >= 0;) {
+    int bits = cols[j].bits, first = j;
+    while (first > 0 && bits + cols[first - 1].bits <= 64) bits += cols[--first].bits;
+    gstart[ng] = first;
+    gcount[ng] = j - first + 1;
+    ++ng;
+    j = first - 1;
+  }
+  uint64_t* w = (uint64_t*)c.scratch((size_t)n * 8);
+  uint64_t* wsorted = (uint64_t*)c.scratch((size_t)n * 8);
+  uint32_t* tmp = (uint32_t*)c.scratch((size_t)n * 4);
+  if (!w || !wsorted || !tmp) return MSV_ERR_OUT_OF_MEMORY;
+  const uint32_t blocks = min((n + 255) / 256, 16384u);
+  MSV_TRY(iota_run(c, tmp, n, 0));
+  uint32_t* cur = tmp;   // permutation so far (rows in the order sorted by the groups done)
+  uint32_t* next = perm;
+  for (int g = 0; g < ng; ++g) {
+    const KeyColumn a = cols[gstart[g]];
+    const KeyColumn b = gcount[g] == 2 ? cols[gstart[g] + 1] : KeyColumn{nullptr, MSV_U32, 32};
+    hipLaunchKernelGGL(pack_key_group_kernel, dim3(blocks), dim3(256), 0, c.stream(), a, b,
+                       gcount[g], g == 0 ? (const uint32_t*)nullptr : cur, n, w);
+    MSV_HIP(hipGetLastError());
+    MSV_TRY(radix_sort_run(c, MSV_U64, ORD_TOTAL, 4, w, cur, n, wsorted, next));
+    uint32_t* t = cur;
+    cur = next;
+    next = t;
+  }
+  if (cur != perm)
+    MSV_HIP(hipMemcpyAsync(perm, cur, (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream()));
+  return MSV_OK;
+}
 }  // namespace msv
 
 using namespace msv;
@@ -174,6 +253,53 @@ extern "C" {
 msv_status msv_radix_sort_by_key(msv_executor* ex, msv_slice keys, msv_slice values,
                                  msv_vec** out_values, msv_vec** out_keys) {
   return sort_api(ex, keys, &values, ORD_TOTAL, out_values, out_keys);
+}
+
+msv_status msv_radix_sort_by_key_columns(msv_executor* ex, const msv_slice* keys, uint32_t nkeys,
+                                         const msv_slice* values, uint32_t nvalues,
+                                         msv_vec** out_values) {
+  clear_error();
+  if (!keys || !values || !out_values || nkeys < 1 || nkeys > 3 || nvalues < 1 || nvalues > 12) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, nkeys, nvalues,
+              "need 1..3 key columns and 1..12 value columns (got %u, %u)", nkeys, nvalues);
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  View kv[3], vv[12];
+  for (uint32_t j = 0; j < nkeys; ++j) MSV_TRY(resolve(ex, keys[j], &kv[j]));
+  for (uint32_t j = 0; j < nvalues; ++j) MSV_TRY(resolve(ex, values[j], &vv[j]));
+  const uint32_t n = kv[0].len;
+  for (uint32_t j = 1; j < nkeys; ++j)
+    if (kv[j].len != n) {
+      set_error(MSV_ERR_LENGTH_MISMATCH, n, kv[j].len, "input lengths differ: %u != %u", n, kv[j].len);
+      return MSV_ERR_LENGTH_MISMATCH;
+    }
+  for (uint32_t j = 0; j < nvalues; ++j)
+    if (vv[j].len != n) {
+      set_error(MSV_ERR_LENGTH_MISMATCH, n, vv[j].len, "input lengths differ: %u != %u", n, vv[j].len);
+      return MSV_ERR_LENGTH_MISMATCH;
+    }
+  for (uint32_t j = 0; j < nvalues; ++j) out_values[j] = nullptr;
+  msv_status s = MSV_OK;
+  for (uint32_t j = 0; j < nvalues && s == MSV_OK; ++j)
+    s = new_vec(ex, vv[j].dtype, n, false, &out_values[j]);
+  Ctx c(ex);
+  if (s == MSV_OK && n > 0) {
+    KeyColumn cols[3];
+    for (uint32_t j = 0; j < nkeys; ++j)
+      cols[j] = KeyColumn{kv[j].ptr, (int)kv[j].dtype, (int)dtype_size(kv[j].dtype) * 8};
+    uint32_t* perm = (uint32_t*)c.scratch((size_t)n * 4);
+    s = perm ? compound_key_perm(c, cols, (int)nkeys, n, perm) : MSV_ERR_OUT_OF_MEMORY;
+    for (uint32_t j = 0; j < nvalues && s == MSV_OK; ++j)
+      s = gather_run(c, vv[j].dtype, vv[j].ptr, n, perm, out_values[j]->ptr, n);
+  }
+  if (s != MSV_OK) {
+    for (uint32_t j = 0; j < nvalues; ++j)
+      if (out_values[j]) {
+        msv_vec_release(out_values[j]);
+        out_values[j] = nullptr;
+      }
+  }
+  return s;
 }
 
 msv_status msv_sort_by_key(msv_executor* ex, msv_slice keys, msv_slice values, msv_cmp cmp,

@@ -129,7 +129,22 @@ def radix_sort_by_key(exec: Executor, keys, values, return_keys: bool = False):
     """massively/src/api/algorithm/by_key.rs:252-297: stable, integer/IEEE total order.
 
     Returns the permuted values (and the sorted keys when ``return_keys``).
+    ``keys`` and ``values`` may also be tuples of columns (the reference's
+    ``zip2``/``zip3`` keys and up to ``zip12`` values, core/radix.rs:499-531):
+    keys sort lexicographically, leftmost column first, and the result is the
+    tuple of permuted value columns.
     """
+    if isinstance(keys, (tuple, list)) or isinstance(values, (tuple, list)):
+        if return_keys:
+            raise ValueError("return_keys is not available for zipped columns")
+        kc = [as_slice(k)._c() for k in (keys if isinstance(keys, (tuple, list)) else (keys,))]
+        vc = [as_slice(v)._c() for v in (values if isinstance(values, (tuple, list)) else (values,))]
+        karr = (L.MsvSlice * len(kc))(*kc)
+        varr = (L.MsvSlice * len(vc))(*vc)
+        outs = (C.c_void_p * len(vc))()
+        check(_lib().msv_radix_sort_by_key_columns(exec.handle, karr, len(kc), varr, len(vc), outs))
+        res = tuple(_out(exec, C.c_void_p(h)) for h in outs)
+        return res if isinstance(values, (tuple, list)) else res[0]
     hv, hk = C.c_void_p(), C.c_void_p()
     check(_lib().msv_radix_sort_by_key(exec.handle, as_slice(keys)._c(), as_slice(values)._c(),
                                        C.byref(hv), C.byref(hk) if return_keys else None))

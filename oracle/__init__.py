@@ -370,3 +370,10 @@ def seg_unique(x, offsets):
 def seg_take(x, offsets, k: int):
     """(values, offsets) of ForEachSegment(Take(k))."""
     return _seg_compact(x, offsets, 2, take=k)
+
+
+def compound_sort_perm(columns) -> np.ndarray:
+    """Stable lexicographic row order over up to 3 key columns, leftmost
+    primary, each column in integer / IEEE total order (core/radix.rs:499-531)."""
+    words = [order_words(col, "total") for col in columns]
+    return np.lexsort(tuple(reversed(words))).astype(np.uint32)

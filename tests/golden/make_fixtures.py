@@ -123,6 +123,21 @@ def literal_vectors() -> list[dict]:
                   values=[10, 11, 20, 30], expected_inclusive=[10, 21, 20, 30],
                   expected_exclusive=[0, 10, 0, 0], expected_reduce_keys=[1, 2, 3],
                   expected_reduce=[21, 20, 30], expected_unique=[10, 20, 30]))
+    first = [1, -1, 1, -1, 1, -1, 1, -1, 1]
+    second = [2, 2, 1, 2, 1, 1, 1, 1, 1]
+    third = [0, 0, 2, 0, 1, 2, 1, 1, 1]
+    order = sorted(range(9), key=lambda i: (first[i], second[i], third[i]))
+    v.append(dict(name="radix_compound_keys_seven_values", src="massively/tests/vector/api/radix.rs:118-170",
+                  op="radix_columns", key_dtypes=["i32", "u32", "u32"], keys=[first, second, third],
+                  values=[[c * 100 + i for i in range(9)] for c in range(7)],
+                  expected=[[c * 100 + i for i in order] for c in range(7)]))
+    rows = [[0, 0, 0], [0, 0, 1], [0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 0], [0, 1, 1]]
+    order = sorted(range(7), key=lambda i: rows[i])
+    v.append(dict(name="radix_three_keys_twelve_values", src="massively/tests/vector/api/radix.rs:172-242",
+                  op="radix_columns", key_dtypes=["u32", "u32", "u32"],
+                  keys=[[r[c] for r in rows] for c in range(3)],
+                  values=[[c * 100 + i for i in range(7)] for c in range(12)],
+                  expected=[[c * 100 + i for i in order] for c in range(12)]))
     v.append(dict(name="partition_even_doc", src="massively/src/api/algorithm/selection.rs:245-271",
                   op="partition", dtype="u32", input=[1, 2, 3, 4], pred="even",
                   expected=[2, 4, 1, 3], expected_boundary=2))

@@ -144,3 +144,42 @@ def test_sort_by_key_u64_2p27(exec_, dist):
     mr.iota_u32(exec_, vals)
     out = exec_.to_host(m.vector.radix_sort_by_key(exec_, keys, vals))
     _check_stable_perm(hk, out)
+
+
+# ---- compound keys: zipped key columns (up to 3) and value columns (up to 12) ----
+
+def test_radix_columns_golden(exec_):
+    from tests.conftest import NP
+
+    for name in ("radix_compound_keys_seven_values", "radix_three_keys_twelve_values"):
+        g = golden(name)
+        keys = tuple(exec_.to_device(np.array(k, NP[d])) for k, d in zip(g["keys"], g["key_dtypes"]))
+        vals = tuple(exec_.to_device(np.array(v, np.uint32)) for v in g["values"])
+        out = m.vector.radix_sort_by_key(exec_, keys, vals)
+        assert [exec_.to_host(o).tolist() for o in out] == g["expected"], name
+
+
+@pytest.mark.parametrize("kdts", [(np.uint32,), (np.uint32, np.uint32), (np.int32, np.uint32, np.uint32),
+                                  (np.uint64, np.float32), (np.float64, np.int64, np.uint32)])
+def test_radix_columns_match_oracle(exec_, kdts):
+    rng = np.random.default_rng(len(kdts) * 7 + sum(np.dtype(d).itemsize for d in kdts))
+    for n in (0, 1, 1000, 70001, 300000):
+        cols = []
+        for d in kdts:
+            dt = np.dtype(d)
+            if dt.kind == "f":
+                c = rng.integers(-3, 4, n).astype(dt)  # many ties; +-0 and NaN below
+                c[rng.random(n) < 0.05] = -0.0
+                c[rng.random(n) < 0.02] = np.nan
+            else:
+                c = rng.integers(0, 5, n).astype(dt)
+                if dt.kind == "i":
+                    c -= 2
+            cols.append(c)
+        vals = [np.arange(n, dtype=np.uint32), rng.standard_normal(n)]
+        perm = oracle.compound_sort_perm(cols)
+        dk = tuple(exec_.to_device(c) if n else exec_.alloc(c.dtype, 0) for c in cols)
+        dv = tuple(exec_.to_device(v) if n else exec_.alloc(v.dtype, 0) for v in vals)
+        out = m.vector.radix_sort_by_key(exec_, dk, dv)
+        for o, v in zip(out, vals):
+            assert exec_.to_host(o).tobytes() == v[perm].tobytes(), (kdts, n)

@@ -461,3 +461,24 @@ def test_seg_unique_uses_numeric_equality():
     assert vals.size == 5 and not np.signbit(vals).any()
     assert vals[0] == 0.0 and np.isnan(vals[1:3]).all() and vals[3:].tolist() == [1.0, 1.0]
     assert off.tolist() == [0, 1, 4, 5]
+
+
+def test_compound_sort_perm_golden_and_tuples():
+    for name in ("radix_compound_keys_seven_values", "radix_three_keys_twelve_values"):
+        g = golden(name)
+        cols = [np.array(k, NP[d]) for k, d in zip(g["keys"], g["key_dtypes"])]
+        perm = oracle.compound_sort_perm(cols)
+        for vals, exp in zip(g["values"], g["expected"]):
+            assert np.array(vals)[perm].tolist() == exp, name
+    # Against Python's stable sort on tuples of total-order words (i64, f32 with
+    # NaN and +-0, u32): a reversed column order or an unstable sort fails.
+    n = 3000
+    a = rand(np.int64, n, -3, 3)
+    b = rand(np.float32, n)
+    b[::7] = np.float32(0.0)
+    b[::11] = np.float32(-0.0)
+    b[::13] = np.nan
+    c = rand(np.uint32, n, 0, 3)
+    wa, wb, wc = (oracle.order_words(x, "total") for x in (a, b, c))
+    want = sorted(range(n), key=lambda i: (int(wa[i]), int(wb[i]), int(wc[i])))
+    assert oracle.compound_sort_perm([a, b, c]).tolist() == want
