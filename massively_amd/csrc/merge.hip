@@ -32,6 +32,42 @@ constexpr uint32_t kMergeTile = kMergeBlock * kMergeIpt;
 constexpr int kSetIpt = 32;
 constexpr uint32_t kSetTile = kMergeBlock * kSetIpt;
 
+// LDS windows are indexed with one padding word per 32 (x -> x + x/32):
+// every thread walks its own range, so the lanes of a wave read at a stride
+// of IPT/2 .. IPT elements, which without padding would hit 2-4 of the 32 LDS
+// banks (16- to 32-way conflicts on every step of the walk).
+__device__ __forceinline__ uint32_t lds_pad(uint32_t x) { return x + (x >> 5); }
+template <uint32_t N>
+struct LdsPadded { static constexpr uint32_t value = N + N / 32 + 1; };
+
+// A window of LDS keys starting at logical index `off`, read through lds_pad.
+template <typename K>
+struct PadWindow {
+  const K* base;
+  uint32_t off;
+  __device__ __forceinline__ K operator[](uint32_t i) const { return base[lds_pad(off + i)]; }
+};
+
+// Loads the two input windows A[i0, i0 + la) ++ B[j0, j0 + lb) into padded
+// LDS: ITEMS loads per thread, all issued before the first LDS store (a loop
+// with a store after every load waits for each load in turn).
+template <typename K, int ITEMS>
+__device__ __forceinline__ void load_windows(const K* __restrict__ A, uint32_t i0, uint32_t la,
+                                             const K* __restrict__ B, uint32_t j0, uint32_t L,
+                                             K* s_keys, uint32_t tid) {
+  K v[ITEMS];
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const uint32_t k = tid + (uint32_t)q * kMergeBlock;
+    v[q] = k < la ? A[i0 + k] : (k < L ? B[j0 + k - la] : K());
+  }
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const uint32_t k = tid + (uint32_t)q * kMergeBlock;
+    if (k < L) s_keys[lds_pad(k)] = v[q];
+  }
+}
+
 // Keys are compared through their order words (order_bits).
 template <typename K, int MODE>
 struct Ord {
@@ -40,8 +76,8 @@ struct Ord {
 
 // Number of A elements among the first d outputs of the stable merge
 // (ties: A first). Works on global or LDS pointers.
-template <typename K, int MODE>
-__device__ inline uint32_t merge_path(const K* A, uint32_t na, const K* B, uint32_t nb, uint32_t d) {
+template <typename K, int MODE, typename AA, typename BA>
+__device__ inline uint32_t merge_path(const AA& A, uint32_t na, const BA& B, uint32_t nb, uint32_t d) {
   uint32_t lo = d > nb ? d - nb : 0;
   uint32_t hi = d < na ? d : na;
   while (lo < hi) {
@@ -52,8 +88,8 @@ __device__ inline uint32_t merge_path(const K* A, uint32_t na, const K* B, uint3
   return lo;
 }
 
-template <typename K, int MODE>
-__device__ inline uint32_t lower_bound_w(const K* A, uint32_t lo, uint32_t hi,
+template <typename K, int MODE, typename AA>
+__device__ inline uint32_t lower_bound_w(const AA& A, uint32_t lo, uint32_t hi,
                                          typename Bits<K>::U v) {
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -63,8 +99,8 @@ __device__ inline uint32_t lower_bound_w(const K* A, uint32_t lo, uint32_t hi,
   return lo;
 }
 
-template <typename K, int MODE>
-__device__ inline uint32_t upper_bound_w(const K* A, uint32_t lo, uint32_t hi,
+template <typename K, int MODE, typename AA>
+__device__ inline uint32_t upper_bound_w(const AA& A, uint32_t lo, uint32_t hi,
                                          typename Bits<K>::U v) {
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -76,8 +112,8 @@ __device__ inline uint32_t upper_bound_w(const K* A, uint32_t lo, uint32_t hi,
 
 // Merge-path split at diagonal d, moved to the nearest pair-aligned point of
 // the oracle's set walk when it falls inside a run of equal keys.
-template <typename K, int MODE>
-__device__ inline void set_split(const K* A, uint32_t na, const K* B, uint32_t nb, uint32_t d,
+template <typename K, int MODE, typename AA, typename BA>
+__device__ inline void set_split(const AA& A, uint32_t na, const BA& B, uint32_t nb, uint32_t d,
                                  uint32_t& i_out, uint32_t& j_out) {
   using U = typename Bits<K>::U;
   uint32_t i = merge_path<K, MODE>(A, na, B, nb, d);
@@ -159,15 +195,16 @@ __global__ __launch_bounds__(kMergeBlock) void merge_tile_kernel(
     const void* __restrict__ Av, const void* __restrict__ Bv, const uint32_t* __restrict__ sa,
     const uint32_t* __restrict__ sb, K* __restrict__ out_k, void* __restrict__ out_v) {
   using VT = typename MergeValue<VS>::T;
-  __shared__ K s_keys[kMergeTile];
-  __shared__ uint32_t s_src[VS ? kMergeTile : 1];
+  constexpr uint32_t CAP = LdsPadded<kMergeTile>::value;
+  __shared__ K s_keys[CAP];
+  __shared__ uint32_t s_src[VS ? CAP : 1];
   const uint32_t tid = threadIdx.x, b = blockIdx.x;
   const uint32_t i0 = sa[b], i1 = sa[b + 1], j0 = sb[b], j1 = sb[b + 1];
   const uint32_t la = i1 - i0, lb = j1 - j0, L = la + lb;
-  for (uint32_t k = tid; k < L; k += kMergeBlock) s_keys[k] = k < la ? A[i0 + k] : B[j0 + k - la];
+  load_windows<K, kMergeIpt>(A, i0, la, B, j0, L, s_keys, tid);
   __syncthreads();
-  const K* LA = s_keys;
-  const K* LB = s_keys + la;
+  const PadWindow<K> LA{s_keys, 0};
+  const PadWindow<K> LB{s_keys, la};
   const uint32_t d = min(tid * (uint32_t)kMergeIpt, L);
   uint32_t ia = merge_path<K, MODE>(LA, la, LB, lb, d);
   uint32_t jb = d - ia;
@@ -195,18 +232,33 @@ __global__ __launch_bounds__(kMergeBlock) void merge_tile_kernel(
 #pragma unroll
   for (int q = 0; q < kMergeIpt; ++q) {
     if (d + q < L) {
-      s_keys[d + q] = ok[q];
-      if constexpr (VS > 0) s_src[d + q] = os[q];
+      s_keys[lds_pad(d + q)] = ok[q];
+      if constexpr (VS > 0) s_src[lds_pad(d + q)] = os[q];
     }
   }
   __syncthreads();
   const size_t o = (size_t)i0 + j0;
-  for (uint32_t k = tid; k < L; k += kMergeBlock) {
-    out_k[o + k] = s_keys[k];
-    if constexpr (VS > 0) {
-      const uint32_t s = s_src[k];
-      ((VT*)out_v)[o + k] = s < na ? ((const VT*)Av)[s] : ((const VT*)Bv)[s - na];
+  if constexpr (VS > 0) {
+    // Values are read at their source positions: all loads first, then the stores.
+    VT vv[kMergeIpt];
+#pragma unroll
+    for (int q = 0; q < kMergeIpt; ++q) {
+      const uint32_t k = tid + (uint32_t)q * kMergeBlock;
+      if (k < L) {
+        const uint32_t s = s_src[lds_pad(k)];
+        vv[q] = s < na ? ((const VT*)Av)[s] : ((const VT*)Bv)[s - na];
+      }
     }
+#pragma unroll
+    for (int q = 0; q < kMergeIpt; ++q) {
+      const uint32_t k = tid + (uint32_t)q * kMergeBlock;
+      if (k < L) {
+        out_k[o + k] = s_keys[lds_pad(k)];
+        ((VT*)out_v)[o + k] = vv[q];
+      }
+    }
+  } else {
+    for (uint32_t k = tid; k < L; k += kMergeBlock) out_k[o + k] = s_keys[lds_pad(k)];
   }
 }
 
@@ -260,7 +312,7 @@ __global__ __launch_bounds__(kMergeBlock) void set_tile_kernel(
     uint32_t* __restrict__ out_len, uint64_t* counter, uint64_t* status, uint32_t gen,
     uint32_t ntiles) {
   constexpr int NW = kMergeBlock / kWave;
-  constexpr uint32_t CAP = kSetTile + 2;
+  constexpr uint32_t CAP = LdsPadded<kSetTile + 2>::value;
   __shared__ K s_keys[CAP];
   __shared__ uint32_t s_split[kMergeBlock + 1][2];
   __shared__ uint32_t s_wave[NW];
@@ -270,10 +322,13 @@ __global__ __launch_bounds__(kMergeBlock) void set_tile_kernel(
   const uint32_t b = blockIdx.x;
   const uint32_t i0 = sa[b], i1 = sa[b + 1], j0 = sb[b], j1 = sb[b + 1];
   const uint32_t la = i1 - i0, lb = j1 - j0, L = la + lb;
-  for (uint32_t k = tid; k < L; k += kMergeBlock) s_keys[k] = k < la ? A[i0 + k] : B[j0 + k - la];
+  // A set tile holds up to kSetTile + 2 inputs (pair-aligned ends).
+  load_windows<K, kSetIpt>(A, i0, la, B, j0, min(L, (uint32_t)kSetTile), s_keys, tid);
+  for (uint32_t k = kSetTile + tid; k < L; k += kMergeBlock)
+    s_keys[lds_pad(k)] = k < la ? A[i0 + k] : B[j0 + k - la];
   __syncthreads();
-  const K* LA = s_keys;
-  const K* LB = s_keys + la;
+  const PadWindow<K> LA{s_keys, 0};
+  const PadWindow<K> LB{s_keys, la};
   {
     uint32_t ia, jb;
     set_split<K, MODE>(LA, la, LB, lb, min(tid * (uint32_t)kSetIpt, L), ia, jb);
@@ -351,13 +406,13 @@ __global__ __launch_bounds__(kMergeBlock) void set_tile_kernel(
 #pragma unroll
   for (int q = 0; q < STEPS; ++q) {
     if ((emit >> q) & 1u) {
-      s_keys[my] = ok[q];
+      s_keys[lds_pad(my)] = ok[q];
       ++my;
     }
   }
   __syncthreads();
   const size_t o = s_prefix;
-  for (uint32_t k = tid; k < agg; k += kMergeBlock) out[o + k] = s_keys[k];
+  for (uint32_t k = tid; k < agg; k += kMergeBlock) out[o + k] = s_keys[lds_pad(k)];
 }
 
 __global__ void set_len_zero_kernel(uint32_t* p) { *p = 0; }

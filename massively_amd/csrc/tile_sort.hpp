@@ -303,12 +303,24 @@ __device__ __forceinline__ void tile_sort_range(TileSmem<EW, BLOCK, IPT, DBITS>&
   if constexpr (SLOTS)
     for (uint32_t i = tid; i < HW; i += BLOCK) s.head[i] = 0;
   U o = 0, a = (U)~(U)0;
-  for (uint32_t i = tid; i < n; i += BLOCK) {
-    const Elem<EW> e = load_kv<KW, EW>(keys, vals, (size_t)g0 + i);
-    s.el[i] = e;
-    const U w = order_word(elem_word<KW, EW>(e), om);
-    o |= w;
-    a &= w;
+  {
+    // All IPT loads in flight before the first LDS store (n <= BLOCK * IPT).
+    Elem<EW> ev[IPT];
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      const uint32_t i = tid + (uint32_t)q * BLOCK;
+      if (i < n) ev[q] = load_kv<KW, EW>(keys, vals, (size_t)g0 + i);
+    }
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      const uint32_t i = tid + (uint32_t)q * BLOCK;
+      if (i < n) {
+        s.el[i] = ev[q];
+        const U w = order_word(elem_word<KW, EW>(ev[q]), om);
+        o |= w;
+        a &= w;
+      }
+    }
   }
   uint32_t slot_bits = 0;
   if constexpr (SLOTS) {
@@ -407,6 +419,8 @@ __device__ __forceinline__ void range_sort(RangeSmem<EW, BLOCK, IPT>& s, RangeIo
   for (uint32_t i = tid; i < 8u * kRadixBins; i += BLOCK) (&s.ghist[0][0])[i] = 0;
   for (uint32_t i = tid; i < (uint32_t)(2 * NW * kRadixBins); i += BLOCK) (&s.hist[0][0][0])[i] = 0;
   __syncthreads();
+  // Unrolled so that several loads are in flight before their atomics.
+#pragma unroll 8
   for (uint32_t i = tid; i < n; i += BLOCK) {
     const auto w = order_word(elem_word<KW, EW>(range_load<KW, EW>(in, i)), om);
 #pragma unroll
