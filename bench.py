@@ -318,6 +318,35 @@ def w_bykey(c):
     return out
 
 
+def w_columns(c):
+    """radix_sort_by_key over zipped columns, 2^26 rows: (u32, u32) keys (one
+    packed 64-bit sort) and (u32, u32, u32) keys (two sorts), each with three
+    u32 value columns gathered by the row permutation."""
+    np, m = c.np, c.m
+    n = 1 << 26
+    cols = []
+    for s in range(3):
+        k = c.ex.alloc(np.uint32, n)
+        c.mr.uniform_u32(c.ex, 0, 2**32 - 1, 40 + s, k)
+        cols.append(k)
+    vals = []
+    for s in range(3):
+        v = c.ex.alloc(np.uint32, n)
+        c.mr.iota_u32(c.ex, v)
+        vals.append(v)
+    out = {}
+    for nk in (2, 3):
+        keys = tuple(cols[:nk])
+        ms = timed(c.ex, lambda: m.vector.radix_sort_by_key(c.ex, keys, tuple(vals)), c.steps, c.warmup)
+        # model: one 64-bit (key + permutation) sort per 64-bit key group
+        # (12 B read + 12 B written per pass, 8 passes), the packing reads, and
+        # three value gathers (4 B index + 4 B value read + 4 B written).
+        groups = 1 if nk == 2 else 2
+        b = n * (groups * (8 * 24 + 4.0 * nk) + 3 * 12.0)
+        out[f"keys{nk}_vals3"] = summarize(f"radix_sort_by_key_{nk}xu32_keys_3xu32_values_2^26", ms, b, n)
+    return out
+
+
 def w_compact(c):
     """seg::{Filter, Unique, Take} over the seg workload's segments (2^26 u32
     values in [0, 4), 2^20 log-normal segments), partition of 2^28 u32 at 50%,
@@ -550,6 +579,7 @@ WORKLOADS = {
     "seg": w_seg,
     "bykey": w_bykey,
     "compact": w_compact,
+    "columns": w_columns,
     "served": w_served,
 }
 

@@ -36,7 +36,7 @@ static bool is_aligned16(const void* p) {
 
 template <typename T, int OP>
 static msv_status reduce_impl(Ctx& c, const T* in, uint32_t n, T init, T* d_result) {
-  constexpr uint32_t TILE = (uint32_t)kScanBlock * kReduceRows * (16 / sizeof(T));
+  constexpr uint32_t TILE = (uint32_t)kReduceBlock * kReduceRows * (16 / sizeof(T));
   const T* src = in;
   uint32_t m = n;
   bool aligned = is_aligned16<T>(in);
@@ -45,8 +45,8 @@ static msv_status reduce_impl(Ctx& c, const T* in, uint32_t n, T init, T* d_resu
     const bool last = tiles == 1;
     T* dst = last ? d_result : (T*)c.scratch((size_t)tiles * sizeof(T));
     if (!dst) return MSV_ERR_OUT_OF_MEMORY;
-    hipLaunchKernelGGL((reduce_tile_kernel<T, OP, kScanBlock, kReduceRows>), dim3(tiles),
-                       dim3(kScanBlock), 0, c.stream(), src, m, aligned, dst, last, init);
+    hipLaunchKernelGGL((reduce_tile_kernel<T, OP, kReduceBlock, kReduceRows>), dim3(tiles),
+                       dim3(kReduceBlock), 0, c.stream(), src, m, aligned, dst, last, init);
     MSV_HIP(hipGetLastError());
     if (last) return MSV_OK;
     src = dst;
