@@ -4,8 +4,8 @@
 // Reference: massively/src/api/algorithm/selection.rs:131-260 and
 // core/selection.rs:151-411 (scan of flags, count, selected-index scatter,
 // gather: four launches and ~30 B/elem). Here one kernel reads each value and
-// its flag once: a 256-thread block owns 4096 elements laid out in 4-element
-// groups, row-striped (group (r, t) starts at (r*256 + t)*4), counts the
+// its flag once: a 512-thread block owns 4096 or 8192 elements laid out in
+// 4-element groups, row-striped (group (r, t) starts at (r*512 + t)*4), counts the
 // selected elements per (row, thread), turns the counts into in-tile ranks
 // with DPP wave scans, resolves the tile's output offset by decoupled
 // lookback on counts, stages the selected elements in LDS in input order and
@@ -17,12 +17,13 @@
 
 namespace msv {
 
-constexpr int kSelBlock = 256;
-// Rows of 4-element groups per thread: 4 (4096-element tiles) for short
-// inputs, 8 (8192) from 2^22 elements on, so that fewer tiles take part in
-// the look-back.
-constexpr int kSelRowsSmall = 4;
-constexpr int kSelRowsLarge = 8;
+constexpr int kSelBlock = 512;
+// Rows of 4-element groups per thread (512-thread blocks, which ran the
+// predicate forms 7-9% faster than 256-thread blocks with the same tiles): 2
+// (4096-element tiles) for short inputs, 4 (8192) from 2^22 elements on, so
+// that fewer tiles take part in the look-back.
+constexpr int kSelRowsSmall = 2;
+constexpr int kSelRowsLarge = 4;
 constexpr uint32_t kSelLargeMin = 1u << 22;
 
 // FORM 0: u32 stencil array (flag = stencil != 0).
