@@ -322,6 +322,38 @@ msv_status msv_seg_unique(msv_executor* ex, msv_slice values, const msv_segmenta
 msv_status msv_seg_take(msv_executor* ex, msv_slice values, const msv_segmentation* seg,
                         uint32_t count, msv_vec** out_values, msv_segmentation** out_seg);
 
+/* ---- user-defined operators (compiled at run time) --------------------
+ * The reference's algorithms take user operators as CubeCL traits compiled
+ * into its kernels (ReductionOp, core/reduce.rs:63-66; PredicateOp,
+ * core/predicate.rs:40-43; runtime-compiled read expressions,
+ * core/eval.rs:21-107). Here a user operator is a C++ expression over the
+ * element type, compiled once by msv_user_op_create (hiprtc, a fraction of a
+ * second) into the library's own kernel templates:
+ *   MSV_USER_REDUCTION: `expr` of `a` and `b` (associative; the scans and the
+ *     reduce combine in index order, so it need not be commutative) and
+ *     `identity`, an expression of the identity element;
+ *   MSV_USER_PREDICATE: `expr` of `x`, nonzero = keep (`identity` unused).
+ * The code runs on the GPU with the caller's privileges, as the reference's
+ * user operators do. A compile error returns MSV_ERR_INVALID_ARGUMENT with
+ * the compiler's first diagnostic in msv_last_error. An operator belongs to
+ * the executor it was created on and to one dtype. */
+typedef struct msv_user_op msv_user_op;
+enum { MSV_USER_REDUCTION = 1, MSV_USER_PREDICATE = 2 };
+msv_status msv_user_op_create(msv_executor* ex, int32_t kind, msv_dtype dtype, const char* expr,
+                              const char* identity, msv_user_op** out);
+msv_status msv_user_op_destroy(msv_user_op* op);
+/* vector::reduce / inclusive_scan / exclusive_scan / copy_where with a user
+ * operator (same outputs and synchronisation as the built-in forms; the scans
+ * use the fixed-shape passes, so results are reproducible bit for bit). */
+msv_status msv_reduce_user(msv_executor* ex, msv_slice input, const void* init,
+                           const msv_user_op* op, void* out_host);
+msv_status msv_inclusive_scan_user(msv_executor* ex, msv_slice input, const msv_user_op* op,
+                                   msv_vec** out);
+msv_status msv_exclusive_scan_user(msv_executor* ex, msv_slice input, const void* init,
+                                   const msv_user_op* op, msv_vec** out);
+msv_status msv_copy_where_user(msv_executor* ex, msv_slice input, const msv_user_op* pred,
+                               msv_vec** out);
+
 /* ---- seeded synthetic inputs (massively/src/api/util/random.rs:12-22,331-396) */
 /* uniform_u32(min, max, seed): pcg_hash32(i ^ seed_key(seed) ^ stream) mapped
  * to [min, max]; bit-identical to the reference generator. */

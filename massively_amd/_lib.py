@@ -180,6 +180,12 @@ PROTOTYPES = {
     "msv_scatter": (_I, [_P, _S, _S, _S]),
     "msv_scatter_reduce": (_I, [_P, _S, _S, _P, _I, _S]),
     "msv_partition": (_I, [_P, _S, _P, _PP, C.POINTER(C.c_uint32)]),
+    "msv_user_op_create": (_I, [_P, C.c_int32, _I, C.c_char_p, C.c_char_p, _PP]),
+    "msv_user_op_destroy": (_I, [_P]),
+    "msv_reduce_user": (_I, [_P, _S, _P, _P, _P]),
+    "msv_inclusive_scan_user": (_I, [_P, _S, _P, _PP]),
+    "msv_exclusive_scan_user": (_I, [_P, _S, _P, _P, _PP]),
+    "msv_copy_where_user": (_I, [_P, _S, _P, _PP]),
     "msv_radix_sort_by_key_columns": (_I, [_P, C.POINTER(MsvSlice), C.c_uint32,
                                            C.POINTER(MsvSlice), C.c_uint32, _PP]),
     "msv_radix_sort_by_key": (_I, [_P, _S, _S, _PP, _PP]),

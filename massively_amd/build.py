@@ -39,9 +39,36 @@ COMMON = [
 ]
 
 
+# Device headers that user-defined operators are compiled against at run time
+# (hiprtc), embedded in the library as string literals under the names the
+# sources include them by.
+JIT_HEADERS = [("device_common.hpp", CSRC / "device_common.hpp"),
+               ("scan_kernels.hpp", CSRC / "scan_kernels.hpp"),
+               ("predicate.hpp", CSRC / "predicate.hpp"),
+               ("select_kernels.hpp", CSRC / "select_kernels.hpp"),
+               ("../../include/msv.h", PKG.parent / "include" / "msv.h")]
+
+
+def _write_jit_headers() -> None:
+    names, bodies = [], []
+    for name, path in JIT_HEADERS:
+        text = path.read_text()
+        # hiprtc supplies the HIP runtime itself (size_t included).
+        text = text.replace("#include <hip/hip_runtime.h>", "").replace("#include <stddef.h>", "")
+        assert ")msvjit\"" not in text
+        names.append(f'"{name}"')
+        bodies.append('R"msvjit(' + text + ')msvjit"')
+    out = OBJ / "jit_headers.inc"
+    body = ("static const char* const kJitHeaderNames[] = {" + ", ".join(names) + "};\n"
+            "static const char* const kJitHeaderBodies[] = {\n" + ",\n".join(bodies) + "};\n"
+            f"static const int kJitHeaderCount = {len(names)};\n")
+    if not out.exists() or out.read_text() != body:
+        out.write_text(body)
+
+
 def _compile(src: Path) -> Path:
     obj = OBJ / (src.stem + ".o")
-    cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *COMMON, "-I", str(OBJ), "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stdout}\n{res.stderr}")
@@ -53,11 +80,13 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     sources = sorted(CSRC.glob("*.hip"))
     if not sources:
         raise RuntimeError("no HIP sources found")
+    _write_jit_headers()
     jobs = jobs or min(len(sources), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as pool:
         objs = list(pool.map(_compile, sources))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-lhiprtc", "-o",
+           str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")

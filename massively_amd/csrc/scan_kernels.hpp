@@ -242,4 +242,40 @@ __global__ __launch_bounds__(BLOCK) void reduce_tile_kernel(const T* __restrict_
   }
 }
 
+// Order-preserving tile reduction, for user operators that need not be
+// commutative: row by row, each row in index order (the thread's vector, then
+// lanes, then waves), and then the rows in order. (reduce_tile_kernel folds a
+// thread's rows first, which interleaves rows that lie far apart.)
+template <typename T, int OP, int BLOCK, int ROWS>
+__global__ __launch_bounds__(BLOCK) void reduce_tile_ordered_kernel(const T* __restrict__ in,
+                                                                    uint32_t n, bool aligned,
+                                                                    T* __restrict__ partials,
+                                                                    bool apply_init, T init) {
+  using C = Combine<T, OP>;
+  constexpr int V = 16 / sizeof(T);
+  constexpr int NW = BLOCK / kWave;
+  constexpr uint32_t TILE = (uint32_t)BLOCK * ROWS * V;
+  __shared__ T s_wave[ROWS][NW];
+  const uint32_t tid = threadIdx.x;
+  const size_t base = (size_t)blockIdx.x * TILE;
+  const bool vec_ok = aligned && (base + TILE <= n);
+  T x[ROWS][V];
+  load_tile<T, BLOCK, ROWS>(in, base, n, vec_ok, x, C::identity());
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    T v = x[r][0];
+#pragma unroll
+    for (int k = 1; k < V; ++k) v = C::apply(v, x[r][k]);
+    v = wave_reduce<T, OP>(v);
+    if ((tid & (kWave - 1)) == 0) s_wave[r][tid / kWave] = v;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    T acc = s_wave[0][0];
+    for (int r = 0; r < ROWS; ++r)
+      for (int w = (r == 0 ? 1 : 0); w < NW; ++w) acc = C::apply(acc, s_wave[r][w]);
+    partials[blockIdx.x] = apply_init ? C::apply(init, acc) : acc;
+  }
+}
+
 }  // namespace msv

@@ -116,3 +116,62 @@ def host_scalar(value, dtype) -> C.Array:
     """A one-element host buffer holding `value` as `dtype` (init arguments)."""
     arr = np.array([value], dtype=np.dtype(dtype))
     return arr
+
+
+class _UserOp:
+    """A user-defined operator compiled at run time (msv_user_op_create): one
+    compiled form per (executor, dtype), made on first use."""
+
+    kind = 0
+
+    def __init__(self, expr: str, identity: str | None = None):
+        self.expr = expr
+        self.identity = identity
+        self._compiled = {}
+
+    def handle(self, exec, dtype):
+        import ctypes as C
+
+        from . import _lib as L
+        from .error import check
+
+        key = (id(exec), np.dtype(dtype).str)
+        h = self._compiled.get(key)
+        if h is None:
+            h = C.c_void_p()
+            check(L.lib().msv_user_op_create(
+                exec.handle, self.kind, L.DTYPE_OF_NP[np.dtype(dtype)], self.expr.encode(),
+                None if self.identity is None else self.identity.encode(), C.byref(h)))
+            self._compiled[key] = (exec, h)
+            return h
+        return h[1]
+
+    def __del__(self):
+        try:
+            from . import _lib as L
+
+            for _, h in self._compiled.values():
+                L.lib().msv_user_op_destroy(h)
+            self._compiled.clear()
+        except Exception:
+            pass
+
+
+class UserReductionOp(_UserOp, ReductionOp):
+    """A reduction given as a C++ expression of `a` and `b` over the element
+    type, with an identity expression, e.g. UserReductionOp("a > b ? a : b",
+    "0"). It must be associative; the scans and the reduce combine in index
+    order, so it need not be commutative."""
+
+    kind = 1
+    name = "UserReductionOp"
+
+
+class UserPredicate(_UserOp):
+    """A predicate given as a C++ expression of `x` (nonzero keeps it),
+    applied by ``vector.copy_where_if(exec, x, x, pred)``."""
+
+    kind = 2
+
+    def __init__(self, expr: str):
+        _UserOp.__init__(self, expr, None)

@@ -12,7 +12,7 @@ import numpy as np
 
 from . import _lib as L
 from .error import LengthMismatch, check
-from .op import BinaryPredicateOp, PredicateOp, ReductionOp
+from .op import BinaryPredicateOp, PredicateOp, ReductionOp, UserPredicate, UserReductionOp
 from .runtime import DeviceSlice, DeviceVec, Executor, as_slice
 
 _lib = L.lib
@@ -30,6 +30,9 @@ def inclusive_scan(exec: Executor, input, op: ReductionOp) -> DeviceVec:
     """massively/src/api/algorithm/scan.rs:96-113."""
     s = as_slice(input)
     h = C.c_void_p()
+    if isinstance(op, UserReductionOp):
+        check(_lib().msv_inclusive_scan_user(exec.handle, s._c(), op.handle(exec, s.dtype), C.byref(h)))
+        return _out(exec, h)
     check(_lib().msv_inclusive_scan(exec.handle, s._c(), op.code, C.byref(h)))
     return _out(exec, h)
 
@@ -39,6 +42,10 @@ def exclusive_scan(exec: Executor, input, init, op: ReductionOp) -> DeviceVec:
     s = as_slice(input)
     init_buf = _scalar(init, s.dtype)
     h = C.c_void_p()
+    if isinstance(op, UserReductionOp):
+        check(_lib().msv_exclusive_scan_user(exec.handle, s._c(), init_buf.ctypes.data_as(C.c_void_p),
+                                             op.handle(exec, s.dtype), C.byref(h)))
+        return _out(exec, h)
     check(
         _lib().msv_exclusive_scan(
             exec.handle, s._c(), init_buf.ctypes.data_as(C.c_void_p), op.code, C.byref(h)
@@ -52,6 +59,10 @@ def reduce(exec: Executor, input, init, op: ReductionOp):
     s = as_slice(input)
     init_buf = _scalar(init, s.dtype)
     out = np.zeros(1, dtype=s.dtype)
+    if isinstance(op, UserReductionOp):
+        check(_lib().msv_reduce_user(exec.handle, s._c(), init_buf.ctypes.data_as(C.c_void_p),
+                                     op.handle(exec, s.dtype), out.ctypes.data_as(C.c_void_p)))
+        return out[0]
     check(
         _lib().msv_reduce(
             exec.handle, s._c(), init_buf.ctypes.data_as(C.c_void_p), op.code,
@@ -78,8 +89,17 @@ def remove_where(exec: Executor, input, stencil) -> DeviceVec:
 
 
 def copy_where_if(exec: Executor, input, stencil, pred: PredicateOp, invert: bool = False):
-    """copy_where with ``lazy::map(stencil, pred)`` as the stencil, fused."""
+    """copy_where with ``lazy::map(stencil, pred)`` as the stencil, fused. A
+    :class:`UserPredicate` is evaluated on the input itself (``stencil`` must be
+    the input)."""
     st = as_slice(stencil)
+    if isinstance(pred, UserPredicate):
+        src = as_slice(input)
+        if invert or src.vec is not st.vec or src.offset != st.offset:
+            raise ValueError("a UserPredicate is applied to the input itself, without invert")
+        h = C.c_void_p()
+        check(_lib().msv_copy_where_user(exec.handle, src._c(), pred.handle(exec, src.dtype), C.byref(h)))
+        return _out(exec, h)
     p = pred.c_struct(st.dtype)
     h = C.c_void_p()
     check(_lib().msv_copy_where_pred(exec.handle, as_slice(input)._c(), st._c(), C.byref(p),
