@@ -354,6 +354,60 @@ msv_status msv_exclusive_scan_user(msv_executor* ex, msv_slice input, const void
 msv_status msv_copy_where_user(msv_executor* ex, msv_slice input, const msv_user_op* pred,
                                msv_vec** out);
 
+/* ---- lazy inputs, fused into the consuming kernel (compiled at run time) ---
+ * The reference's lazy inputs (api/lazy.rs:664-920: constant, counting, stride,
+ * map, permute, reverse, repeat_each, tile, adjacent_difference, identity) are
+ * read expressions over up to 13 device slots, compiled into the kernel that
+ * consumes them (core/eval.rs:21-107, core/read.rs). Here a lazy input is
+ * `read_src`, C++ source that defines, for the element type UT,
+ *     __device__ UT msv_read(const LazyArgs& la, size_t i)
+ * giving element i. It reads slot k with MSV_SLOT(k, type, index) (0 for an
+ * index past the slot's length, so no read leaves a slot) and run-time
+ * parameters la.q[0..15]; helper functions may precede it. msv_lazy_op_create
+ * compiles it once together with the consuming operator: kind
+ * MSV_USER_REDUCTION (a user expression and identity as for msv_user_op_create,
+ * or expr NULL and builtin_op 0 add, 1 min, 2 max, 3 xor), MSV_USER_PREDICATE
+ * (a user expression of `x`, or expr NULL for the built-in predicates, whose
+ * msv_predicate is given per call) or MSV_LAZY_MAP (the lazy input written
+ * out: vector::map, api/algorithm/transform.rs:91-108). The source does not
+ * depend on the slots' addresses or the parameters' values, so one compiled
+ * operator serves every call with the same expression shape. A reduction
+ * compiled with a read source also accepts plain slices (msv_*_user). */
+#define MSV_LAZY_SLOTS 13
+#define MSV_LAZY_PARAMS 16
+enum { MSV_LAZY_MAP = 3 };
+typedef struct msv_lazy_args {
+  msv_slice slots[MSV_LAZY_SLOTS]; /* owned slices of the executor; MSV_SLOT(k,...) reads slots[k] */
+  uint32_t nslots;
+  uint32_t len;                    /* logical length of the lazy input */
+  uint64_t params[MSV_LAZY_PARAMS];
+} msv_lazy_args;
+msv_status msv_lazy_op_create(msv_executor* ex, int32_t kind, msv_dtype dtype,
+                              const char* read_src, int32_t builtin_op, const char* expr,
+                              const char* identity, msv_user_op** out);
+/* Generates and compiles an operator as msv_lazy_op_create does, without a
+ * device or an executor: MSV_OK, or the error msv_lazy_op_create would give. */
+msv_status msv_lazy_op_check(int32_t kind, msv_dtype dtype, const char* read_src,
+                             int32_t builtin_op, const char* expr, const char* identity);
+/* vector::reduce / inclusive_scan / exclusive_scan (api/algorithm/reduce.rs:32,
+ * scan.rs:96,216) of a lazy input; same outputs and synchronisation as the
+ * msv_*_user forms. */
+msv_status msv_reduce_lazy(msv_executor* ex, const msv_lazy_args* input, const void* init,
+                           const msv_user_op* op, void* out_host);
+msv_status msv_inclusive_scan_lazy(msv_executor* ex, const msv_lazy_args* input,
+                                   const msv_user_op* op, msv_vec** out);
+msv_status msv_exclusive_scan_lazy(msv_executor* ex, const msv_lazy_args* input, const void* init,
+                                   const msv_user_op* op, msv_vec** out);
+/* vector::copy_where with the predicate applied to the lazy input itself
+ * (selection.rs:146-165, device-resident length); `builtin` is used when the
+ * operator was created with expr NULL (NULL there means nonzero). */
+msv_status msv_copy_where_lazy(msv_executor* ex, const msv_lazy_args* input,
+                               const msv_user_op* pred, const msv_predicate* builtin,
+                               msv_vec** out);
+/* vector::map (api/algorithm/transform.rs:91-108): the lazy input, written out. */
+msv_status msv_map_lazy(msv_executor* ex, const msv_lazy_args* input, const msv_user_op* op,
+                        msv_vec** out);
+
 /* ---- seeded synthetic inputs (massively/src/api/util/random.rs:12-22,331-396) */
 /* uniform_u32(min, max, seed): pcg_hash32(i ^ seed_key(seed) ^ stream) mapped
  * to [min, max]; bit-identical to the reference generator. */

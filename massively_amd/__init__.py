@@ -6,7 +6,7 @@ Host-side mirror of the reference's public API (``Executor``, ``DeviceVec``,
 kernels; importing this package does not load the library, but every
 operation raises if it is missing.
 """
-from . import op, seg, vector
+from . import lazy, op, seg, vector
 from .error import (
     Error,
     ForeignExecutor,
@@ -33,6 +33,7 @@ __all__ = [
     "Unsupported",
     "segsort_configs",
     "segsort_tuned_config",
+    "lazy",
     "op",
     "seg",
     "vector",

@@ -56,6 +56,19 @@ class MsvSlice(C.Structure):
     _fields_ = [("vec", C.c_void_p), ("offset", C.c_uint32), ("len", C.c_uint32)]
 
 
+LAZY_SLOTS = 13
+LAZY_PARAMS = 16
+
+
+class MsvLazyArgs(C.Structure):
+    _fields_ = [
+        ("slots", MsvSlice * LAZY_SLOTS),
+        ("nslots", C.c_uint32),
+        ("len", C.c_uint32),
+        ("params", C.c_uint64 * LAZY_PARAMS),
+    ]
+
+
 class MsvPredicate(C.Structure):
     _fields_ = [
         ("kind", C.c_uint32),
@@ -186,6 +199,13 @@ PROTOTYPES = {
     "msv_inclusive_scan_user": (_I, [_P, _S, _P, _PP]),
     "msv_exclusive_scan_user": (_I, [_P, _S, _P, _P, _PP]),
     "msv_copy_where_user": (_I, [_P, _S, _P, _PP]),
+    "msv_lazy_op_create": (_I, [_P, C.c_int32, _I, C.c_char_p, C.c_int32, C.c_char_p, C.c_char_p, _PP]),
+    "msv_lazy_op_check": (_I, [C.c_int32, _I, C.c_char_p, C.c_int32, C.c_char_p, C.c_char_p]),
+    "msv_reduce_lazy": (_I, [_P, C.POINTER(MsvLazyArgs), _P, _P, _P]),
+    "msv_inclusive_scan_lazy": (_I, [_P, C.POINTER(MsvLazyArgs), _P, _PP]),
+    "msv_exclusive_scan_lazy": (_I, [_P, C.POINTER(MsvLazyArgs), _P, _P, _PP]),
+    "msv_copy_where_lazy": (_I, [_P, C.POINTER(MsvLazyArgs), _P, C.POINTER(MsvPredicate), _PP]),
+    "msv_map_lazy": (_I, [_P, C.POINTER(MsvLazyArgs), _P, _PP]),
     "msv_radix_sort_by_key_columns": (_I, [_P, C.POINTER(MsvSlice), C.c_uint32,
                                            C.POINTER(MsvSlice), C.c_uint32, _PP]),
     "msv_radix_sort_by_key": (_I, [_P, _S, _S, _PP, _PP]),

@@ -225,6 +225,28 @@ __device__ inline void load4(const T* __restrict__ p, size_t i, uint32_t n, bool
   }
 }
 
+// Kernels read their input through a "source": a pointer (the element at i is
+// p[i]) or a lazy source, an object whose operator()(i) computes element i
+// (inputs composed of counting / constant / mapped / permuted reads, compiled
+// at run time). src_at and the non-pointer load4 serve the lazy form.
+// Lazy sources derive from this empty tag (no effect on their layout); a
+// pointer, restrict-qualified or not, is not one.
+struct LazySource {};
+template <typename Src>
+constexpr bool is_lazy_src = __is_base_of(LazySource, Src);
+
+template <typename Src>
+__device__ inline auto src_at(const Src& s, size_t i) {
+  if constexpr (is_lazy_src<Src>) return s(i);
+  else return s[i];
+}
+
+template <typename T, typename Src, typename = std::enable_if_t<is_lazy_src<Src>>>
+__device__ inline void load4(const Src& s, size_t i, uint32_t n, bool, T (&v)[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = (i + k < n) ? (T)s(i + k) : (T)0;
+}
+
 // The by-key family's built-in Equal: numeric == on the key dtype (bitwise
 // for integers; for floats -0.0 equals +0.0 and a NaN equals nothing).
 template <typename K>

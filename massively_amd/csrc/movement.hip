@@ -122,7 +122,7 @@ __global__ void iota_u32_kernel(uint32_t* __restrict__ out, uint32_t n, uint32_t
 msv_status gather_run(Ctx& c, msv_dtype dt, const void* values, uint32_t vlen,
                       const uint32_t* idx, void* out, uint32_t n) {
   if (n == 0) return MSV_OK;
-  const uint32_t blocks = (n + kMoveTile - 1) / kMoveTile;
+  const uint32_t blocks = div_up(n, kMoveTile);
   const bool aligned = (((uintptr_t)idx | (uintptr_t)out) & 15u) == 0;
   if (dtype_size(dt) == 4)
     hipLaunchKernelGGL(gather_kernel<uint32_t>, dim3(blocks), dim3(kMoveBlock), 0, c.stream(),
@@ -137,7 +137,7 @@ msv_status gather_run(Ctx& c, msv_dtype dt, const void* values, uint32_t vlen,
 msv_status scatter_run(Ctx& c, msv_dtype dt, const void* values, const uint32_t* idx,
                        uint32_t n, void* out, uint32_t olen) {
   if (n == 0) return MSV_OK;
-  const uint32_t blocks = (n + kMoveTile - 1) / kMoveTile;
+  const uint32_t blocks = div_up(n, kMoveTile);
   if (dtype_size(dt) == 4)
     hipLaunchKernelGGL(scatter_kernel<uint32_t>, dim3(blocks), dim3(kMoveBlock), 0, c.stream(),
                        (const uint32_t*)values, idx, n, (uint32_t*)out, olen);

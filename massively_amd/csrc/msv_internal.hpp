@@ -52,6 +52,9 @@ inline size_t dtype_size(msv_dtype d) {
   return 0;
 }
 inline bool dtype_valid(int d) { return d >= MSV_U32 && d <= MSV_F64; }
+// ceil(n / d) for lengths up to 2^32 - 1: the sum is formed in 64 bits, so
+// n + d - 1 cannot wrap.
+inline uint32_t div_up(uint64_t n, uint64_t d) { return (uint32_t)((n + d - 1) / d); }
 inline bool dtype_is_float(msv_dtype d) { return d == MSV_F32 || d == MSV_F64; }
 
 // ---- caching allocator (one per executor) ---------------------------------

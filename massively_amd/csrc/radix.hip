@@ -52,7 +52,7 @@ static msv_status radix_sort_impl(Ctx& c, const void* keys, const void* vals, ui
   constexpr int IPT = radix_ipt<EW>();
   constexpr uint32_t TILE = (uint32_t)BLOCK * IPT;
   if (n == 0) return MSV_OK;
-  const uint32_t ntiles = (n + TILE - 1) / TILE;
+  const uint32_t ntiles = div_up(n, TILE);
   const size_t elem_bytes = (size_t)n * EW * 4;
   void* aos0 = c.scratch(elem_bytes);
   void* aos1 = c.scratch(elem_bytes);

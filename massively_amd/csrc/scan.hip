@@ -41,7 +41,7 @@ static msv_status reduce_impl(Ctx& c, const T* in, uint32_t n, T init, T* d_resu
   uint32_t m = n;
   bool aligned = is_aligned16<T>(in);
   for (;;) {
-    const uint32_t tiles = m == 0 ? 1 : (m + TILE - 1) / TILE;
+    const uint32_t tiles = m == 0 ? 1 : div_up(m, TILE);
     const bool last = tiles == 1;
     T* dst = last ? d_result : (T*)c.scratch((size_t)tiles * sizeof(T));
     if (!dst) return MSV_ERR_OUT_OF_MEMORY;
@@ -60,7 +60,7 @@ static msv_status float_scan_impl(Ctx& c, const T* in, T* out, uint32_t n, T ini
   constexpr int kScanRows = kScanRowsSmall;
   constexpr uint32_t TILE = (uint32_t)kScanBlock * kScanRows * (16 / sizeof(T));
   const bool aligned = is_aligned16<T>(in) && is_aligned16<T>(out);
-  const uint32_t ntiles = (n + TILE - 1) / TILE;
+  const uint32_t ntiles = div_up(n, TILE);
   const T* prefixes = nullptr;
   if (ntiles > 1) {
     T* partials = (T*)c.scratch((size_t)ntiles * sizeof(T));
@@ -83,7 +83,7 @@ static msv_status float_scan_impl(Ctx& c, const T* in, T* out, uint32_t n, T ini
 template <typename T, int OP, bool EXCL, int ROWS>
 static msv_status int_scan_rows(Ctx& c, const T* in, T* out, uint32_t n, T init) {
   constexpr uint32_t TILE = (uint32_t)kScanBlock * ROWS * (16 / sizeof(T));
-  const uint32_t ntiles = (n + TILE - 1) / TILE;
+  const uint32_t ntiles = div_up(n, TILE);
   uint64_t *counter, *status;
   uint32_t gen;
   MSV_TRY(c.lookback((size_t)ntiles * TileStatus<T>::kWords, &counter, &status, &gen));

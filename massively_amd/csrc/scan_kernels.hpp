@@ -63,6 +63,23 @@ __device__ inline void load_tile(const T* __restrict__ in, size_t base, uint32_t
   }
 }
 
+// A lazy source: element i is in(i) (the same thread-to-element layout).
+template <typename T, int BLOCK, int ROWS, bool NT = false, typename Src,
+          typename = std::enable_if_t<is_lazy_src<Src>>>
+__device__ inline void load_tile(const Src& in, size_t base, uint32_t n, bool,
+                                 T (&x)[ROWS][16 / sizeof(T)], T fill) {
+  constexpr int V = 16 / sizeof(T);
+  const uint32_t tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const size_t i = base + ((size_t)r * BLOCK + tid) * V + k;
+      x[r][k] = (i < n) ? (T)in(i) : fill;
+    }
+  }
+}
+
 template <typename T, int BLOCK, int ROWS, bool NT = false>
 __device__ inline void store_tile(T* __restrict__ out, size_t base, uint32_t n, bool vec_ok,
                                   const T (&y)[ROWS][16 / sizeof(T)]) {
@@ -90,13 +107,13 @@ __device__ inline void store_tile(T* __restrict__ out, size_t base, uint32_t n, 
 
 // Aggregate of tile q computed by one wave straight from the input (the
 // lookback's self-help path).
-template <typename T, int OP, uint32_t TILE>
-__device__ inline T wave_tile_fold(const T* __restrict__ in, uint32_t n, uint32_t q) {
+template <typename T, int OP, uint32_t TILE, typename Src>
+__device__ inline T wave_tile_fold(const Src& in, uint32_t n, uint32_t q) {
   using C = Combine<T, OP>;
   const size_t b = (size_t)q * TILE;
   const size_t e = min((size_t)n, b + TILE);
   T acc = C::identity();
-  for (size_t i = b + __lane_id(); i < e; i += kWave) acc = C::apply(acc, in[i]);
+  for (size_t i = b + __lane_id(); i < e; i += kWave) acc = C::apply(acc, (T)src_at(in, i));
   return wave_reduce<T, OP>(acc);
 }
 
@@ -104,9 +121,11 @@ enum ScanMode : int { SCAN_LOOKBACK = 0, SCAN_PREFIXED = 1 };
 
 // One tile per block, tile = blockIdx.x. SCAN_LOOKBACK resolves the tile
 // prefix by decoupled lookback (with self-help); SCAN_PREFIXED uses
-// `prefixes[tile]` (or the identity when prefixes is null).
-template <typename T, int OP, bool EXCL, int MODE, int BLOCK, int ROWS>
-__global__ __launch_bounds__(BLOCK) void scan_tile_kernel(const T* __restrict__ in,
+// `prefixes[tile]` (or the identity when prefixes is null). Src: the input
+// source (a pointer, or a lazy source compiled at run time).
+template <typename T, int OP, bool EXCL, int MODE, int BLOCK, int ROWS,
+          typename Src = const T* __restrict__>
+__global__ __launch_bounds__(BLOCK) void scan_tile_kernel(Src in,
                                                           T* __restrict__ out, uint32_t n, T init,
                                                           bool aligned, uint64_t* status,
                                                           uint32_t gen,
@@ -155,7 +174,7 @@ __global__ __launch_bounds__(BLOCK) void scan_tile_kernel(const T* __restrict__ 
       } else {
         if (lane == 0) st.publish(tile, kFlagAgg, 0, agg);
         excl = wave_lookback_sh<T, OP>(
-            st, tile, [&](uint32_t q) { return wave_tile_fold<T, OP, TILE>(in, n, q); });
+            st, tile, [&](uint32_t q) { return wave_tile_fold<T, OP, TILE, Src>(in, n, q); });
         if (lane == 0) st.publish(tile, kFlagInc, 0, C::apply(excl, agg));
       }
       if (lane == 0) s_prefix = excl;
@@ -246,8 +265,8 @@ __global__ __launch_bounds__(BLOCK) void reduce_tile_kernel(const T* __restrict_
 // commutative: row by row, each row in index order (the thread's vector, then
 // lanes, then waves), and then the rows in order. (reduce_tile_kernel folds a
 // thread's rows first, which interleaves rows that lie far apart.)
-template <typename T, int OP, int BLOCK, int ROWS>
-__global__ __launch_bounds__(BLOCK) void reduce_tile_ordered_kernel(const T* __restrict__ in,
+template <typename T, int OP, int BLOCK, int ROWS, typename Src = const T* __restrict__>
+__global__ __launch_bounds__(BLOCK) void reduce_tile_ordered_kernel(Src in,
                                                                     uint32_t n, bool aligned,
                                                                     T* __restrict__ partials,
                                                                     bool apply_init, T init) {

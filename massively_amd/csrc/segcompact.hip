@@ -304,7 +304,7 @@ static msv_status seg_compact_launch(Ctx& c, const void* in, uint32_t n, const u
     return MSV_OK;
   }
   const Pred<T> pred = make_pred<T>(p);
-  const uint32_t ntiles = (n + kCmpTile - 1) / kCmpTile;
+  const uint32_t ntiles = div_up(n, kCmpTile);
   uint64_t *counter, *status;
   uint32_t gen;
   MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));

@@ -544,7 +544,7 @@ static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
     }
     return MSV_OK;
   }
-  const uint32_t ntiles = (n + TILE - 1) / TILE;
+  const uint32_t ntiles = div_up(n, TILE);
   // Per-element outputs are stored as vectors; the others only need the
   // inputs aligned.
   const bool elem_out = outk == SEG_INCL || outk == SEG_EXCL;

@@ -49,7 +49,7 @@ static msv_status select_launch(Ctx& c, const void* in, const void* st, const ms
   const Pred<S> pred = make_pred<S>(p);
   const bool large = n >= kSelLargeMin;
   const uint32_t tile = (uint32_t)kSelBlock * (large ? kSelRowsLarge : kSelRowsSmall) * 4;
-  const uint32_t ntiles = (n + tile - 1) / tile;
+  const uint32_t ntiles = div_up(n, tile);
   uint64_t *counter, *status;
   uint32_t gen;
   MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));
@@ -113,7 +113,7 @@ static msv_status partition_launch(Ctx& c, const void* in, uint32_t n, const msv
   MSV_HIP(hipGetLastError());
   const bool large = n >= kSelLargeMin;
   const uint32_t tile = (uint32_t)kSelBlock * (large ? kSelRowsLarge : kSelRowsSmall) * 4;
-  const uint32_t ntiles = (n + tile - 1) / tile;
+  const uint32_t ntiles = div_up(n, tile);
   uint64_t *counter, *status;
   uint32_t gen;
   MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));

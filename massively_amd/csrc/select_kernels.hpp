@@ -26,9 +26,11 @@ constexpr uint32_t kSelLargeMin = 1u << 22;
 // ones: `part_total` holds the number of passing elements, and a failing
 // element goes to part_total + (failing elements before it).
 // PredT: the predicate's type (the built-in Pred<S>, or a user-defined
-// functor compiled at run time).
-template <typename T, typename S, int FORM, int ROWS, bool PART = false, typename PredT = Pred<S>>
-__global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__ in,
+// functor compiled at run time). Src: the input source (a pointer, or a lazy
+// source compiled at run time).
+template <typename T, typename S, int FORM, int ROWS, bool PART = false, typename PredT = Pred<S>,
+          typename Src = const T* __restrict__>
+__global__ __launch_bounds__(kSelBlock) void select_kernel(Src in,
                                                            const S* __restrict__ stencil,
                                                            PredT pred, bool invert,
                                                            uint32_t n, bool aligned,
@@ -119,7 +121,7 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(const T* __restrict__
           bool f;
           if constexpr (FORM == 0) f = stencil[i] != 0;
           else if constexpr (FORM == 1) f = pred(stencil[i]);
-          else if constexpr (FORM == 2) f = pred(bit_cast<S>(in[i]));
+          else if constexpr (FORM == 2) f = pred(bit_cast<S>((T)src_at(in, i)));
           else f = i == 0 || !key_equal(stencil[i - 1], stencil[i]);
           c += (f != invert) ? 1u : 0u;
         }

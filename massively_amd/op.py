@@ -175,3 +175,21 @@ class UserPredicate(_UserOp):
 
     def __init__(self, expr: str):
         _UserOp.__init__(self, expr, None)
+
+
+class UnaryOp:
+    """``op::UnaryOp`` (massively/src/core/op.rs:33-60): a C++ expression of
+    `x` (the input element; `i` is its index) giving the output element, of
+    dtype `output` (the input's dtype when None). Used by ``lazy.map`` and
+    ``vector.map``; compiled into the consuming kernel at run time."""
+
+    def __init__(self, expr: str, output=None):
+        self.expr = expr
+        self.output = None if output is None else np.dtype(output)
+
+    def __repr__(self):
+        return f"UnaryOp({self.expr!r})"
+
+
+# op::Identity (massively/src/core/op.rs:86-96).
+Identity = UnaryOp("x")

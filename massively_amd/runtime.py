@@ -26,6 +26,8 @@ class Executor:
         check(self._lib.msv_executor_create(int(device), C.byref(h)))
         self._h = h
         self.device = int(device)
+        # Operators compiled at run time for lazy inputs, by source (lazy.py).
+        self._jit_ops = {}
 
     @property
     def handle(self) -> C.c_void_p:
@@ -42,6 +44,9 @@ class Executor:
         return int(self._lib.msv_executor_stream(self.handle) or 0)
 
     def close(self) -> None:
+        for h in getattr(self, "_jit_ops", {}).values():
+            self._lib.msv_user_op_destroy(h)
+        self._jit_ops = {}
         if self._h is not None:
             self._lib.msv_executor_destroy(self._h)
             self._h = None
@@ -203,7 +208,11 @@ class DeviceSlice:
         return self.vec.executor
 
     def _c(self) -> L.MsvSlice:
-        return L.MsvSlice(self.vec.handle, self.offset, self.length)
+        c = L.MsvSlice(self.vec.handle, self.offset, self.length)
+        # The struct keeps the vector alive while it is a call's argument (a
+        # temporary vector, e.g. a written-out lazy input, has no other owner).
+        c._keep = self.vec
+        return c
 
     def slice(self, start: int = 0, end: int | None = None) -> "DeviceSlice":
         base_len = self.vec.len() - self.offset if self.length == L.WHOLE else self.length

@@ -12,10 +12,18 @@ import numpy as np
 
 from . import _lib as L
 from .error import LengthMismatch, check
-from .op import BinaryPredicateOp, PredicateOp, ReductionOp, UserPredicate, UserReductionOp
+from .op import (BinaryPredicateOp, PredicateOp, ReductionOp, UnaryOp, UserPredicate,
+                 UserReductionOp)
+from . import lazy
 from .runtime import DeviceSlice, DeviceVec, Executor, as_slice
 
 _lib = L.lib
+
+
+def _in(exec: Executor, x) -> DeviceSlice:
+    """A device slice of an input; a lazy input is written out first (the
+    algorithms that fuse lazy inputs dispatch before this)."""
+    return lazy.as_input(exec, x)
 
 
 def _out(exec: Executor, h: C.c_void_p) -> DeviceVec:
@@ -27,8 +35,16 @@ def _scalar(value, dtype) -> np.ndarray:
 
 
 def inclusive_scan(exec: Executor, input, op: ReductionOp) -> DeviceVec:
-    """massively/src/api/algorithm/scan.rs:96-113."""
-    s = as_slice(input)
+    """massively/src/api/algorithm/scan.rs:96-113. A lazy input is computed
+    inside the scan kernels."""
+    if isinstance(input, lazy.Lazy):
+        src, args = input.lower()
+        h = C.c_void_p()
+        check(_lib().msv_inclusive_scan_lazy(exec.handle, C.byref(args),
+                                             lazy.compiled(exec, "reduction", input.dtype, src, op),
+                                             C.byref(h)))
+        return _out(exec, h)
+    s = _in(exec, input)
     h = C.c_void_p()
     if isinstance(op, UserReductionOp):
         check(_lib().msv_inclusive_scan_user(exec.handle, s._c(), op.handle(exec, s.dtype), C.byref(h)))
@@ -39,7 +55,16 @@ def inclusive_scan(exec: Executor, input, op: ReductionOp) -> DeviceVec:
 
 def exclusive_scan(exec: Executor, input, init, op: ReductionOp) -> DeviceVec:
     """massively/src/api/algorithm/scan.rs:216-257: out[0] = init."""
-    s = as_slice(input)
+    if isinstance(input, lazy.Lazy):
+        src, args = input.lower()
+        init_buf = _scalar(init, input.dtype)
+        h = C.c_void_p()
+        check(_lib().msv_exclusive_scan_lazy(exec.handle, C.byref(args),
+                                             init_buf.ctypes.data_as(C.c_void_p),
+                                             lazy.compiled(exec, "reduction", input.dtype, src, op),
+                                             C.byref(h)))
+        return _out(exec, h)
+    s = _in(exec, input)
     init_buf = _scalar(init, s.dtype)
     h = C.c_void_p()
     if isinstance(op, UserReductionOp):
@@ -55,8 +80,17 @@ def exclusive_scan(exec: Executor, input, init, op: ReductionOp) -> DeviceVec:
 
 
 def reduce(exec: Executor, input, init, op: ReductionOp):
-    """massively/src/api/algorithm/reduce.rs:32-50: op(init, fold(input))."""
-    s = as_slice(input)
+    """massively/src/api/algorithm/reduce.rs:32-50: op(init, fold(input)). A
+    lazy input is computed inside the reduction kernel."""
+    if isinstance(input, lazy.Lazy):
+        src, args = input.lower()
+        init_buf = _scalar(init, input.dtype)
+        out = np.zeros(1, dtype=input.dtype)
+        check(_lib().msv_reduce_lazy(exec.handle, C.byref(args), init_buf.ctypes.data_as(C.c_void_p),
+                                     lazy.compiled(exec, "reduction", input.dtype, src, op),
+                                     out.ctypes.data_as(C.c_void_p)))
+        return out[0]
+    s = _in(exec, input)
     init_buf = _scalar(init, s.dtype)
     out = np.zeros(1, dtype=s.dtype)
     if isinstance(op, UserReductionOp):
@@ -75,7 +109,7 @@ def reduce(exec: Executor, input, init, op: ReductionOp):
 def copy_where(exec: Executor, input, stencil) -> DeviceVec:
     """massively/src/api/algorithm/selection.rs:146-165 (device-resident length)."""
     h = C.c_void_p()
-    check(_lib().msv_copy_where(exec.handle, as_slice(input)._c(), as_slice(stencil)._c(),
+    check(_lib().msv_copy_where(exec.handle, _in(exec, input)._c(), _in(exec, stencil)._c(),
                                 C.byref(h)))
     return _out(exec, h)
 
@@ -83,7 +117,7 @@ def copy_where(exec: Executor, input, stencil) -> DeviceVec:
 def remove_where(exec: Executor, input, stencil) -> DeviceVec:
     """massively/src/api/algorithm/selection.rs:203-222."""
     h = C.c_void_p()
-    check(_lib().msv_remove_where(exec.handle, as_slice(input)._c(), as_slice(stencil)._c(),
+    check(_lib().msv_remove_where(exec.handle, _in(exec, input)._c(), _in(exec, stencil)._c(),
                                   C.byref(h)))
     return _out(exec, h)
 
@@ -91,10 +125,24 @@ def remove_where(exec: Executor, input, stencil) -> DeviceVec:
 def copy_where_if(exec: Executor, input, stencil, pred: PredicateOp, invert: bool = False):
     """copy_where with ``lazy::map(stencil, pred)`` as the stencil, fused. A
     :class:`UserPredicate` is evaluated on the input itself (``stencil`` must be
-    the input)."""
-    st = as_slice(stencil)
+    the input). A lazy input with ``stencil`` the same object is computed
+    inside the compaction kernel, and the predicate (built-in or user) is
+    applied to it."""
+    if isinstance(input, lazy.Lazy) and stencil is input:
+        if invert:
+            raise ValueError("remove_where_if of a lazy input: use the negated predicate")
+        src, args = input.lower()
+        user = isinstance(pred, UserPredicate)
+        h = C.c_void_p()
+        p = None if user else pred.c_struct(input.dtype)
+        check(_lib().msv_copy_where_lazy(exec.handle, C.byref(args),
+                                         lazy.compiled(exec, "predicate", input.dtype, src,
+                                                       pred if user else None),
+                                         None if user else C.byref(p), C.byref(h)))
+        return _out(exec, h)
+    st = _in(exec, stencil)
     if isinstance(pred, UserPredicate):
-        src = as_slice(input)
+        src = _in(exec, input)
         if invert or src.vec is not st.vec or src.offset != st.offset:
             raise ValueError("a UserPredicate is applied to the input itself, without invert")
         h = C.c_void_p()
@@ -102,7 +150,7 @@ def copy_where_if(exec: Executor, input, stencil, pred: PredicateOp, invert: boo
         return _out(exec, h)
     p = pred.c_struct(st.dtype)
     h = C.c_void_p()
-    check(_lib().msv_copy_where_pred(exec.handle, as_slice(input)._c(), st._c(), C.byref(p),
+    check(_lib().msv_copy_where_pred(exec.handle, _in(exec, input)._c(), st._c(), C.byref(p),
                                      1 if invert else 0, C.byref(h)))
     return _out(exec, h)
 
@@ -114,7 +162,7 @@ def remove_where_if(exec: Executor, input, stencil, pred: PredicateOp):
 def partition(exec: Executor, input, pred: PredicateOp):
     """massively/src/api/algorithm/selection.rs:245-288: stable partition,
     passing items first. Returns (output, boundary)."""
-    src = as_slice(input)
+    src = _in(exec, input)
     p = pred.c_struct(src.dtype)
     h = C.c_void_p()
     boundary = C.c_uint32()
@@ -125,23 +173,23 @@ def partition(exec: Executor, input, pred: PredicateOp):
 def gather(exec: Executor, values, indices) -> DeviceVec:
     """massively/src/api/algorithm/indexed.rs:157-208: out[i] = values[indices[i]]."""
     h = C.c_void_p()
-    check(_lib().msv_gather(exec.handle, as_slice(values)._c(), as_slice(indices)._c(),
+    check(_lib().msv_gather(exec.handle, _in(exec, values)._c(), _in(exec, indices)._c(),
                             C.byref(h)))
     return _out(exec, h)
 
 
 def scatter(exec: Executor, values, indices, output) -> None:
     """massively/src/api/algorithm/scatter.rs:248-265: output[indices[i]] = values[i]."""
-    check(_lib().msv_scatter(exec.handle, as_slice(values)._c(), as_slice(indices)._c(),
+    check(_lib().msv_scatter(exec.handle, _in(exec, values)._c(), _in(exec, indices)._c(),
                              as_slice(output)._c()))
 
 
 def scatter_reduce(exec: Executor, values, indices, init, op: ReductionOp, output) -> None:
     """massively/src/api/algorithm/scatter.rs:337-420: output[d] = op(output[d],
     init (+) the values aimed at d); ``init`` must be op's identity."""
-    v = as_slice(values)
+    v = _in(exec, values)
     i = np.array([init], dtype=v.dtype)
-    check(_lib().msv_scatter_reduce(exec.handle, v._c(), as_slice(indices)._c(),
+    check(_lib().msv_scatter_reduce(exec.handle, v._c(), _in(exec, indices)._c(),
                                     i.ctypes.data_as(C.c_void_p), op.code, as_slice(output)._c()))
 
 
@@ -157,8 +205,8 @@ def radix_sort_by_key(exec: Executor, keys, values, return_keys: bool = False):
     if isinstance(keys, (tuple, list)) or isinstance(values, (tuple, list)):
         if return_keys:
             raise ValueError("return_keys is not available for zipped columns")
-        kc = [as_slice(k)._c() for k in (keys if isinstance(keys, (tuple, list)) else (keys,))]
-        vc = [as_slice(v)._c() for v in (values if isinstance(values, (tuple, list)) else (values,))]
+        kc = [_in(exec, k)._c() for k in (keys if isinstance(keys, (tuple, list)) else (keys,))]
+        vc = [_in(exec, v)._c() for v in (values if isinstance(values, (tuple, list)) else (values,))]
         karr = (L.MsvSlice * len(kc))(*kc)
         varr = (L.MsvSlice * len(vc))(*vc)
         outs = (C.c_void_p * len(vc))()
@@ -166,7 +214,7 @@ def radix_sort_by_key(exec: Executor, keys, values, return_keys: bool = False):
         res = tuple(_out(exec, C.c_void_p(h)) for h in outs)
         return res if isinstance(values, (tuple, list)) else res[0]
     hv, hk = C.c_void_p(), C.c_void_p()
-    check(_lib().msv_radix_sort_by_key(exec.handle, as_slice(keys)._c(), as_slice(values)._c(),
+    check(_lib().msv_radix_sort_by_key(exec.handle, _in(exec, keys)._c(), _in(exec, values)._c(),
                                        C.byref(hv), C.byref(hk) if return_keys else None))
     v = _out(exec, hv)
     return (v, _out(exec, hk)) if return_keys else v
@@ -175,7 +223,7 @@ def radix_sort_by_key(exec: Executor, keys, values, return_keys: bool = False):
 def sort_by_key(exec: Executor, keys, values, less: BinaryPredicateOp, return_keys=False):
     """massively/src/api/algorithm/by_key.rs:193-250: stable comparison sort."""
     hv, hk = C.c_void_p(), C.c_void_p()
-    check(_lib().msv_sort_by_key(exec.handle, as_slice(keys)._c(), as_slice(values)._c(),
+    check(_lib().msv_sort_by_key(exec.handle, _in(exec, keys)._c(), _in(exec, values)._c(),
                                  less.code, C.byref(hv), C.byref(hk) if return_keys else None))
     v = _out(exec, hv)
     return (v, _out(exec, hk)) if return_keys else v
@@ -184,14 +232,14 @@ def sort_by_key(exec: Executor, keys, values, less: BinaryPredicateOp, return_ke
 def sort(exec: Executor, input, less: BinaryPredicateOp) -> DeviceVec:
     """massively/src/api/algorithm/ordering.rs:64-76."""
     h = C.c_void_p()
-    check(_lib().msv_sort(exec.handle, as_slice(input)._c(), less.code, C.byref(h)))
+    check(_lib().msv_sort(exec.handle, _in(exec, input)._c(), less.code, C.byref(h)))
     return _out(exec, h)
 
 
 def merge(exec: Executor, left, right, less: BinaryPredicateOp) -> DeviceVec:
     """massively/src/api/algorithm/merge.rs:109-190: stable, ties take left."""
     h = C.c_void_p()
-    check(_lib().msv_merge(exec.handle, as_slice(left)._c(), as_slice(right)._c(), less.code,
+    check(_lib().msv_merge(exec.handle, _in(exec, left)._c(), _in(exec, right)._c(), less.code,
                            C.byref(h)))
     return _out(exec, h)
 
@@ -201,15 +249,15 @@ def merge_by_key(exec: Executor, left_keys, left_values, right_keys, right_value
     """massively/src/api/algorithm/merge.rs:195-282. Returns (keys, values)."""
     hk, hv = C.c_void_p(), C.c_void_p()
     check(_lib().msv_merge_by_key(
-        exec.handle, as_slice(left_keys)._c(), as_slice(left_values)._c(),
-        as_slice(right_keys)._c(), as_slice(right_values)._c(), less.code,
+        exec.handle, _in(exec, left_keys)._c(), _in(exec, left_values)._c(),
+        _in(exec, right_keys)._c(), _in(exec, right_values)._c(), less.code,
         C.byref(hk), C.byref(hv)))
     return _out(exec, hk), _out(exec, hv)
 
 
 def _set(kind: int, exec: Executor, left, right, less: BinaryPredicateOp) -> DeviceVec:
     h = C.c_void_p()
-    check(_lib().msv_set_op(exec.handle, kind, as_slice(left)._c(), as_slice(right)._c(),
+    check(_lib().msv_set_op(exec.handle, kind, _in(exec, left)._c(), _in(exec, right)._c(),
                             less.code, C.byref(h)))
     return _out(exec, h)
 
@@ -233,17 +281,17 @@ def set_difference(exec: Executor, left, right, less: BinaryPredicateOp) -> Devi
 def inclusive_scan_by_key(exec: Executor, keys, values, op: ReductionOp) -> DeviceVec:
     """massively/src/api/algorithm/by_key.rs:399-427 with the built-in Equal."""
     h = C.c_void_p()
-    check(_lib().msv_inclusive_scan_by_key(exec.handle, as_slice(keys)._c(), as_slice(values)._c(),
+    check(_lib().msv_inclusive_scan_by_key(exec.handle, _in(exec, keys)._c(), _in(exec, values)._c(),
                                            op.code, C.byref(h)))
     return _out(exec, h)
 
 
 def exclusive_scan_by_key(exec: Executor, keys, values, init, op: ReductionOp) -> DeviceVec:
     """massively/src/api/algorithm/by_key.rs:497-546 with the built-in Equal."""
-    v = as_slice(values)
+    v = _in(exec, values)
     i = _scalar(init, v.dtype)
     h = C.c_void_p()
-    check(_lib().msv_exclusive_scan_by_key(exec.handle, as_slice(keys)._c(), v._c(),
+    check(_lib().msv_exclusive_scan_by_key(exec.handle, _in(exec, keys)._c(), v._c(),
                                            i.ctypes.data_as(C.c_void_p), op.code, C.byref(h)))
     return _out(exec, h)
 
@@ -251,10 +299,10 @@ def exclusive_scan_by_key(exec: Executor, keys, values, init, op: ReductionOp) -
 def reduce_by_key(exec: Executor, keys, values, init, op: ReductionOp):
     """massively/src/api/algorithm/by_key.rs:618-682: (run keys, init (+) run values),
     both with a device-resident length."""
-    v = as_slice(values)
+    v = _in(exec, values)
     i = _scalar(init, v.dtype)
     hk, hv = C.c_void_p(), C.c_void_p()
-    check(_lib().msv_reduce_by_key(exec.handle, as_slice(keys)._c(), v._c(),
+    check(_lib().msv_reduce_by_key(exec.handle, _in(exec, keys)._c(), v._c(),
                                    i.ctypes.data_as(C.c_void_p), op.code, C.byref(hk), C.byref(hv)))
     return _out(exec, hk), _out(exec, hv)
 
@@ -262,6 +310,17 @@ def reduce_by_key(exec: Executor, keys, values, init, op: ReductionOp):
 def unique_by_key(exec: Executor, keys, values) -> DeviceVec:
     """massively/src/api/algorithm/by_key.rs:759-789: the first value of each key run."""
     h = C.c_void_p()
-    check(_lib().msv_unique_by_key(exec.handle, as_slice(keys)._c(), as_slice(values)._c(),
+    check(_lib().msv_unique_by_key(exec.handle, _in(exec, keys)._c(), _in(exec, values)._c(),
                                    C.byref(h)))
     return _out(exec, h)
+
+
+def map(exec: Executor, input, op: UnaryOp) -> DeviceVec:  # noqa: A001 - the reference's name
+    """massively/src/api/algorithm/transform.rs:91-108: op(x) for each element,
+    computed from the (possibly lazy) input in one kernel."""
+    return lazy.map_into_vec(exec, lazy.map(input, op))
+
+
+def adjacent_difference(exec: Executor, input, op: ReductionOp) -> DeviceVec:
+    """massively/src/api/algorithm/scan.rs:156-175: x[0], then op(x[i-1], x[i])."""
+    return lazy.map_into_vec(exec, lazy.adjacent_difference(input, op))

@@ -155,6 +155,35 @@ def literal_vectors() -> list[dict]:
                   offsets=[0, 0, 3, 6, 7, 10], take=2, expected=[2, 4, 1, 1, 5, 7, 8],
                   expected_offsets=[0, 0, 2, 4, 5, 7], expected_take0=[],
                   expected_offsets_take0=[0, 0, 0, 0, 0, 0]))
+    # lazy inputs, each written out with vector::map(.., op::Identity)
+    L = "massively/src/api/lazy.rs"
+    v.append(dict(name="lazy_stride_doc", src=f"{L}:700-703", op="lazy", tree="stride",
+                  start=2, step=3, take=4, expected=[2, 5, 8, 11]))
+    v.append(dict(name="lazy_map_double_doc", src=f"{L}:771-776", op="lazy", tree="map",
+                  input=[1, 2, 3], unary="x * 2u", expected=[2, 4, 6]))
+    v.append(dict(name="lazy_permute_doc", src=f"{L}:790-796", op="lazy", tree="permute",
+                  values=[10, 20, 30], indices=[2, 0], expected=[30, 10]))
+    v.append(dict(name="lazy_reverse_doc", src=f"{L}:813-817", op="lazy", tree="reverse",
+                  input=[10, 20, 30], expected=[30, 20, 10]))
+    v.append(dict(name="lazy_repeat_each_doc", src=f"{L}:834-842", op="lazy", tree="repeat_each",
+                  input=[10, 20], repeats=3, expected=[10, 10, 10, 20, 20, 20]))
+    v.append(dict(name="lazy_tile_doc", src=f"{L}:859-863", op="lazy", tree="tile",
+                  input=[10, 20], repeats=3, expected=[10, 20, 10, 20, 10, 20]))
+    v.append(dict(name="lazy_adjacent_difference_doc", src=f"{L}:880-898", op="lazy",
+                  tree="adjacent_difference", input=[3, 8, 10], reduction="b - a",
+                  expected=[3, 5, 2]))
+    v.append(dict(name="lazy_identity_doc", src=f"{L}:914-918", op="lazy", tree="identity",
+                  input=[1, 2, 3], expected=[1, 2, 3]))
+    v.append(dict(name="lazy_counting_slice", src="massively/tests/vector/api/runtime.rs:46-63",
+                  op="lazy", tree="counting_slice", start=10, take=5, slice=[1, 4],
+                  expected=[11, 12, 13]))
+    v.append(dict(name="lazy_constant_gather", src="massively/tests/vector/api/runtime.rs:77-87",
+                  op="lazy", tree="constant_gather", value=7, take=1, indices_value=0,
+                  indices_take=3, expected_gather=[7, 7, 7], expected_map=[7]))
+    v.append(dict(name="vector_map_doc", src="massively/src/api/algorithm/transform.rs:72-90",
+                  op="map", dtype="u32", input=[1, 2, 3], unary="x + 1u", expected=[2, 3, 4]))
+    v.append(dict(name="adjacent_difference_doc", src="massively/src/api/algorithm/scan.rs:132-155",
+                  op="adjacent_difference", dtype="u32", input=[1, 2, 3, 4], expected=[1, 3, 5, 7]))
     v.append(dict(name="segmentation_from_lengths", src="massively/src/seg/segmentation.rs:105-128",
                   op="segmentation", lengths=[1, 2, 3], expected_offsets=[0, 1, 3, 6],
                   expected_segment_ids=[0, 1, 1, 2, 2, 2]))
