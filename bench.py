@@ -399,6 +399,46 @@ def w_compact(c):
     return out
 
 
+def w_lazy(c):
+    """Lazy inputs fused into the consuming kernel, 2^28 elements: reduce of
+    map(x, x*x) over f32 (against writing the map out and reducing it), the
+    max of counting(0).take(2^32 - 1) (no memory traffic), and the scan and
+    copy_where of map(y, x*3+1) over u32."""
+    np, m = c.np, c.m
+    from massively_amd import lazy
+    from massively_amd.op import UnaryOp
+
+    N = 1 << 28
+    raw = c.ex.alloc(np.uint32, N)
+    c.mr.uniform_u32(c.ex, 0x3F000000, 0x3F7FFFFF, 41, raw)
+    x = c.ex.wrap(raw.device_ptr, np.float32, N)
+    sq = lazy.map(x, UnaryOp("x * x"))
+    out = {}
+    ms = timed(c.ex, lambda: m.vector.reduce(c.ex, sq, 0.0, m.op.Add), c.steps, c.warmup)
+    out["reduce_map_f32"] = summarize("reduce_map_sq_f32_2^28_fused", ms, 4.0 * N, N)
+    ms = timed(c.ex, lambda: m.vector.reduce(c.ex, sq.materialize(c.ex), 0.0, m.op.Add),
+               c.steps, c.warmup)
+    out["reduce_map_f32_unfused"] = summarize("reduce_map_sq_f32_2^28_written_out", ms, 4.0 * N, N)
+    cnt = lazy.counting(0).take(0xFFFFFFFF)
+    ms = timed(c.ex, lambda: m.vector.reduce(c.ex, cnt, 0, m.op.Max), c.steps, c.warmup)
+    out["reduce_counting_max"] = summarize("reduce_max_counting_2^32-1", ms, None, 0xFFFFFFFF)
+    y = c.ex.alloc(np.uint32, N)
+    c.mr.uniform_u32(c.ex, 0, 2**32 - 1, 42, y)
+    t = lazy.map(y, UnaryOp("x * 3u + 1u"))
+    ms = timed(c.ex, lambda: m.vector.inclusive_scan(c.ex, t, m.op.Add), c.steps, c.warmup)
+    out["scan_map_u32"] = summarize("inclusive_scan_map_u32_2^28_fused", ms, 8.0 * N, N)
+    last = [None]
+
+    def sel():
+        last[0] = m.vector.copy_where_if(c.ex, t, t, m.op.Lt(1 << 31))
+
+    ms = timed(c.ex, sel, c.steps, c.warmup)
+    kept = last[0].len()  # the device-resident length, read after the timed region
+    out["copy_where_map_u32"] = summarize("copy_where_lt_map_u32_2^28_fused", ms,
+                                          4.0 * N + 4.0 * kept, N, kept=kept)
+    return out
+
+
 def served_trace(np, jobs=500, seed=10, log2_max=26):
     """The served mixed stream: kinds uniform over sort_by_key / inclusive_scan /
     reduce / copy_where, sizes log-uniform in 2^10..2^log2_max, key dtypes
@@ -580,6 +620,7 @@ WORKLOADS = {
     "bykey": w_bykey,
     "compact": w_compact,
     "columns": w_columns,
+    "lazy": w_lazy,
     "served": w_served,
 }
 
