@@ -241,10 +241,52 @@ __device__ inline auto src_at(const Src& s, size_t i) {
   else return s[i];
 }
 
+// A lazy source also evaluates K consecutive elements at once (s.load<K>),
+// which turns its slot reads into vector loads where the range allows; used
+// when all four elements are in range (`full`).
 template <typename T, typename Src, typename = std::enable_if_t<is_lazy_src<Src>>>
-__device__ inline void load4(const Src& s, size_t i, uint32_t n, bool, T (&v)[4]) {
+__device__ inline void load4(const Src& s, size_t i, uint32_t n, bool full, T (&v)[4]) {
+  if (full) {
+    s.template load<4>(i, v);
+  } else {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) v[k] = (i + k < n) ? (T)s(i + k) : (T)0;
+    for (int k = 0; k < 4; ++k) v[k] = (i + k < n) ? (T)s(i + k) : (T)0;
+  }
+}
+
+// K consecutive elements of a device slot of n elements from index i: one
+// 8- or 16-byte load when they are all in range and the address is aligned to
+// their size, else guarded element loads (0 past n). Lazy read sources use it
+// for their slot leaves.
+template <int K, typename D>
+__device__ inline void slot_read(const void* base, uint32_t n, size_t i, D (&o)[K]) {
+  const D* p = (const D*)base;
+  constexpr size_t B = K * sizeof(D);
+  if constexpr (B == 16) {
+    if (i + K <= n && ((uintptr_t)(p + i) & 15u) == 0) {
+      const uint4 q = *reinterpret_cast<const uint4*>(p + i);
+      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if constexpr (sizeof(D) == 4) o[k] = bit_cast<D>(w[k]);
+        else o[k] = bit_cast<D>(((uint64_t)w[2 * k + 1] << 32) | w[2 * k]);
+      }
+      return;
+    }
+  } else if constexpr (B == 8) {
+    if (i + K <= n && ((uintptr_t)(p + i) & 7u) == 0) {
+      const uint2 q = *reinterpret_cast<const uint2*>(p + i);
+      if constexpr (sizeof(D) == 4) {
+        o[0] = bit_cast<D>(q.x);
+        o[1] = bit_cast<D>(q.y);
+      } else {
+        o[0] = bit_cast<D>(((uint64_t)q.y << 32) | q.x);
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) o[k] = (i + k < n) ? p[i + k] : (D)0;
 }
 
 // The by-key family's built-in Equal: numeric == on the key dtype (bitwise

@@ -67,6 +67,9 @@ struct msv_user_op {
   // the same kernels reading the lazy source
   hipFunction_t f_reduce_lz = nullptr, f_partials_lz = nullptr, f_scan_ex_lz = nullptr,
                 f_scan_in_lz = nullptr;
+  // built-in integer operator over a lazy input: the one-pass look-back scan
+  // [inclusive, exclusive] x [short tiles, long tiles]
+  hipFunction_t f_lb[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   uint64_t identity = 0;  // bits of the identity, for host-side padding values
   // predicate: compaction kernels for short and long inputs
   hipFunction_t f_sel_small = nullptr, f_sel_large = nullptr;
@@ -294,8 +297,30 @@ static msv_status user_scan_api(msv_executor* ex, const msv_slice* input, const 
   if (in.len == 0) return MSV_OK;
   Ctx c(ex);
   const size_t es = dtype_size(in.dtype);
-  const Scalar iv = exclusive ? scalar_of(init, es) : scalar_of(&op->identity, es);
-  msv_status s = user_scan(c, op, src, (*out)->ptr, in.len, exclusive, iv);
+  Scalar iv = exclusive ? scalar_of(init, es) : scalar_of(&op->identity, es);
+  msv_status s;
+  if (lz && op->f_lb[exclusive][0]) {
+    // One pass with decoupled look-back (integer results do not depend on
+    // the order of combination), as the built-in scans of device vectors.
+    const bool large = in.len >= kScanLargeMin;
+    const int rows = large ? (es == 4 ? scan_rows_large<uint32_t>() : scan_rows_large<uint64_t>())
+                           : kScanRowsSmall;
+    const uint32_t tile = (uint32_t)kScanBlock * (uint32_t)rows * (uint32_t)(16 / es);
+    uint32_t n = in.len, ntiles = div_up(n, tile);
+    uint64_t *counter, *status;
+    uint32_t gen;
+    s = c.lookback((size_t)ntiles * (es == 4 ? 1 : 2), &counter, &status, &gen);
+    if (s == MSV_OK) {
+      void* o = (*out)->ptr;
+      bool aligned = ((uintptr_t)o & 15u) == 0;
+      const void* prefixes = nullptr;
+      void* a[] = {(void*)&h,        (void*)&o,      (void*)&n,   (void*)&iv,
+                   (void*)&aligned, (void*)&status, (void*)&gen, (void*)&prefixes};
+      s = launch(op->f_lb[exclusive][large], ntiles, kScanBlock, c.stream(), a);
+    }
+  } else {
+    s = user_scan(c, op, src, (*out)->ptr, in.len, exclusive, iv);
+  }
   if (s != MSV_OK) {
     msv_vec_release(*out);
     *out = nullptr;
@@ -471,11 +496,21 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
     src += "#define MSV_SLOT(k, D, i) ((size_t)(i) < (size_t)la.n[k] ? "
            "((const D*)la.p[k])[(size_t)(i)] : (D)0)\n";
     src += read_src;
+    // K consecutive elements: the source's own msv_read_v (vector slot reads)
+    // or element by element.
+    if (std::string(read_src).find("msv_read_v") == std::string::npos)
+      src += "\ntemplate <int K>\n__device__ inline void msv_read_v(const LazyArgs& la, size_t i, UT (&o)[K]) {\n"
+             "#pragma unroll\n  for (int k = 0; k < K; ++k) o[k] = msv_read(la, i + k);\n}\n";
     src += "\nstruct LazySrc : LazySource {\n  LazyArgs la;\n"
-           "  __device__ inline UT operator()(size_t i) const { return msv_read(la, i); }\n};\n"
+           "  __device__ inline UT operator()(size_t i) const { return msv_read(la, i); }\n"
+           "  template <int K>\n"
+           "  __device__ inline void load(size_t i, UT (&v)[K]) const { msv_read_v<K>(la, i, v); }\n};\n"
            "struct LazySrcBits : LazySource {\n  LazyArgs la;\n"
-           "  __device__ inline typename Bits<UT>::U operator()(size_t i) const {\n"
-           "    return bit_cast<typename Bits<UT>::U>(msv_read(la, i));\n  }\n};\n"
+           "  using U = typename Bits<UT>::U;\n"
+           "  __device__ inline U operator()(size_t i) const { return bit_cast<U>(msv_read(la, i)); }\n"
+           "  template <int K>\n  __device__ inline void load(size_t i, U (&v)[K]) const {\n"
+           "    UT t[K];\n    msv_read_v<K>(la, i, t);\n"
+           "#pragma unroll\n    for (int k = 0; k < K; ++k) v[k] = bit_cast<U>(t[k]);\n  }\n};\n"
            "static_assert(sizeof(LazySrc) == 288 && sizeof(LazySrcBits) == 288, \"layout\");\n"
            "}  // namespace msv\n";
   }
@@ -495,6 +530,13 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
     exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d>", T, code, kScanBlock, kScanRowsSmall));
     exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, true, %d, %d, %d>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
     exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, false, %d, %d, %d>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
+    if (lazy && builtin && !dtype_is_float(dtype)) {
+      const int large = wide ? scan_rows_large<uint64_t>() : scan_rows_large<uint32_t>();
+      for (int excl = 0; excl < 2; ++excl)
+        for (int rows : {kScanRowsSmall, large})
+          exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, %s, %d, %d, %d, msv::LazySrc>", T, code,
+                              excl ? "true" : "false", (int)SCAN_LOOKBACK, kScanBlock, rows));
+    }
     if (lazy) {
       exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, msv::LazySrc>", T, code, kReduceBlock, kReduceRows));
       exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, msv::LazySrc>", T, code, kScanBlock, kScanRowsSmall));
@@ -520,12 +562,11 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
                "  const size_t i0 = ((size_t)blockIdx.x * %d + threadIdx.x) * V;\n"
                "  if (i0 >= n) return;\n"
                "  msv::Vec16<msv::UT> w;\n"
-               "#pragma unroll\n"
-               "  for (int k = 0; k < V; ++k) w.v[k] = (i0 + k < n) ? s(i0 + k) : (msv::UT)0;\n"
                "  if (i0 + V <= n) {\n"
+               "    s.load<V>(i0, w.v);\n"
                "    *reinterpret_cast<msv::Vec16<msv::UT>*>(out + i0) = w;\n"
                "  } else {\n"
-               "    for (int k = 0; k < V; ++k) if (i0 + k < n) out[i0 + k] = w.v[k];\n"
+               "    for (int k = 0; k < V; ++k) if (i0 + k < n) out[i0 + k] = s(i0 + k);\n"
                "  }\n}\n",
                kMapBlock, kMapBlock);
   }
@@ -560,11 +601,18 @@ msv_status msv_lazy_op_create(msv_executor* ex, int32_t kind, msv_dtype dtype,
     op->f_partials = fns[1];
     op->f_scan_ex = fns[2];
     op->f_scan_in = fns[3];
+    size_t at = 4;
+    if (op->lazy && op->builtin && !dtype_is_float(dtype)) {
+      op->f_lb[0][0] = fns[at++];
+      op->f_lb[0][1] = fns[at++];
+      op->f_lb[1][0] = fns[at++];
+      op->f_lb[1][1] = fns[at++];
+    }
     if (op->lazy) {
-      op->f_reduce_lz = fns[4];
-      op->f_partials_lz = fns[5];
-      op->f_scan_ex_lz = fns[6];
-      op->f_scan_in_lz = fns[7];
+      op->f_reduce_lz = fns[at++];
+      op->f_partials_lz = fns[at++];
+      op->f_scan_ex_lz = fns[at++];
+      op->f_scan_in_lz = fns[at++];
     }
     // The identity's bits, evaluated once on the device.
     hipFunction_t fid = nullptr;

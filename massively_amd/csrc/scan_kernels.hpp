@@ -64,18 +64,24 @@ __device__ inline void load_tile(const T* __restrict__ in, size_t base, uint32_t
 }
 
 // A lazy source: element i is in(i) (the same thread-to-element layout).
+// A full tile reads V consecutive elements per row at once (in.load<V>).
 template <typename T, int BLOCK, int ROWS, bool NT = false, typename Src,
           typename = std::enable_if_t<is_lazy_src<Src>>>
-__device__ inline void load_tile(const Src& in, size_t base, uint32_t n, bool,
+__device__ inline void load_tile(const Src& in, size_t base, uint32_t n, bool vec_ok,
                                  T (&x)[ROWS][16 / sizeof(T)], T fill) {
   constexpr int V = 16 / sizeof(T);
   const uint32_t tid = threadIdx.x;
+  if (vec_ok) {
 #pragma unroll
-  for (int r = 0; r < ROWS; ++r) {
+    for (int r = 0; r < ROWS; ++r) in.template load<V>(base + ((size_t)r * BLOCK + tid) * V, x[r]);
+  } else {
 #pragma unroll
-    for (int k = 0; k < V; ++k) {
-      const size_t i = base + ((size_t)r * BLOCK + tid) * V + k;
-      x[r][k] = (i < n) ? (T)in(i) : fill;
+    for (int r = 0; r < ROWS; ++r) {
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        const size_t i = base + ((size_t)r * BLOCK + tid) * V + k;
+        x[r][k] = (i < n) ? (T)in(i) : fill;
+      }
     }
   }
 }

@@ -61,6 +61,8 @@ class _Gen:
         self.params: list[int] = []
         self.lines: list[str] = []
         self.memo: dict[int, str] = {}
+        self.memo_v: dict[int, str] = {}
+        self.info: dict[int, tuple] = {}  # slot / parameter indices of a node's scalar form
 
     def slot(self, s: DeviceSlice) -> int:
         if len(self.slots) == L.LAZY_SLOTS:
@@ -85,6 +87,22 @@ class _Gen:
         got = self.memo.get(id(node))
         return got if got is not None else node._emit(self)
 
+    def fn_v(self, node: "Lazy", body: str) -> str:
+        """The K-element form: o[k] = element i0 + k."""
+        name = self.memo[id(node)] + "_v"
+        self.memo_v[id(node)] = name
+        self.lines.append(
+            f"template <int K>\n__device__ inline void {name}(const LazyArgs& la, size_t i0, "
+            f"{_CTYPE[node.dtype]} (&o)[K]) {{ {body} }}")
+        return name
+
+    def emit_v(self, node: "Lazy") -> str:
+        got = self.memo_v.get(id(node))
+        if got is not None:
+            return got
+        self.emit(node)  # the scalar form first: it assigns the slots and parameters
+        return node._emit_v(self)
+
 
 class Lazy:
     """A lazy input of ``dtype`` and length ``len(self)``."""
@@ -100,6 +118,12 @@ class Lazy:
     def _emit(self, g: _Gen) -> str:
         raise NotImplementedError
 
+    def _emit_v(self, g: _Gen) -> str:
+        """Default K-element form: the scalar form per element (nodes whose
+        elements come from scattered indices)."""
+        sc = g.memo[id(self)]
+        return g.fn_v(self, f"for (int k = 0; k < K; ++k) o[k] = {sc}(la, i0 + k);")
+
     def slice(self, start: int = 0, end: int | None = None) -> "Lazy":
         """The lazy sub-range [start, end) (``MIter::slice``)."""
         n = self.length()
@@ -112,8 +136,12 @@ class Lazy:
         """The read source (``msv_read``) and the slots / parameters of a call."""
         g = _Gen()
         root = g.emit(self)
+        root_v = g.emit_v(self)
         src = "\n".join(g.lines)
-        src += f"\n__device__ inline UT msv_read(const LazyArgs& la, size_t i) {{ return (UT){root}(la, i); }}\n"
+        t = _CTYPE[self.dtype]
+        src += (f"\n__device__ inline UT msv_read(const LazyArgs& la, size_t i) {{ return (UT){root}(la, i); }}\n"
+                f"template <int K>\n__device__ inline void msv_read_v(const LazyArgs& la, size_t i, UT (&o)[K]) "
+                f"{{ {t} t[K]; {root_v}<K>(la, i, t); for (int k = 0; k < K; ++k) o[k] = (UT)t[k]; }}\n")
         args = L.MsvLazyArgs()
         for k, s in enumerate(g.slots):
             args.slots[k] = s._c()
@@ -141,7 +169,12 @@ class _Slot(Lazy):
 
     def _emit(self, g: _Gen) -> str:
         k = g.slot(self.s)
+        g.info[id(self)] = (k,)
         return g.fn(self, f"return MSV_SLOT({k}, {_CTYPE[self.dtype]}, i);")
+
+    def _emit_v(self, g: _Gen) -> str:
+        (k,) = g.info[id(self)]
+        return g.fn_v(self, f"slot_read<K, {_CTYPE[self.dtype]}>(la.p[{k}], la.n[{k}], i0, o);")
 
 
 def _node(x) -> Lazy:
@@ -165,7 +198,13 @@ class _Slice(Lazy):
     def _emit(self, g: _Gen) -> str:
         c = g.emit(self.inner)
         j = g.param(self.start)
+        g.info[id(self)] = (j,)
         return g.fn(self, f"return {c}(la, i + la.q[{j}]);")
+
+    def _emit_v(self, g: _Gen) -> str:
+        (j,) = g.info[id(self)]
+        c = g.emit_v(self.inner)
+        return g.fn_v(self, f"{c}<K>(la, i0 + la.q[{j}], o);")
 
 
 # ---- unbounded sources (lazy.rs:566-720) ------------------------------------
@@ -198,8 +237,14 @@ class _Constant(Lazy):
 
     def _emit(self, g: _Gen) -> str:
         j = g.param(_bits_u64(self.c.value, self.dtype))
+        g.info[id(self)] = (j,)
         u = "uint32_t" if self.dtype.itemsize == 4 else "uint64_t"
         return g.fn(self, f"return bit_cast<{_CTYPE[self.dtype]}>(({u})la.q[{j}]);")
+
+    def _emit_v(self, g: _Gen) -> str:
+        sc = g.memo[id(self)]
+        return g.fn_v(self, f"const {_CTYPE[self.dtype]} v = {sc}(la, 0); "
+                            f"for (int k = 0; k < K; ++k) o[k] = v;")
 
 
 class Counting:
@@ -223,7 +268,12 @@ class _Counting(Lazy):
 
     def _emit(self, g: _Gen) -> str:
         j = g.param(self.start)
+        g.info[id(self)] = (j,)
         return g.fn(self, f"return (uint32_t)(la.q[{j}] + i);")
+
+    def _emit_v(self, g: _Gen) -> str:
+        (j,) = g.info[id(self)]
+        return g.fn_v(self, f"for (int k = 0; k < K; ++k) o[k] = (uint32_t)(la.q[{j}] + i0 + k);")
 
 
 class Stride:
@@ -247,7 +297,13 @@ class _Stride(Lazy):
 
     def _emit(self, g: _Gen) -> str:
         a, b = g.param(self.start), g.param(self.step)
+        g.info[id(self)] = (a, b)
         return g.fn(self, f"return (uint32_t)(la.q[{a}] + (uint64_t)i * la.q[{b}]);")
+
+    def _emit_v(self, g: _Gen) -> str:
+        a, b = g.info[id(self)]
+        return g.fn_v(self, f"for (int k = 0; k < K; ++k) "
+                            f"o[k] = (uint32_t)(la.q[{a}] + (uint64_t)(i0 + k) * la.q[{b}]);")
 
 
 def constant(value, dtype=None) -> Constant:
@@ -277,6 +333,13 @@ class _Map(Lazy):
         ti, to = _CTYPE[self.inner.dtype], _CTYPE[self.dtype]
         return g.fn(self, f"const {ti} x = {c}(la, i); return ({to})({self.op.expr});")
 
+    def _emit_v(self, g: _Gen) -> str:
+        c = g.emit_v(self.inner)
+        ti, to = _CTYPE[self.inner.dtype], _CTYPE[self.dtype]
+        return g.fn_v(self, f"{ti} t[K]; {c}<K>(la, i0, t); for (int k = 0; k < K; ++k) "
+                            f"{{ const size_t i = i0 + k; (void)i; const {ti} x = t[k]; "
+                            f"o[k] = ({to})({self.op.expr}); }}")
+
 
 class _ZipMap(Lazy):
     """map over zipped inputs (``lazy::map(zip2(a, b), op)``, api/iter.rs:1169-1300):
@@ -301,6 +364,14 @@ class _ZipMap(Lazy):
         reads = " ".join(f"const {_CTYPE[n.dtype]} x{k} = {c}(la, i);"
                          for k, (n, c) in enumerate(zip(self.inners, names)))
         return g.fn(self, f"{reads} return ({_CTYPE[self.dtype]})({self.op.expr});")
+
+    def _emit_v(self, g: _Gen) -> str:
+        names = [g.emit_v(n) for n in self.inners]
+        arrays = " ".join(f"{_CTYPE[n.dtype]} t{k}[K]; {c}<K>(la, i0, t{k});"
+                          for k, (n, c) in enumerate(zip(self.inners, names)))
+        reads = " ".join(f"const {_CTYPE[n.dtype]} x{k} = t{k}[j];" for k, n in enumerate(self.inners))
+        return g.fn_v(self, f"{arrays} for (int j = 0; j < K; ++j) {{ const size_t i = i0 + j; (void)i; "
+                            f"{reads} o[j] = ({_CTYPE[self.dtype]})({self.op.expr}); }}")
 
 
 class _Permute(Lazy):
