@@ -26,6 +26,38 @@ def _in(exec: Executor, x) -> DeviceSlice:
     return lazy.as_input(exec, x)
 
 
+def _cols(x):
+    """The columns of a zipped input (a tuple or list, the reference's zip2 ..
+    zip12 rows), or None for a single column."""
+    if isinstance(x, (tuple, list)):
+        if not 1 <= len(x) <= 12:
+            raise ValueError("a zipped input has 1 to 12 columns")
+        return list(x)
+    return None
+
+
+def _col_len(exec: Executor, x) -> int:
+    s = _in(exec, x) if not isinstance(x, lazy.Lazy) else None
+    if s is None:
+        return len(x)
+    return s.vec.len() - s.offset if s.length == L.WHOLE else s.length
+
+
+def _same_lengths(exec: Executor, cols) -> int:
+    lens = [_col_len(exec, c) for c in cols]
+    for n in lens[1:]:
+        if n != lens[0]:
+            raise LengthMismatch(lens[0], n)
+    return lens[0]
+
+
+def _iota(exec: Executor, n: int, start: int = 0) -> DeviceVec:
+    v = exec.alloc(np.uint32, n)
+    if n:
+        check(_lib().msv_iota_u32(exec.handle, start, v.slice()._c()))
+    return v
+
+
 def _out(exec: Executor, h: C.c_void_p) -> DeviceVec:
     return DeviceVec(exec, h)
 
@@ -107,7 +139,12 @@ def reduce(exec: Executor, input, init, op: ReductionOp):
 
 
 def copy_where(exec: Executor, input, stencil) -> DeviceVec:
-    """massively/src/api/algorithm/selection.rs:146-165 (device-resident length)."""
+    """massively/src/api/algorithm/selection.rs:146-165 (device-resident length).
+    A zipped input (tuple of columns) gives the tuple of compacted columns."""
+    cols = _cols(input)
+    if cols is not None:
+        _same_lengths(exec, cols)
+        return tuple(copy_where(exec, c, stencil) for c in cols)
     h = C.c_void_p()
     check(_lib().msv_copy_where(exec.handle, _in(exec, input)._c(), _in(exec, stencil)._c(),
                                 C.byref(h)))
@@ -115,7 +152,11 @@ def copy_where(exec: Executor, input, stencil) -> DeviceVec:
 
 
 def remove_where(exec: Executor, input, stencil) -> DeviceVec:
-    """massively/src/api/algorithm/selection.rs:203-222."""
+    """massively/src/api/algorithm/selection.rs:203-222 (zipped inputs as copy_where)."""
+    cols = _cols(input)
+    if cols is not None:
+        _same_lengths(exec, cols)
+        return tuple(remove_where(exec, c, stencil) for c in cols)
     h = C.c_void_p()
     check(_lib().msv_remove_where(exec.handle, _in(exec, input)._c(), _in(exec, stencil)._c(),
                                   C.byref(h)))
@@ -127,7 +168,14 @@ def copy_where_if(exec: Executor, input, stencil, pred: PredicateOp, invert: boo
     :class:`UserPredicate` is evaluated on the input itself (``stencil`` must be
     the input). A lazy input with ``stencil`` the same object is computed
     inside the compaction kernel, and the predicate (built-in or user) is
-    applied to it."""
+    applied to it. A zipped input (tuple of columns) with a separate stencil
+    gives the tuple of compacted columns."""
+    cols = _cols(input)
+    if cols is not None:
+        if isinstance(pred, UserPredicate):
+            raise ValueError("a UserPredicate applies to a single input column")
+        _same_lengths(exec, cols)
+        return tuple(copy_where_if(exec, c, stencil, pred, invert) for c in cols)
     if isinstance(input, lazy.Lazy) and stencil is input:
         if invert:
             raise ValueError("remove_where_if of a lazy input: use the negated predicate")
@@ -171,7 +219,13 @@ def partition(exec: Executor, input, pred: PredicateOp):
 
 
 def gather(exec: Executor, values, indices) -> DeviceVec:
-    """massively/src/api/algorithm/indexed.rs:157-208: out[i] = values[indices[i]]."""
+    """massively/src/api/algorithm/indexed.rs:157-208: out[i] = values[indices[i]].
+    Zipped values (tuple of columns) give the tuple of gathered columns."""
+    cols = _cols(values)
+    if cols is not None:
+        _same_lengths(exec, cols)
+        idx = _in(exec, indices)
+        return tuple(gather(exec, c, idx) for c in cols)
     h = C.c_void_p()
     check(_lib().msv_gather(exec.handle, _in(exec, values)._c(), _in(exec, indices)._c(),
                             C.byref(h)))
@@ -179,7 +233,19 @@ def gather(exec: Executor, values, indices) -> DeviceVec:
 
 
 def scatter(exec: Executor, values, indices, output) -> None:
-    """massively/src/api/algorithm/scatter.rs:248-265: output[indices[i]] = values[i]."""
+    """massively/src/api/algorithm/scatter.rs:248-265: output[indices[i]] = values[i].
+    Zipped values scatter column by column into the zipped output."""
+    cols = _cols(values)
+    if cols is not None:
+        outs = _cols(output)
+        if outs is None or len(outs) != len(cols):
+            raise ValueError("zipped values need a zipped output of as many columns")
+        _same_lengths(exec, cols)
+        _same_lengths(exec, outs)
+        idx = _in(exec, indices)
+        for c, o in zip(cols, outs):
+            scatter(exec, c, idx, o)
+        return
     check(_lib().msv_scatter(exec.handle, _in(exec, values)._c(), _in(exec, indices)._c(),
                              as_slice(output)._c()))
 
@@ -221,7 +287,15 @@ def radix_sort_by_key(exec: Executor, keys, values, return_keys: bool = False):
 
 
 def sort_by_key(exec: Executor, keys, values, less: BinaryPredicateOp, return_keys=False):
-    """massively/src/api/algorithm/by_key.rs:193-250: stable comparison sort."""
+    """massively/src/api/algorithm/by_key.rs:193-250: stable comparison sort.
+    Zipped values (tuple of columns): the keys are sorted once with their row
+    indices and each column is gathered by that permutation."""
+    cols = _cols(values)
+    if cols is not None:
+        n = _same_lengths(exec, cols)
+        perm, sk = sort_by_key(exec, keys, _iota(exec, n), less, return_keys=True)
+        res = gather(exec, tuple(cols), perm)
+        return (res, sk) if return_keys else res
     hv, hk = C.c_void_p(), C.c_void_p()
     check(_lib().msv_sort_by_key(exec.handle, _in(exec, keys)._c(), _in(exec, values)._c(),
                                  less.code, C.byref(hv), C.byref(hk) if return_keys else None))
@@ -246,7 +320,26 @@ def merge(exec: Executor, left, right, less: BinaryPredicateOp) -> DeviceVec:
 
 def merge_by_key(exec: Executor, left_keys, left_values, right_keys, right_values,
                  less: BinaryPredicateOp):
-    """massively/src/api/algorithm/merge.rs:195-282. Returns (keys, values)."""
+    """massively/src/api/algorithm/merge.rs:195-282. Returns (keys, values).
+    Zipped values (tuples of as many columns on both sides): the keys are merged
+    once with their row indices (left rows first, then right rows) and each
+    column is gathered from the two sides by that order."""
+    lc, rc = _cols(left_values), _cols(right_values)
+    if lc is not None or rc is not None:
+        if lc is None or rc is None or len(lc) != len(rc):
+            raise ValueError("zipped values need as many columns on both sides")
+        nl, nr = _same_lengths(exec, lc), _same_lengths(exec, rc)
+        keys, pos = merge_by_key(exec, left_keys, _iota(exec, nl), right_keys, _iota(exec, nr, nl),
+                                 less)
+        outs = []
+        for a, b in zip(lc, rc):
+            both = exec.alloc(_in(exec, a).dtype, nl + nr)
+            if nl:
+                scatter(exec, a, lazy.counting(0).take(nl), both)
+            if nr:
+                scatter(exec, b, lazy.counting(nl).take(nr), both)
+            outs.append(gather(exec, both, pos))
+        return keys, tuple(outs)
     hk, hv = C.c_void_p(), C.c_void_p()
     check(_lib().msv_merge_by_key(
         exec.handle, _in(exec, left_keys)._c(), _in(exec, left_values)._c(),
@@ -279,7 +372,13 @@ def set_difference(exec: Executor, left, right, less: BinaryPredicateOp) -> Devi
 
 # ---- by-key family (built-in Equal: numeric == on the key dtype) -------------
 def inclusive_scan_by_key(exec: Executor, keys, values, op: ReductionOp) -> DeviceVec:
-    """massively/src/api/algorithm/by_key.rs:399-427 with the built-in Equal."""
+    """massively/src/api/algorithm/by_key.rs:399-427 with the built-in Equal.
+    Zipped values: op applies to each column (a row-wise operator made of one
+    built-in operator per component)."""
+    cols = _cols(values)
+    if cols is not None:
+        _same_lengths(exec, cols)
+        return tuple(inclusive_scan_by_key(exec, keys, c, op) for c in cols)
     h = C.c_void_p()
     check(_lib().msv_inclusive_scan_by_key(exec.handle, _in(exec, keys)._c(), _in(exec, values)._c(),
                                            op.code, C.byref(h)))
@@ -287,7 +386,13 @@ def inclusive_scan_by_key(exec: Executor, keys, values, op: ReductionOp) -> Devi
 
 
 def exclusive_scan_by_key(exec: Executor, keys, values, init, op: ReductionOp) -> DeviceVec:
-    """massively/src/api/algorithm/by_key.rs:497-546 with the built-in Equal."""
+    """massively/src/api/algorithm/by_key.rs:497-546 with the built-in Equal.
+    Zipped values: per column, ``init`` a tuple of one value per column."""
+    cols = _cols(values)
+    if cols is not None:
+        _same_lengths(exec, cols)
+        inits = init if isinstance(init, (tuple, list)) else (init,) * len(cols)
+        return tuple(exclusive_scan_by_key(exec, keys, c, i0, op) for c, i0 in zip(cols, inits))
     v = _in(exec, values)
     i = _scalar(init, v.dtype)
     h = C.c_void_p()
@@ -298,7 +403,14 @@ def exclusive_scan_by_key(exec: Executor, keys, values, init, op: ReductionOp) -
 
 def reduce_by_key(exec: Executor, keys, values, init, op: ReductionOp):
     """massively/src/api/algorithm/by_key.rs:618-682: (run keys, init (+) run values),
-    both with a device-resident length."""
+    both with a device-resident length. Zipped values: per column (``init`` a
+    tuple of one value per column), with the run keys once."""
+    cols = _cols(values)
+    if cols is not None:
+        _same_lengths(exec, cols)
+        inits = init if isinstance(init, (tuple, list)) else (init,) * len(cols)
+        pairs = [reduce_by_key(exec, keys, c, i0, op) for c, i0 in zip(cols, inits)]
+        return pairs[0][0], tuple(v for _, v in pairs)
     v = _in(exec, values)
     i = _scalar(init, v.dtype)
     hk, hv = C.c_void_p(), C.c_void_p()
@@ -308,7 +420,12 @@ def reduce_by_key(exec: Executor, keys, values, init, op: ReductionOp):
 
 
 def unique_by_key(exec: Executor, keys, values) -> DeviceVec:
-    """massively/src/api/algorithm/by_key.rs:759-789: the first value of each key run."""
+    """massively/src/api/algorithm/by_key.rs:759-789: the first value of each key run
+    (zipped values: per column)."""
+    cols = _cols(values)
+    if cols is not None:
+        _same_lengths(exec, cols)
+        return tuple(unique_by_key(exec, keys, c) for c in cols)
     h = C.c_void_p()
     check(_lib().msv_unique_by_key(exec.handle, _in(exec, keys)._c(), _in(exec, values)._c(),
                                    C.byref(h)))

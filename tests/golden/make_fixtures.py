@@ -180,6 +180,18 @@ def literal_vectors() -> list[dict]:
     v.append(dict(name="lazy_constant_gather", src="massively/tests/vector/api/runtime.rs:77-87",
                   op="lazy", tree="constant_gather", value=7, take=1, indices_value=0,
                   indices_take=3, expected_gather=[7, 7, 7], expected_map=[7]))
+    T = "massively/tests/vector/api/lazy.rs"
+    v.append(dict(name="lazy_reverse_sliced", src=f"{T}:295-299", op="lazy", tree="reverse_slices",
+                  input=[10, 20, 30, 40, 50], slices=[[1, 4], [1, 2]], expected=[30]))
+    v.append(dict(name="lazy_repeat_each_sliced", src=f"{T}:311-325", op="lazy",
+                  tree="repeat_each_slices", input=[10, 20, 30], repeats=3, slices=[[2, 8], [1, 5]],
+                  expected=[20, 20, 20, 30], input_second=[1, 2, 3], expected_second=[2, 2, 2, 3]))
+    v.append(dict(name="lazy_tile_sliced", src=f"{T}:327-341", op="lazy", tree="tile_slices",
+                  input=[10, 20, 30], repeats=3, slices=[[2, 8], [1, 5]], expected=[10, 20, 30, 10],
+                  input_second=[1, 2, 3], expected_second=[1, 2, 3, 1]))
+    v.append(dict(name="zip3_map_sliced", src="massively/tests/vector/api/iter.rs:178-189", op="lazy",
+                  tree="zip3_map_slices", columns=[[1, 2, 3, 4], [10, 20, 30, 40], [100, 200, 300, 400]],
+                  slices=[[1, 4], [1, 2]], expected=[333]))
     v.append(dict(name="vector_map_doc", src="massively/src/api/algorithm/transform.rs:72-90",
                   op="map", dtype="u32", input=[1, 2, 3], unary="x + 1u", expected=[2, 3, 4]))
     v.append(dict(name="adjacent_difference_doc", src="massively/src/api/algorithm/scan.rs:132-155",
