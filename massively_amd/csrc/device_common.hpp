@@ -235,6 +235,15 @@ struct LazySource {};
 template <typename Src>
 constexpr bool is_lazy_src = __is_base_of(LazySource, Src);
 
+// Whether a lazy source can be read as unchecked vectors over [lo, hi) (its
+// slots in range and aligned); a pointer source always can (the caller checks
+// its alignment). One test per tile keeps the tile's row loads branch-free.
+template <typename Src>
+__device__ inline bool src_tile_ok(const Src& s, size_t lo, size_t hi) {
+  if constexpr (is_lazy_src<Src>) return s.tile_ok(lo, hi);
+  else return true;
+}
+
 template <typename Src>
 __device__ inline auto src_at(const Src& s, size_t i) {
   if constexpr (is_lazy_src<Src>) return s(i);
@@ -246,8 +255,8 @@ __device__ inline auto src_at(const Src& s, size_t i) {
 // when all four elements are in range (`full`).
 template <typename T, typename Src, typename = std::enable_if_t<is_lazy_src<Src>>>
 __device__ inline void load4(const Src& s, size_t i, uint32_t n, bool full, T (&v)[4]) {
-  if (full) {
-    s.template load<4>(i, v);
+  if (full) {  // the caller has checked src_tile_ok over the tile
+    s.template load_fast<4>(i, v);
   } else {
 #pragma unroll
     for (int k = 0; k < 4; ++k) v[k] = (i + k < n) ? (T)s(i + k) : (T)0;
@@ -287,6 +296,30 @@ __device__ inline void slot_read(const void* base, uint32_t n, size_t i, D (&o)[
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) o[k] = (i + k < n) ? p[i + k] : (D)0;
+}
+
+// The same without checks: the caller knows the K elements are in range and
+// p + i is aligned to 16 bytes (16-byte groups) or to the element.
+template <int K, typename D>
+__device__ inline void slot_read_fast(const void* base, size_t i, D (&o)[K]) {
+  const D* p = (const D*)base + i;
+  constexpr size_t B = K * sizeof(D);
+  if constexpr (B % 16 == 0) {
+#pragma unroll
+    for (int g = 0; g < (int)(B / 16); ++g) {
+      const uint4 q = reinterpret_cast<const uint4*>(p)[g];
+      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+      constexpr int E = 16 / sizeof(D);
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        if constexpr (sizeof(D) == 4) o[g * E + k] = bit_cast<D>(w[k]);
+        else o[g * E + k] = bit_cast<D>(((uint64_t)w[2 * k + 1] << 32) | w[2 * k]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k) o[k] = p[k];
+  }
 }
 
 // The by-key family's built-in Equal: numeric == on the key dtype (bitwise

@@ -43,6 +43,12 @@
 namespace msv {
 constexpr int kOpUser = 100;  // Combine<T, kOpUser> is the user's operator
 constexpr int kMapBlock = 256;
+// Rows of the long tiles of the one-pass scan over a lazy input: half of the
+// built-in scan's (a lazy tile's per-element fallback for the last tile holds
+// more registers; 16 rows of u32 take 134 VGPRs, 32 rows 265).
+inline int lazy_scan_rows_large(size_t es) {
+  return (es == 4 ? scan_rows_large<uint32_t>() : scan_rows_large<uint64_t>()) / 2;
+}
 
 // Host image of the device LazyArgs (same layout: kernels take it by value).
 struct LazyArgsHost {
@@ -303,8 +309,7 @@ static msv_status user_scan_api(msv_executor* ex, const msv_slice* input, const 
     // One pass with decoupled look-back (integer results do not depend on
     // the order of combination), as the built-in scans of device vectors.
     const bool large = in.len >= kScanLargeMin;
-    const int rows = large ? (es == 4 ? scan_rows_large<uint32_t>() : scan_rows_large<uint64_t>())
-                           : kScanRowsSmall;
+    const int rows = large ? lazy_scan_rows_large(es) : kScanRowsSmall;
     const uint32_t tile = (uint32_t)kScanBlock * (uint32_t)rows * (uint32_t)(16 / es);
     uint32_t n = in.len, ntiles = div_up(n, tile);
     uint64_t *counter, *status;
@@ -498,19 +503,33 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
     src += read_src;
     // K consecutive elements: the source's own msv_read_v (vector slot reads)
     // or element by element.
+    // Without them: element by element, and no tile is read unchecked.
     if (std::string(read_src).find("msv_read_v") == std::string::npos)
       src += "\ntemplate <int K>\n__device__ inline void msv_read_v(const LazyArgs& la, size_t i, UT (&o)[K]) {\n"
              "#pragma unroll\n  for (int k = 0; k < K; ++k) o[k] = msv_read(la, i + k);\n}\n";
+    if (std::string(read_src).find("msv_tile_ok") == std::string::npos)
+      src += "\ntemplate <int K>\n__device__ inline void msv_read_vf(const LazyArgs& la, size_t i, UT (&o)[K]) {\n"
+             "  msv_read_v<K>(la, i, o);\n}\n"
+             "__device__ inline bool msv_tile_ok(const LazyArgs&, size_t, size_t) { return false; }\n";
     src += "\nstruct LazySrc : LazySource {\n  LazyArgs la;\n"
            "  __device__ inline UT operator()(size_t i) const { return msv_read(la, i); }\n"
            "  template <int K>\n"
-           "  __device__ inline void load(size_t i, UT (&v)[K]) const { msv_read_v<K>(la, i, v); }\n};\n"
+           "  __device__ inline void load(size_t i, UT (&v)[K]) const { msv_read_v<K>(la, i, v); }\n"
+           "  template <int K>\n"
+           "  __device__ inline void load_fast(size_t i, UT (&v)[K]) const { msv_read_vf<K>(la, i, v); }\n"
+           "  __device__ inline bool tile_ok(size_t lo, size_t hi) const { return msv_tile_ok(la, lo, hi); }\n"
+           "};\n"
            "struct LazySrcBits : LazySource {\n  LazyArgs la;\n"
            "  using U = typename Bits<UT>::U;\n"
            "  __device__ inline U operator()(size_t i) const { return bit_cast<U>(msv_read(la, i)); }\n"
            "  template <int K>\n  __device__ inline void load(size_t i, U (&v)[K]) const {\n"
            "    UT t[K];\n    msv_read_v<K>(la, i, t);\n"
-           "#pragma unroll\n    for (int k = 0; k < K; ++k) v[k] = bit_cast<U>(t[k]);\n  }\n};\n"
+           "#pragma unroll\n    for (int k = 0; k < K; ++k) v[k] = bit_cast<U>(t[k]);\n  }\n"
+           "  template <int K>\n  __device__ inline void load_fast(size_t i, U (&v)[K]) const {\n"
+           "    UT t[K];\n    msv_read_vf<K>(la, i, t);\n"
+           "#pragma unroll\n    for (int k = 0; k < K; ++k) v[k] = bit_cast<U>(t[k]);\n  }\n"
+           "  __device__ inline bool tile_ok(size_t lo, size_t hi) const { return msv_tile_ok(la, lo, hi); }\n"
+           "};\n"
            "static_assert(sizeof(LazySrc) == 288 && sizeof(LazySrcBits) == 288, \"layout\");\n"
            "}  // namespace msv\n";
   }
@@ -531,7 +550,7 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
     exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, true, %d, %d, %d>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
     exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, false, %d, %d, %d>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
     if (lazy && builtin && !dtype_is_float(dtype)) {
-      const int large = wide ? scan_rows_large<uint64_t>() : scan_rows_large<uint32_t>();
+      const int large = lazy_scan_rows_large(dtype_size(dtype));
       for (int excl = 0; excl < 2; ++excl)
         for (int rows : {kScanRowsSmall, large})
           exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, %s, %d, %d, %d, msv::LazySrc>", T, code,

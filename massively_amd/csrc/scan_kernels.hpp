@@ -71,9 +71,9 @@ __device__ inline void load_tile(const Src& in, size_t base, uint32_t n, bool ve
                                  T (&x)[ROWS][16 / sizeof(T)], T fill) {
   constexpr int V = 16 / sizeof(T);
   const uint32_t tid = threadIdx.x;
-  if (vec_ok) {
+  if (vec_ok) {  // the kernel has checked src_tile_ok over the tile
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) in.template load<V>(base + ((size_t)r * BLOCK + tid) * V, x[r]);
+    for (int r = 0; r < ROWS; ++r) in.template load_fast<V>(base + ((size_t)r * BLOCK + tid) * V, x[r]);
   } else {
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(BLOCK) void scan_tile_kernel(Src in,
   const uint32_t wid = tid / kWave;
   const uint32_t tile = blockIdx.x;
   const size_t base = (size_t)tile * TILE;
-  const bool vec_ok = aligned && (base + TILE <= n);
+  const bool vec_ok = aligned && (base + TILE <= n) && src_tile_ok(in, base, base + TILE);
   const T id = C::identity();
 
   T x[ROWS][V];
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(BLOCK) void reduce_tile_kernel(const T* __restrict_
   __shared__ T s_wave[NW];
   const uint32_t tid = threadIdx.x;
   const size_t base = (size_t)blockIdx.x * TILE;
-  const bool vec_ok = aligned && (base + TILE <= n);
+  const bool vec_ok = aligned && (base + TILE <= n) && src_tile_ok(in, base, base + TILE);
   T x[ROWS][V];
   load_tile<T, BLOCK, ROWS>(in, base, n, vec_ok, x, C::identity());
   T flat[ROWS * V];
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(BLOCK) void reduce_tile_ordered_kernel(Src in,
   __shared__ T s_wave[ROWS][NW];
   const uint32_t tid = threadIdx.x;
   const size_t base = (size_t)blockIdx.x * TILE;
-  const bool vec_ok = aligned && (base + TILE <= n);
+  const bool vec_ok = aligned && (base + TILE <= n) && src_tile_ok(in, base, base + TILE);
   T x[ROWS][V];
   load_tile<T, BLOCK, ROWS>(in, base, n, vec_ok, x, C::identity());
 #pragma unroll

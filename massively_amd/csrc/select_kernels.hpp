@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(Src in,
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
   const uint32_t tile = blockIdx.x;
   const size_t base = (size_t)tile * kSelTile;
-  const bool full = aligned && base + kSelTile <= n;
+  const bool full = aligned && base + kSelTile <= n && src_tile_ok(in, base, base + kSelTile);
 
   T x[ROWS][4];
   uint32_t f[ROWS][4];

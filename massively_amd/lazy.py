@@ -87,13 +87,18 @@ class _Gen:
         got = self.memo.get(id(node))
         return got if got is not None else node._emit(self)
 
-    def fn_v(self, node: "Lazy", body: str) -> str:
-        """The K-element form: o[k] = element i0 + k."""
+    def fn_v(self, node: "Lazy", body: str, ok: str = "false") -> str:
+        """The K-element form (o[k] = element i0 + k; F: unchecked slot reads,
+        valid where the node's tile_ok holds) and the node's tile_ok(lo, hi):
+        whether its slots can be read unchecked over [lo, hi)."""
         name = self.memo[id(node)] + "_v"
         self.memo_v[id(node)] = name
         self.lines.append(
-            f"template <int K>\n__device__ inline void {name}(const LazyArgs& la, size_t i0, "
+            f"template <int K, bool F>\n__device__ inline void {name}(const LazyArgs& la, size_t i0, "
             f"{_CTYPE[node.dtype]} (&o)[K]) {{ {body} }}")
+        self.lines.append(
+            f"__device__ inline bool {name}_ok(const LazyArgs& la, size_t lo, size_t hi) "
+            f"{{ (void)la; (void)lo; (void)hi; return {ok}; }}")
         return name
 
     def emit_v(self, node: "Lazy") -> str:
@@ -141,7 +146,11 @@ class Lazy:
         t = _CTYPE[self.dtype]
         src += (f"\n__device__ inline UT msv_read(const LazyArgs& la, size_t i) {{ return (UT){root}(la, i); }}\n"
                 f"template <int K>\n__device__ inline void msv_read_v(const LazyArgs& la, size_t i, UT (&o)[K]) "
-                f"{{ {t} t[K]; {root_v}<K>(la, i, t); for (int k = 0; k < K; ++k) o[k] = (UT)t[k]; }}\n")
+                f"{{ {t} t[K]; {root_v}<K, false>(la, i, t); for (int k = 0; k < K; ++k) o[k] = (UT)t[k]; }}\n"
+                f"template <int K>\n__device__ inline void msv_read_vf(const LazyArgs& la, size_t i, UT (&o)[K]) "
+                f"{{ {t} t[K]; {root_v}<K, true>(la, i, t); for (int k = 0; k < K; ++k) o[k] = (UT)t[k]; }}\n"
+                f"__device__ inline bool msv_tile_ok(const LazyArgs& la, size_t lo, size_t hi) "
+                f"{{ return {root_v}_ok(la, lo, hi); }}\n")
         args = L.MsvLazyArgs()
         for k, s in enumerate(g.slots):
             args.slots[k] = s._c()
@@ -174,7 +183,10 @@ class _Slot(Lazy):
 
     def _emit_v(self, g: _Gen) -> str:
         (k,) = g.info[id(self)]
-        return g.fn_v(self, f"slot_read<K, {_CTYPE[self.dtype]}>(la.p[{k}], la.n[{k}], i0, o);")
+        t = _CTYPE[self.dtype]
+        return g.fn_v(self, f"if constexpr (F) slot_read_fast<K, {t}>(la.p[{k}], i0, o); "
+                            f"else slot_read<K, {t}>(la.p[{k}], la.n[{k}], i0, o);",
+                      ok=f"hi <= la.n[{k}] && ((uintptr_t)((const {t}*)la.p[{k}] + lo) & 15u) == 0")
 
 
 def _node(x) -> Lazy:
@@ -204,7 +216,8 @@ class _Slice(Lazy):
     def _emit_v(self, g: _Gen) -> str:
         (j,) = g.info[id(self)]
         c = g.emit_v(self.inner)
-        return g.fn_v(self, f"{c}<K>(la, i0 + la.q[{j}], o);")
+        return g.fn_v(self, f"{c}<K, F>(la, i0 + la.q[{j}], o);",
+                      ok=f"{c}_ok(la, lo + la.q[{j}], hi + la.q[{j}])")
 
 
 # ---- unbounded sources (lazy.rs:566-720) ------------------------------------
@@ -244,7 +257,7 @@ class _Constant(Lazy):
     def _emit_v(self, g: _Gen) -> str:
         sc = g.memo[id(self)]
         return g.fn_v(self, f"const {_CTYPE[self.dtype]} v = {sc}(la, 0); "
-                            f"for (int k = 0; k < K; ++k) o[k] = v;")
+                            f"for (int k = 0; k < K; ++k) o[k] = v;", ok="true")
 
 
 class Counting:
@@ -273,7 +286,8 @@ class _Counting(Lazy):
 
     def _emit_v(self, g: _Gen) -> str:
         (j,) = g.info[id(self)]
-        return g.fn_v(self, f"for (int k = 0; k < K; ++k) o[k] = (uint32_t)(la.q[{j}] + i0 + k);")
+        return g.fn_v(self, f"for (int k = 0; k < K; ++k) o[k] = (uint32_t)(la.q[{j}] + i0 + k);",
+                      ok="true")
 
 
 class Stride:
@@ -303,7 +317,7 @@ class _Stride(Lazy):
     def _emit_v(self, g: _Gen) -> str:
         a, b = g.info[id(self)]
         return g.fn_v(self, f"for (int k = 0; k < K; ++k) "
-                            f"o[k] = (uint32_t)(la.q[{a}] + (uint64_t)(i0 + k) * la.q[{b}]);")
+                            f"o[k] = (uint32_t)(la.q[{a}] + (uint64_t)(i0 + k) * la.q[{b}]);", ok="true")
 
 
 def constant(value, dtype=None) -> Constant:
@@ -336,9 +350,9 @@ class _Map(Lazy):
     def _emit_v(self, g: _Gen) -> str:
         c = g.emit_v(self.inner)
         ti, to = _CTYPE[self.inner.dtype], _CTYPE[self.dtype]
-        return g.fn_v(self, f"{ti} t[K]; {c}<K>(la, i0, t); for (int k = 0; k < K; ++k) "
+        return g.fn_v(self, f"{ti} t[K]; {c}<K, F>(la, i0, t); for (int k = 0; k < K; ++k) "
                             f"{{ const size_t i = i0 + k; (void)i; const {ti} x = t[k]; "
-                            f"o[k] = ({to})({self.op.expr}); }}")
+                            f"o[k] = ({to})({self.op.expr}); }}", ok=f"{c}_ok(la, lo, hi)")
 
 
 class _ZipMap(Lazy):
@@ -367,11 +381,12 @@ class _ZipMap(Lazy):
 
     def _emit_v(self, g: _Gen) -> str:
         names = [g.emit_v(n) for n in self.inners]
-        arrays = " ".join(f"{_CTYPE[n.dtype]} t{k}[K]; {c}<K>(la, i0, t{k});"
+        arrays = " ".join(f"{_CTYPE[n.dtype]} t{k}[K]; {c}<K, F>(la, i0, t{k});"
                           for k, (n, c) in enumerate(zip(self.inners, names)))
         reads = " ".join(f"const {_CTYPE[n.dtype]} x{k} = t{k}[j];" for k, n in enumerate(self.inners))
+        ok = " && ".join(f"{c}_ok(la, lo, hi)" for c in names)
         return g.fn_v(self, f"{arrays} for (int j = 0; j < K; ++j) {{ const size_t i = i0 + j; (void)i; "
-                            f"{reads} o[j] = ({_CTYPE[self.dtype]})({self.op.expr}); }}")
+                            f"{reads} o[j] = ({_CTYPE[self.dtype]})({self.op.expr}); }}", ok=ok)
 
 
 class _Permute(Lazy):
