@@ -436,7 +436,9 @@ static msv_status user_select_api(msv_executor* ex, const msv_slice* input,
     }
   } else {
     const bool large = n >= kSelLargeMin;
-    const uint32_t tile = (uint32_t)kSelBlock * (large ? kSelRowsLarge : kSelRowsSmall) * 4;
+    const uint32_t tile = (uint32_t)kSelBlock *
+        (large ? (dtype_size(in.dtype) == 4 ? sel_rows_large<uint32_t>() : sel_rows_large<uint64_t>())
+               : kSelRowsSmall) * 4;
     uint32_t ntiles = div_up(n, tile);
     uint64_t *counter, *status;
     uint32_t gen;
@@ -573,7 +575,8 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
     const char* U = wide ? "uint64_t" : "uint32_t";
     const char* S = lazy ? ", msv::LazySrcBits" : "";
     exprs.push_back(fmt("msv::select_kernel<%s, %s, 2, %d, false, %s%s>", U, T, kSelRowsSmall, P.c_str(), S));
-    exprs.push_back(fmt("msv::select_kernel<%s, %s, 2, %d, false, %s%s>", U, T, kSelRowsLarge, P.c_str(), S));
+    exprs.push_back(fmt("msv::select_kernel<%s, %s, 2, %d, false, %s%s>", U, T,
+                        wide ? sel_rows_large<uint64_t>() : sel_rows_large<uint32_t>(), P.c_str(), S));
   } else {
     src += fmt("extern \"C\" __global__ __launch_bounds__(%d) void msv_lazy_map(msv::LazySrc s,\n"
                "    msv::UT* __restrict__ out, uint32_t n) {\n"

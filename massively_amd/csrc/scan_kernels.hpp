@@ -8,7 +8,8 @@ namespace msv {
 
 constexpr int kScanBlock = 256;
 constexpr int kScanRowsSmall = 4;   // 16 KiB per tile (4096 u32)
-// 64 KiB per tile for long inputs (amortises the lookback): 16384 u32 / 8192 u64.
+// Long inputs use 32 / 16 rows (amortises the lookback): 32768 u32 (128 KiB) /
+// 8192 u64 (64 KiB) per tile.
 template <typename T>
 constexpr int scan_rows_large() { return sizeof(T) == 4 ? 32 : 16; }
 constexpr int kReduceRows = 16;

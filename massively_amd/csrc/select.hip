@@ -48,14 +48,14 @@ static msv_status select_launch(Ctx& c, const void* in, const void* st, const ms
   }
   const Pred<S> pred = make_pred<S>(p);
   const bool large = n >= kSelLargeMin;
-  const uint32_t tile = (uint32_t)kSelBlock * (large ? kSelRowsLarge : kSelRowsSmall) * 4;
+  const uint32_t tile = (uint32_t)kSelBlock * (large ? sel_rows_large<T>() : kSelRowsSmall) * 4;
   const uint32_t ntiles = div_up(n, tile);
   uint64_t *counter, *status;
   uint32_t gen;
   MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));
   const bool aligned = (((uintptr_t)in | (uintptr_t)st) & 15u) == 0;
   if (large)
-    hipLaunchKernelGGL((select_kernel<T, S, FORM, kSelRowsLarge>), dim3(ntiles), dim3(kSelBlock), 0,
+    hipLaunchKernelGGL((select_kernel<T, S, FORM, sel_rows_large<T>()>), dim3(ntiles), dim3(kSelBlock), 0,
                        c.stream(), (const T*)in, (const S*)st, pred, invert, n, aligned, (T*)out,
                        out_len, status, gen, ntiles, (const uint32_t*)nullptr);
   else
@@ -112,13 +112,13 @@ static msv_status partition_launch(Ctx& c, const void* in, uint32_t n, const msv
                      pred, aligned, d_total);
   MSV_HIP(hipGetLastError());
   const bool large = n >= kSelLargeMin;
-  const uint32_t tile = (uint32_t)kSelBlock * (large ? kSelRowsLarge : kSelRowsSmall) * 4;
+  const uint32_t tile = (uint32_t)kSelBlock * (large ? sel_rows_large<U>() : kSelRowsSmall) * 4;
   const uint32_t ntiles = div_up(n, tile);
   uint64_t *counter, *status;
   uint32_t gen;
   MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));
   if (large)
-    hipLaunchKernelGGL((select_kernel<U, S, 2, kSelRowsLarge, true>), dim3(ntiles), dim3(kSelBlock),
+    hipLaunchKernelGGL((select_kernel<U, S, 2, sel_rows_large<U>(), true>), dim3(ntiles), dim3(kSelBlock),
                        0, c.stream(), (const U*)in, (const S*)in, pred, false, n, aligned, (U*)out,
                        (uint32_t*)nullptr, status, gen, ntiles, (const uint32_t*)d_total);
   else

@@ -15,6 +15,10 @@ constexpr int kSelBlock = 512;
 // that fewer tiles take part in the look-back.
 constexpr int kSelRowsSmall = 2;
 constexpr int kSelRowsLarge = 4;
+// Long inputs of 4-byte elements: 8 rows (16384-element tiles, half as many
+// look-back steps; the staging LDS is 64 KiB). 8-byte elements keep 4.
+template <typename T>
+constexpr int sel_rows_large() { return sizeof(T) == 4 ? 8 : kSelRowsLarge; }
 constexpr uint32_t kSelLargeMin = 1u << 22;
 
 // FORM 0: u32 stencil array (flag = stencil != 0).
