@@ -3,8 +3,8 @@ inside the kernel that consumes them.
 
 A lazy input is a tree of the reference's constructors: ``constant``,
 ``counting``, ``stride`` (each with ``.take(len)``), ``map``, ``permute``,
-``reverse``, ``repeat_each``, ``tile``, ``adjacent_difference``, ``identity``
-and ``.slice``, whose leaves are device vectors or slices. The algorithms that
+``reverse``, ``repeat_each``, ``tile``, ``adjacent_difference``, ``identity``,
+``with_table`` (through ``map``) and ``.slice``, whose leaves are device vectors or slices. The algorithms that
 fuse it (``vector.reduce``, ``inclusive_scan``, ``exclusive_scan``,
 ``copy_where_if``, ``map``, ``adjacent_difference``) lower the tree to C++
 source giving element ``i`` (``msv_read``) over at most 13 device slots and 16
@@ -389,6 +389,47 @@ class _ZipMap(Lazy):
                             f"{reads} o[j] = ({_CTYPE[self.dtype]})({self.op.expr}); }}", ok=ok)
 
 
+class WithTable:
+    """``lazy::with_table(contexts, table)`` (lazy.rs:20-110, 720-747): each
+    row is a context element plus a view of the whole table. Consumed by
+    ``map``, whose expression reads ``x`` (the context element) and
+    ``table.at(k)`` / ``table.len()`` (``at`` reads 0 past the table's end)."""
+
+    def __init__(self, contexts, table):
+        if not isinstance(table, (DeviceVec, DeviceSlice)):
+            raise TypeError("the table of with_table is a device vector or slice")
+        self.contexts = _node(contexts)
+        self.table = _Slot(as_slice(table))
+
+
+def with_table(contexts, table) -> WithTable:
+    return WithTable(contexts, table)
+
+
+_TABLE_VIEW = ("template <typename D> struct TableView { const D* p; uint32_t n; "
+               "__device__ inline D at(uint64_t k) const { return k < n ? p[k] : (D)0; } "
+               "__device__ inline uint32_t len() const { return n; } };")
+
+
+class _WithTableMap(Lazy):
+    def __init__(self, wt: WithTable, op: UnaryOp):
+        self.wt, self.op = wt, op
+        self.dtype = op.output if op.output is not None else wt.contexts.dtype
+
+    def length(self) -> int:
+        return self.wt.contexts.length()
+
+    def _emit(self, g: _Gen) -> str:
+        c = g.emit(self.wt.contexts)
+        k = g.slot(self.wt.table.s)
+        if _TABLE_VIEW not in g.lines:
+            g.lines.insert(0, _TABLE_VIEW)
+        tc, tt, to = _CTYPE[self.wt.contexts.dtype], _CTYPE[self.wt.table.dtype], _CTYPE[self.dtype]
+        return g.fn(self, f"const {tc} x = {c}(la, i); "
+                          f"const TableView<{tt}> table{{(const {tt}*)la.p[{k}], la.n[{k}]}}; "
+                          f"return ({to})({self.op.expr});")
+
+
 class _Permute(Lazy):
     def __init__(self, values: Lazy, indices: Lazy):
         if indices.dtype != _U32:
@@ -484,6 +525,8 @@ def map(input, op: UnaryOp) -> Lazy:  # noqa: A001 - the reference's name
     x0, x1, ... and ``op.output`` names the result dtype."""
     if isinstance(input, (tuple, list)):
         return _ZipMap([_node(x) for x in input], op)
+    if isinstance(input, WithTable):
+        return _WithTableMap(input, op)
     return _Map(_node(input), op)
 
 

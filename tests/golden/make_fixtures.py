@@ -180,6 +180,8 @@ def literal_vectors() -> list[dict]:
     v.append(dict(name="lazy_constant_gather", src="massively/tests/vector/api/runtime.rs:77-87",
                   op="lazy", tree="constant_gather", value=7, take=1, indices_value=0,
                   indices_take=3, expected_gather=[7, 7, 7], expected_map=[7]))
+    v.append(dict(name="lazy_with_table_doc", src=f"{L}:724-744", op="lazy", tree="with_table",
+                  contexts=[2, 0, 1], table=[10, 20, 30], unary="table.at(x)", expected=[30, 10, 20]))
     T = "massively/tests/vector/api/lazy.rs"
     v.append(dict(name="lazy_reverse_sliced", src=f"{T}:295-299", op="lazy", tree="reverse_slices",
                   input=[10, 20, 30, 40, 50], slices=[[1, 4], [1, 2]], expected=[30]))

@@ -226,3 +226,18 @@ def test_lazy_op_errors(exec_):
             m.vector.reduce(exec_, lazy.map(v, UnaryOp("x")), 0, m.op.Add)
     finally:
         other.close()
+
+
+def test_with_table(exec_):
+    g = golden("lazy_with_table_doc")
+    t = lazy.map(lazy.with_table(_u32(exec_, g["contexts"]), _u32(exec_, g["table"])),
+                 UnaryOp(g["unary"], np.uint32))
+    assert _host(exec_, m.vector.map(exec_, t, UnaryOp("x"))).tolist() == g["expected"]
+    n, size = 200003, 5000
+    ctx = RNG.integers(0, size + 100, n).astype(np.uint32)  # some past the table's end
+    table = RNG.standard_normal(size)
+    t = lazy.map(lazy.with_table(exec_.to_device(ctx), exec_.to_device(table)),
+                 UnaryOp("table.at(x) + (double)table.len()", np.float64))
+    want = np.where(ctx < size, table[np.minimum(ctx, size - 1)], 0.0) + size
+    assert _host(exec_, m.vector.map(exec_, t, UnaryOp("x"))).tobytes() == want.tobytes()
+    assert float(m.vector.reduce(exec_, t, 0.0, m.op.Max)) == want.max()

@@ -138,3 +138,13 @@ def test_user_ops_on_device_vectors_compile(dtype):
     hiprtc, whose standard headers differ from the library build's."""
     lazy.check_compiles("reduction", dtype, None, UserReductionOp("a > b ? a : b", "0"))
     lazy.check_compiles("predicate", dtype, None, UserPredicate("x > 1"))
+
+
+def test_with_table_lowering():
+    t = lazy.map(lazy.with_table(_slot(np.uint32, 7), _slot(np.float64, 3)),
+                 UnaryOp("table.at(x) * 2.0", np.float64))
+    src, args = t.lower()
+    assert len(t) == 7 and args.nslots == 2 and "TableView<double>" in src
+    lazy.check_compiles("map", np.float64, src)
+    with pytest.raises(TypeError):
+        lazy.with_table(_slot(np.uint32, 3), lazy.counting(0).take(3))  # the table is device data
