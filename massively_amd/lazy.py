@@ -530,29 +530,49 @@ def map(input, op: UnaryOp) -> Lazy:  # noqa: A001 - the reference's name
     return _Map(_node(input), op)
 
 
-def identity(input) -> Lazy:
+def _rows(values, f):
+    """An adaptor of zipped rows (a tuple or list of equally long columns)
+    is the tuple of the adapted columns; a single input is adapted as is."""
+    if isinstance(values, (tuple, list)):
+        cols = [_node(v) for v in values]
+        lens = {c.length() for c in cols}
+        if len(lens) > 1:
+            a, b = sorted(lens)[:2]
+            raise LengthMismatch(a, b)
+        return tuple(f(c) for c in cols)
+    return f(_node(values))
+
+
+def identity(input):
     """``lazy::identity(input)`` (``map`` with ``op::Identity``)."""
-    return _Map(_node(input), UnaryOp("x"))
+    return _rows(input, lambda c: _Map(c, UnaryOp("x")))
 
 
-def permute(values, indices) -> Lazy:
-    """``lazy::permute(values, indices)``: values[indices[i]]."""
-    return _Permute(_node(values), _node(indices))
+def permute(values, indices):
+    """``lazy::permute(values, indices)``: values[indices[i]] (zipped values:
+    every column through the same indices)."""
+    idx = _node(indices)
+    return _rows(values, lambda c: _Permute(c, idx))
 
 
-def reverse(values) -> Lazy:
+def reverse(values):
     """``lazy::reverse(values)``: values[len - 1 - i]."""
-    return _Reverse(_node(values))
+    return _rows(values, _Reverse)
 
 
-def repeat_each(values, repeats: int) -> Lazy:
+def repeat_each(values, repeats: int):
     """``lazy::repeat_each(values, r)``: each element r times in a row."""
-    return _RepeatEach(_node(values), repeats)
+    return _rows(values, lambda c: _RepeatEach(c, repeats))
 
 
-def tile(values, repeats: int) -> Lazy:
+def tile(values, repeats: int):
     """``lazy::tile(values, r)``: the whole input r times."""
-    return _Tile(_node(values), repeats)
+    return _rows(values, lambda c: _Tile(c, repeats))
+
+
+def slice_rows(values, start: int = 0, end: int | None = None):
+    """``MIter::slice`` of a lazy input or of zipped lazy rows."""
+    return _rows(values, lambda c: c.slice(start, end))
 
 
 def adjacent_difference(input, op) -> Lazy:
@@ -604,6 +624,8 @@ def check_compiles(kind: str, dtype, src: str | None, op=None) -> None:
 
 
 def map_into_vec(exec: Executor, lazy: Lazy) -> DeviceVec:
+    if isinstance(lazy, tuple):  # zipped rows: one vector per column (into_columns)
+        return tuple(map_into_vec(exec, c) for c in lazy)
     src, args = lazy.lower()
     h = C.c_void_p()
     check(L.lib().msv_map_lazy(exec.handle, C.byref(args),

@@ -434,7 +434,10 @@ def unique_by_key(exec: Executor, keys, values) -> DeviceVec:
 
 def map(exec: Executor, input, op: UnaryOp) -> DeviceVec:  # noqa: A001 - the reference's name
     """massively/src/api/algorithm/transform.rs:91-108: op(x) for each element,
-    computed from the (possibly lazy) input in one kernel."""
+    computed from the (possibly lazy) input in one kernel. Zipped rows with
+    ``op::Identity`` give one vector per column (``MStorage::into_columns``)."""
+    if isinstance(input, tuple) and op.expr.strip() == "x" and op.output is None:
+        return tuple(lazy.map_into_vec(exec, lazy.map(c, op)) for c in input)
     return lazy.map_into_vec(exec, lazy.map(input, op))
 
 

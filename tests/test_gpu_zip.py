@@ -143,3 +143,24 @@ def test_zipped_length_mismatch(exec_):
         m.vector.gather(exec_, (a, b), exec_.to_device(np.zeros(2, np.uint32)))
     with pytest.raises(m.LengthMismatch):
         m.vector.copy_where(exec_, (a, b), a)
+
+
+def test_lazy_adaptors_over_zipped_rows(exec_):
+    """tests/vector/api/lazy.rs:301-341: reverse / repeat_each / tile of zip2,
+    sliced, written out column by column."""
+    first = exec_.to_device(np.array([1, 2, 3], np.uint32))
+    second = exec_.to_device(np.array([10, 20, 30], np.uint32))
+    a, b = m.vector.map(exec_, lazy.reverse((first, second)), UnaryOp("x"))
+    assert exec_.to_host(a).tolist() == [3, 2, 1] and exec_.to_host(b).tolist() == [30, 20, 10]
+    for name, ctor in (("lazy_repeat_each_sliced", lazy.repeat_each), ("lazy_tile_sliced", lazy.tile)):
+        g = golden(name)
+        rows = ctor((exec_.to_device(np.array(g["input"], np.uint32)),
+                     exec_.to_device(np.array(g["input_second"], np.uint32))), g["repeats"])
+        for lo, hi in g["slices"]:
+            rows = lazy.slice_rows(rows, lo, hi)
+        a, b = m.vector.map(exec_, rows, UnaryOp("x"))
+        assert exec_.to_host(a).tolist() == g["expected"], name
+        assert exec_.to_host(b).tolist() == g["expected_second"], name
+    idx = exec_.to_device(np.array([2, 0, 7], np.uint32))
+    a, b = m.vector.map(exec_, lazy.permute((first, second), idx), UnaryOp("x"))
+    assert exec_.to_host(a).tolist() == [3, 1, 0] and exec_.to_host(b).tolist() == [30, 10, 0]

@@ -148,3 +148,13 @@ def test_with_table_lowering():
     lazy.check_compiles("map", np.float64, src)
     with pytest.raises(TypeError):
         lazy.with_table(_slot(np.uint32, 3), lazy.counting(0).take(3))  # the table is device data
+
+
+def test_adaptors_over_zipped_rows():
+    a, b = _slot(np.uint32, 3), _slot(np.float32, 3)
+    r = lazy.reverse((a, b))
+    assert isinstance(r, tuple) and [len(c) for c in r] == [3, 3]
+    assert [c.dtype for c in lazy.tile((a, b), 2)] == [np.dtype(np.uint32), np.dtype(np.float32)]
+    assert [len(c) for c in lazy.slice_rows(lazy.repeat_each((a, b), 3), 2, 8)] == [6, 6]
+    with pytest.raises(m.LengthMismatch):
+        lazy.reverse((a, _slot(np.uint32, 4)))
