@@ -124,7 +124,7 @@ msv_status msv_executor_sync(msv_executor* ex);
 uint64_t msv_executor_id(const msv_executor* ex);
 /* The executor's HIP stream (hipStream_t), for callers that interleave their own work. */
 void* msv_executor_stream(msv_executor* ex);
-/* Returns cached device memory to the driver (synchronises). */
+/* Returns cached device memory to the HIP runtime (synchronises). */
 msv_status msv_executor_trim(msv_executor* ex);
 
 /* Tuning options of the batching layer. No counterpart in the reference
