@@ -299,15 +299,18 @@ __device__ inline void slot_read(const void* base, uint32_t n, size_t i, D (&o)[
 }
 
 // The same without checks: the caller knows the K elements are in range and
-// p + i is aligned to 16 bytes (16-byte groups) or to the element.
+// p + i is aligned to 16 bytes (16-byte groups) or to the element. The
+// 16-byte loads are non-temporal: a lazy slot is streamed once per pass.
 template <int K, typename D>
 __device__ inline void slot_read_fast(const void* base, size_t i, D (&o)[K]) {
+  typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
   const D* p = (const D*)base + i;
   constexpr size_t B = K * sizeof(D);
   if constexpr (B % 16 == 0) {
 #pragma unroll
     for (int g = 0; g < (int)(B / 16); ++g) {
-      const uint4 q = reinterpret_cast<const uint4*>(p)[g];
+      const u4_t v = __builtin_nontemporal_load(reinterpret_cast<const u4_t*>(p) + g);
+      const uint4 q = make_uint4(v.x, v.y, v.z, v.w);
       const uint32_t w[4] = {q.x, q.y, q.z, q.w};
       constexpr int E = 16 / sizeof(D);
 #pragma unroll
