@@ -249,7 +249,7 @@ __global__ __launch_bounds__(BLOCK) void reduce_tile_kernel(const T* __restrict_
   const size_t base = (size_t)blockIdx.x * TILE;
   const bool vec_ok = aligned && (base + TILE <= n) && src_tile_ok(in, base, base + TILE);
   T x[ROWS][V];
-  load_tile<T, BLOCK, ROWS>(in, base, n, vec_ok, x, C::identity());
+  load_tile<T, BLOCK, ROWS, true>(in, base, n, vec_ok, x, C::identity());  // streamed once
   T flat[ROWS * V];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r)
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(BLOCK) void reduce_tile_ordered_kernel(Src in,
   const size_t base = (size_t)blockIdx.x * TILE;
   const bool vec_ok = aligned && (base + TILE <= n) && src_tile_ok(in, base, base + TILE);
   T x[ROWS][V];
-  load_tile<T, BLOCK, ROWS>(in, base, n, vec_ok, x, C::identity());
+  load_tile<T, BLOCK, ROWS, true>(in, base, n, vec_ok, x, C::identity());  // streamed once
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
     T v = x[r][0];
