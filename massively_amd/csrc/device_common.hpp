@@ -202,18 +202,26 @@ __device__ inline T shift_down1(T v, T fill) {
   return lane == 63 ? fill : out;
 }
 
+// A non-temporal 16-byte load (input streamed once).
+__device__ inline uint4 load16_nt(const void* p) {
+  typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
+  const u4_t v = __builtin_nontemporal_load(reinterpret_cast<const u4_t*>(p));
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // Four consecutive elements from p + i: one or two 16-byte loads when `full`
-// (the caller has checked alignment and range), else guarded scalar loads
-// (0 past n).
-template <typename T>
+// (the caller has checked alignment and range), else guarded scalar loads (0
+// past n). NT: non-temporal loads, for a kernel that streams one input once
+// (with two interleaved streams, input and stencil, they measured slower).
+template <typename T, bool NT = false>
 __device__ inline void load4(const T* __restrict__ p, size_t i, uint32_t n, bool full, T (&v)[4]) {
   if (full) {
     if constexpr (sizeof(T) == 4) {
-      const uint4 q = *reinterpret_cast<const uint4*>(p + i);
+      const uint4 q = NT ? load16_nt(p + i) : *reinterpret_cast<const uint4*>(p + i);
       v[0] = bit_cast<T>(q.x); v[1] = bit_cast<T>(q.y); v[2] = bit_cast<T>(q.z); v[3] = bit_cast<T>(q.w);
     } else {
-      const uint4 q0 = *reinterpret_cast<const uint4*>(p + i);
-      const uint4 q1 = *reinterpret_cast<const uint4*>(p + i + 2);
+      const uint4 q0 = NT ? load16_nt(p + i) : *reinterpret_cast<const uint4*>(p + i);
+      const uint4 q1 = NT ? load16_nt(p + i + 2) : *reinterpret_cast<const uint4*>(p + i + 2);
       v[0] = bit_cast<T>(((uint64_t)q0.y << 32) | q0.x);
       v[1] = bit_cast<T>(((uint64_t)q0.w << 32) | q0.z);
       v[2] = bit_cast<T>(((uint64_t)q1.y << 32) | q1.x);
@@ -253,7 +261,7 @@ __device__ inline auto src_at(const Src& s, size_t i) {
 // A lazy source also evaluates K consecutive elements at once (s.load<K>),
 // which turns its slot reads into vector loads where the range allows; used
 // when all four elements are in range (`full`).
-template <typename T, typename Src, typename = std::enable_if_t<is_lazy_src<Src>>>
+template <typename T, bool NT = false, typename Src, typename = std::enable_if_t<is_lazy_src<Src>>>
 __device__ inline void load4(const Src& s, size_t i, uint32_t n, bool full, T (&v)[4]) {
   if (full) {  // the caller has checked src_tile_ok over the tile
     s.template load_fast<4>(i, v);

@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kCmpBlock) void seg_compact_kernel(
   for (int r = 0; r < kCmpRows; ++r) {
     const uint32_t li = ((uint32_t)r * kCmpBlock + tid) * 4;
     const size_t i = base + li;
-    load4<T>(in, i, n, full, x[r]);
+    load4<T, true>(in, i, n, full, x[r]);
     uint32_t b = 0;
     if constexpr (FORM == CMP_FILTER) {
 #pragma unroll

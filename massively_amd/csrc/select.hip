@@ -28,7 +28,7 @@ __global__ __launch_bounds__(256) void pred_count_kernel(const S* __restrict__ i
   for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups;
        g += (size_t)gridDim.x * blockDim.x) {
     S v[4];
-    load4<S>(in, g * 4, n, aligned && g * 4 + 4 <= n, v);
+    load4<S, true>(in, g * 4, n, aligned && g * 4 + 4 <= n, v);
 #pragma unroll
     for (int k = 0; k < 4; ++k) c += (g * 4 + k < n && pred(v[k])) ? 1u : 0u;
   }

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(Src in,
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
     const size_t i = base + ((size_t)r * kSelBlock + tid) * 4;
-    load4<T>(in, i, n, full, x[r]);
+    load4<T, FORM == 2>(in, i, n, full, x[r]);  // non-temporal when the input is the only stream
     if constexpr (FORM == 0) {
       S s[4];
       load4<S>(stencil, i, n, full, s);
