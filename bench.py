@@ -95,12 +95,26 @@ def w_scan(c, log2n=28):
 
 
 def w_reduce(c):
+    """f32 sum of 2^28 N(0,1) values (Box-Muller in f64 over the seed-3 u64
+    stream, rounded to f32), checked against the FP64 sum and its bound
+    (ceil(log2 n) + 2) * 2^-24 * sum|x| outside the timed region."""
+    np = c.np
     n = 1 << 28
-    x = c.ex.alloc(c.np.uint32, n)
-    c.mr.uniform_u32(c.ex, 0x3F000000, 0x3F800000, 3, x)  # f32 bit patterns in [0.5, 1)
-    xf = c.ex.wrap(x.device_ptr, c.np.float32, n)
-    ms = timed(c.ex, lambda: c.m.vector.reduce(c.ex, xf, 0.0, c.m.op.Add), c.steps, c.warmup)
-    return summarize("reduce_f32_add_2^28", ms, 4.0 * n, n, note="includes the scalar device-to-host copy")
+    hx = c.mr.normal_f32_host(c.ex, 3, n)
+    xf = c.ex.to_device(hx)
+    res = []
+    ms = timed(c.ex, lambda: res.append(c.m.vector.reduce(c.ex, xf, 0.0, c.m.op.Add)), c.steps, c.warmup)
+    s64 = a64 = 0.0
+    for lo in range(0, n, 1 << 24):
+        part = hx[lo:lo + (1 << 24)].astype(np.float64)
+        s64 += float(part.sum())
+        a64 += float(np.abs(part).sum())
+    bound = (math.ceil(math.log2(n)) + 2) * 2.0**-24 * a64
+    repeatable = len({np.float32(r).tobytes() for r in res}) == 1
+    return summarize("reduce_f32_add_normal_2^28", ms, 4.0 * n, n,
+                     note="includes the scalar device-to-host copy",
+                     abs_err_vs_fp64=abs(float(res[-1]) - s64), fp64_bound=bound,
+                     within_bound=abs(float(res[-1]) - s64) <= bound, bitwise_repeatable=repeatable)
 
 
 def w_copy_where(c):
@@ -161,6 +175,10 @@ def w_sort(c):
     vals = c.ex.alloc(np.uint32, n)
     c.mr.iota_u32(c.ex, vals)
     out["u64_uniform_2^27"] = _sort_case(c, "sort_by_key_u64_uniform_2^27", keys, vals, n, 8, 4, 8)
+    ref = c.ex.to_host(m.vector.radix_sort_by_key(c.ex, keys, vals))
+    out["rocprim_u64_uniform_2^27"] = rocprim_sort_case(c, "sort_pairs_u64_u32_2^27", keys, vals, n, 8,
+                                                        check=ref)
+    del ref
     hk = c.ex.to_host(keys)
     rng = np.random.default_rng(6)
     # Zipf s=1.1 over 2^20 distinct values: inverse-CDF sampling of ranks.
@@ -182,6 +200,7 @@ def w_sort(c):
     v28 = c.ex.alloc(np.uint32, n28)
     c.mr.iota_u32(c.ex, v28)
     out["u32_2^28"] = _sort_case(c, "sort_by_key_u32_2^28", k28, v28, n28, 4, 4, 4)
+    out["rocprim_u32_2^28"] = rocprim_sort_case(c, "sort_pairs_u32_u32_2^28", k28, v28, n28, 4)
     out["reference_schedule_proxy_u32_2^28"] = refproxy_case(c, k28, v28, n28, reps=2)
     del k28, v28
     n20 = 1 << 20
@@ -197,13 +216,50 @@ def w_sort(c):
     return out
 
 
+def rocprim_lib():
+    """The vendor yardstick library (bench/vendor/), or None if not built."""
+    import ctypes
+
+    path = ROOT / "bench" / "vendor" / "librocprim_yardstick.so"
+    if not path.exists():
+        return None
+    lib = ctypes.CDLL(str(path))
+    P, U32, I = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
+    for name in ("rocprim_sort_pairs_u32_u32_ms", "rocprim_sort_pairs_u64_u32_ms"):
+        getattr(lib, name).restype = ctypes.c_double
+        getattr(lib, name).argtypes = [P, P, U32, P, P, I]
+    lib.rocprim_segmented_sort_pairs_u32_ms.restype = ctypes.c_double
+    lib.rocprim_segmented_sort_pairs_u32_ms.argtypes = [P, P, U32, P, U32, P, P, I]
+    lib.rocprim_inclusive_scan_u32_ms.restype = ctypes.c_double
+    lib.rocprim_inclusive_scan_u32_ms.argtypes = [P, U32, P, I]
+    return lib
+
+
+def rocprim_sort_case(c, name, keys, vals, n, key_bytes, reps=5, check=None):
+    """rocPRIM radix_sort_pairs on the same device-resident keys and values
+    (a labelled vendor yardstick; never part of the library)."""
+    lib = rocprim_lib()
+    if lib is None:
+        return None
+    ok, ov = c.ex.alloc(keys.dtype, n), c.ex.alloc(c.np.uint32, n)
+    c.ex.sync()
+    f = lib.rocprim_sort_pairs_u32_u32_ms if key_bytes == 4 else lib.rocprim_sort_pairs_u64_u32_ms
+    ms = f(keys.device_ptr, vals.device_ptr, n, ok.device_ptr, ov.device_ptr, reps)
+    if ms < 0:
+        return None
+    d = summarize(f"rocprim_{name}", [ms], None, n, label="vendor yardstick (rocPRIM radix_sort_pairs)")
+    if check is not None:
+        d["same_values_as_msv"] = bool(c.ex.to_host(ov).tobytes() == check.tobytes())
+    return d
+
+
 def refproxy_case(c, keys, vals, n, reps, check=None):
     """The reference's radix schedule (4-bit LSD, 4 launches per pass, key
     gathers through the permutation, final gathers), timed on the same GPU
-    as a stand-in for its CubeCL-HIP build; see massively_amd/proxy/."""
+    as a stand-in for its CubeCL-HIP build; see bench/proxy/."""
     import ctypes
 
-    path = ROOT / "massively_amd" / "proxy" / "libmsv_refproxy.so"
+    path = ROOT / "bench" / "proxy" / "libmsv_refproxy.so"
     if not path.exists():
         return None
     lib = ctypes.CDLL(str(path))
@@ -224,8 +280,6 @@ def refproxy_case(c, keys, vals, n, reps, check=None):
 
 def w_merge_sets(c):
     np, m = c.np, c.m
-    import oracle
-
     n = 1 << 27
     a = c.ex.alloc(np.uint32, n)
     c.mr.uniform_u32(c.ex, 0, (1 << 26) - 1, 11, a)
@@ -235,10 +289,13 @@ def w_merge_sets(c):
     out = {}
     ms = timed(c.ex, lambda: m.vector.merge(c.ex, sa, sb, m.op.Less), c.steps, c.warmup)
     out["merge"] = summarize("merge_u32_2x2^27", ms, 8.0 * 2 * n, 2 * n)
-    # 25% overlap: sorted pcg images of [0, n) and [0.75n, 1.75n).
-    ha = np.sort(oracle.uniform_u32(0, 2**32 - 1, 0, n))
-    hb = np.sort(oracle.uniform_u32(0, 2**32 - 1, 0, n, start=3 * n // 4))
-    A, B = c.ex.to_device(ha), c.ex.to_device(hb)
+    # 25% overlap: sorted pcg images of [0, n) and [0.75n, 1.75n) (one
+    # stream of 1.75n values of seed 0; pcg_hash32 is a bijection).
+    stream = c.ex.alloc(np.uint32, 7 * n // 4)
+    c.mr.uniform_u32(c.ex, 0, 2**32 - 1, 0, stream)
+    A = m.vector.sort(c.ex, stream.slice(0, n), m.op.Less)
+    B = m.vector.sort(c.ex, stream.slice(3 * n // 4, 7 * n // 4), m.op.Less)
+    del stream
     for kind, fn, outlen in (("intersection", m.vector.set_intersection, n // 4),
                              ("difference", m.vector.set_difference, 3 * n // 4),
                              ("union", m.vector.set_union, 7 * n // 4)):
@@ -295,6 +352,19 @@ def w_seg(c):
         log(f"[bench]   tuned {d['tuned_config']}, candidates (block x ipt x digit bits): {cands}")
         out["sort_u32_f32" + tag] = d
     c.ex.set_option(c.ex.OPT_SEG_LAUNCH, c.ex.SEG_LAUNCH_PERSISTENT)
+    lib = rocprim_lib()
+    if lib is not None:
+        # Vendor yardstick: rocPRIM segmented_radix_sort_pairs on the same
+        # segments (u32 keys, the f32 values moved as 4-byte payloads).
+        doff = c.ex.to_device(off)
+        ok, ov = c.ex.alloc(np.uint32, n), c.ex.alloc(np.uint32, n)
+        c.ex.sync()
+        ms = lib.rocprim_segmented_sort_pairs_u32_ms(x.device_ptr, f.device_ptr, n, doff.device_ptr, S,
+                                                     ok.device_ptr, ov.device_ptr, 3)
+        if ms > 0:
+            out["rocprim_seg_sort_u32"] = summarize("rocprim_segmented_sort_pairs_u32_2^26_in_2^20", [ms],
+                                                    16.0 * n + 4.0 * (S + 1), n,
+                                                    label="vendor yardstick (rocPRIM segmented_radix_sort_pairs)")
     return out
 
 
@@ -544,6 +614,28 @@ def w_served(c):
     return out
 
 
+def m_np():
+    import numpy
+
+    return numpy
+
+
+def host_cpu():
+    """The host's CPU model and logical CPU counts (all, and usable by this process)."""
+    import os
+
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return dict(cpu_model=model, nproc=os.cpu_count(), nproc_usable=usable)
+
+
 def cpu_baseline(n_sample: int, reps: int):
     import oracle
 
@@ -557,21 +649,28 @@ def cpu_baseline(n_sample: int, reps: int):
     sec = statistics.median(t)
     return dict(value=round(8.0 * n_sample / sec / 1e9, 3), unit="GB/s", cores=1, kind="port",
                 sample=f"inclusive_scan add over {n_sample} u32 (seed 2), median of {reps} "
-                       f"single-thread runs of the C oracle (restatement of the reference CPU path)")
+                       f"single-thread runs of the C oracle (restatement of the reference CPU path; "
+                       f"{sum(t):.1f} s of CPU work in all)",
+                runs=reps, min_s=round(min(t), 4), median_s=round(sec, 4), **host_cpu())
 
 
-def cpu_sort_baseline(n_sample: int):
+def cpu_sort_baseline(n_sample: int, reps: int):
     import numpy as np
 
     import oracle
 
     keys = oracle.uniform_u64(0, 2**64 - 1, 6, n_sample)
     vals = np.arange(n_sample, dtype=np.uint32)
-    t0 = time.perf_counter()
-    oracle.radix_sort_by_key(keys, vals)
-    sec = time.perf_counter() - t0
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.radix_sort_by_key(keys, vals)
+        t.append(time.perf_counter() - t0)
+    sec = statistics.median(t)
     return dict(value=round(n_sample / sec / 1e9, 5), unit="Gkeys/s", cores=1, kind="port",
-                sample=f"stable sort_by_key of {n_sample} u64 keys (seed 6) + u32 payload, one run")
+                sample=f"stable sort_by_key of {n_sample} u64 keys (seed 6) + u32 payload, median "
+                       f"of {reps} single-thread runs of the C oracle",
+                runs=reps, **host_cpu())
 
 
 def w_copy(c):
@@ -647,6 +746,18 @@ def main() -> int:
     log(f"[bench] scan n=2^28 steps={a.steps} warmup={a.warmup}")
     scan = w_scan(c)
     extra = {}
+    lib = rocprim_lib()
+    if lib is not None and (not only or "vendor" in only):
+        x = ex.alloc(m_np().uint32, n)
+        c.mr.uniform_u32(ex, 0, 1023, 2, x)
+        y = ex.alloc(m_np().uint32, n)
+        ex.sync()
+        ms = lib.rocprim_inclusive_scan_u32_ms(x.device_ptr, n, y.device_ptr, 10)
+        if ms > 0:
+            extra["rocprim_inclusive_scan_u32_2^28"] = summarize(
+                "rocprim_inclusive_scan_u32_add_2^28", [ms], 8.0 * n, n,
+                label="vendor yardstick (rocPRIM inclusive_scan)")
+        del x, y
     for name, fn in WORKLOADS.items():
         if (only and name not in only) or (not only and name == "copy"):
             continue
@@ -657,10 +768,10 @@ def main() -> int:
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu baseline (C oracle, 1 thread)")
-        cpu = cpu_baseline(n, 3)
+        cpu = cpu_baseline(n, 15)
         log(f"[bench] cpu scan: {cpu['value']:.2f} GB/s")
         if not only or "sort" in only:
-            extra["cpu_sort_baseline"] = cpu_sort_baseline(1 << 23)
+            extra["cpu_sort_baseline"] = cpu_sort_baseline(1 << 23, 5)
             log(f"[bench] cpu sort: {extra['cpu_sort_baseline']['value']:.4f} Gkeys/s")
 
     sort_g = extra.get("sort", {}).get("u32_2^28", {}).get("gitems_per_s")

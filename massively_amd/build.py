@@ -93,26 +93,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(sources)} sources", file=sys.stderr)
-    build_proxy(verbose)
     return LIB
-
-
-PROXY_SRC = PKG / "proxy" / "reference_schedule.hip"
-PROXY_LIB = PKG / "proxy" / "libmsv_refproxy.so"
-
-
-def build_proxy(verbose: bool = True) -> Path:
-    """The timing proxy of the reference's radix schedule (bench only; not
-    linked into libmsv)."""
-    tmp = PROXY_LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, *COMMON, "-shared", str(PROXY_SRC), "-o", str(tmp)]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {PROXY_SRC.name}:\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, PROXY_LIB)
-    if verbose:
-        print(f"built {PROXY_LIB}", file=sys.stderr)
-    return PROXY_LIB
 
 
 if __name__ == "__main__":

@@ -416,7 +416,7 @@ msv_status run_jobs(Ctx& c, const msv_job* jobs, uint32_t count) {
     MSV_HIP(hipMemcpyAsync(d_items, src, sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice,
                            c.stream()));
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, c.stream(), queues, 8u, 0u);
-    static int per_cu[7] = {0};
+    int* per_cu = c.ex()->batch_per_cu;
     MSV_TRY(launch_family(c, batch_stream_kernel, d_items + start[0], start[1] - start[0],
                           queues + 0, &per_cu[0]));
     MSV_TRY(launch_family(c, batch_sort_kernel<1, 1>, d_items + start[1], start[2] - start[1], queues + 1, &per_cu[1]));

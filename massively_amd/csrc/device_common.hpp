@@ -429,15 +429,6 @@ constexpr uint32_t kFlagInc = 2;
 constexpr uint32_t kGenMask = (1u << 29) - 1;
 constexpr uint32_t kSelfHelpPolls = 1024;
 
-// Tile ids drawn from an atomic counter (kernels whose fallback would need a
-// whole block keep this form). The block that draws the last id (ntiles-1)
-// has seen every other block draw first, so it rewinds the counter to zero.
-__device__ inline uint32_t draw_tile(uint64_t* counter, uint32_t ntiles) {
-  const uint32_t tile = (uint32_t)add_agent(counter, 1ull);
-  if (tile == ntiles - 1) add_agent(counter, (uint64_t)0 - (uint64_t)ntiles);
-  return tile;
-}
-
 __device__ inline uint64_t pack_status(uint32_t gen, uint32_t flag, uint32_t head,
                                        uint32_t payload) {
   return ((uint64_t)((gen << 3) | (head << 2) | flag) << 32) | payload;
@@ -481,60 +472,7 @@ struct TileStatus {
       return (fl != 0 && fl == fh) ? fl : 0u;
     }
   }
-  // Spins until the status of `tile` is valid; returns flag and payload.
-  __device__ inline uint32_t wait(uint32_t tile, uint32_t& head, T& v) const {
-    if constexpr (kWords == 1) {
-      uint64_t w;
-      uint32_t f;
-      while ((f = status_flag(w = ld_agent(words + tile), gen)) == 0) __builtin_amdgcn_s_sleep(1);
-      head = (uint32_t)(w >> 34) & 1u;
-      v = bit_cast<T>((uint32_t)w);
-      return f;
-    } else {
-      uint64_t lo, hi;
-      uint32_t fl, fh;
-      for (;;) {
-        lo = ld_agent(words + 2 * (size_t)tile);
-        hi = ld_agent(words + 2 * (size_t)tile + 1);
-        fl = status_flag(lo, gen);
-        fh = status_flag(hi, gen);
-        if (fl != 0 && fl == fh) break;
-        __builtin_amdgcn_s_sleep(1);
-      }
-      head = (uint32_t)(lo >> 34) & 1u;
-      v = bit_cast<T>(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
-      return fl;
-    }
-  }
 };
-
-// Wave-cooperative lookback over predecessors of `tile` (called by one whole
-// wave). Returns the exclusive prefix of the tile. Lane k inspects tile
-// (base - k); the nearest inclusive-prefix tile ends the walk.
-template <typename T, int OP>
-__device__ inline T wave_lookback(const TileStatus<T>& st, uint32_t tile) {
-  using C = Combine<T, OP>;
-  const uint32_t lane = __lane_id();
-  T prefix = C::identity();
-  int64_t base = (int64_t)tile - 1;
-  for (;;) {
-    const int64_t p = base - (int64_t)lane;
-    T v = C::identity();
-    uint32_t flag = kFlagInc, head = 0;
-    if (p >= 0) flag = st.wait((uint32_t)p, head, v);
-    const uint64_t inc_mask = ballot(flag == kFlagInc);
-    if (inc_mask) {
-      const uint32_t first = (uint32_t)__builtin_ctzll(inc_mask);
-      const T contrib = lane <= first ? v : C::identity();
-      // Lanes hold tiles in reverse order, but all supported operators are
-      // commutative, so the order of combination inside the window is free.
-      prefix = C::apply(wave_reduce<T, OP>(contrib), prefix);
-      return prefix;
-    }
-    prefix = C::apply(wave_reduce<T, OP>(v), prefix);
-    base -= kWave;
-  }
-}
 
 // Lookback with self-help: `agg_of(q)` is called by the whole wave with a
 // uniform tile index q and returns that tile's aggregate (uniform). It is used
@@ -578,47 +516,13 @@ __device__ inline T wave_lookback_sh(const TileStatus<T>& st, uint32_t tile, Agg
   }
 }
 
-// Segmented variant: walk stops at the nearest tile that is inclusive or that
-// contains a segment head (its aggregate then already is the carry).
-template <typename T, int OP>
-__device__ inline void wave_lookback_seg(const TileStatus<T>& st, uint32_t tile, uint32_t& head_out,
-                                         T& carry) {
-  using C = Combine<T, OP>;
-  const uint32_t lane = __lane_id();
-  T prefix = C::identity();
-  uint32_t any_head = 0;
-  int64_t base = (int64_t)tile - 1;
-  for (;;) {
-    const int64_t p = base - (int64_t)lane;
-    T v = C::identity();
-    uint32_t flag = kFlagInc, head = 0;
-    if (p >= 0) flag = st.wait((uint32_t)p, head, v);
-    const uint64_t stop_mask = ballot(flag == kFlagInc || head != 0);
-    const uint64_t head_mask = ballot(head != 0);
-    if (stop_mask) {
-      const uint32_t first = (uint32_t)__builtin_ctzll(stop_mask);
-      // Tiles nearer than `first` have no head; combine them with the stop
-      // tile. Segmented combination is not commutative, so fold in order from
-      // the farthest (lane `first`) to the nearest (lane 0).
-      T acc = C::identity();
-      for (int k = (int)first; k >= 0; --k) acc = C::apply(acc, readlane(v, k));
-      any_head |= (head_mask & ((first == 63) ? ~0ull : ((2ull << first) - 1))) ? 1u : 0u;
-      prefix = C::apply(acc, prefix);
-      break;
-    }
-    T acc = C::identity();
-    for (int k = kWave - 1; k >= 0; --k) acc = C::apply(acc, readlane(v, k));
-    prefix = C::apply(acc, prefix);
-    base -= kWave;
-  }
-  head_out = any_head;
-  carry = prefix;
-}
-
-// Segmented lookback with self-help: as wave_lookback_seg, but a predecessor
-// whose status stays invalid for kSelfHelpPolls polls is computed by the
-// whole wave with `agg_of(q, head, value)` (uniform q), as in
-// wave_lookback_sh, so no tile waits on a tile that is not resident.
+// Segmented lookback with self-help: the walk stops at the nearest tile that
+// is inclusive or that contains a segment head (its aggregate then already is
+// the carry). A predecessor whose status stays invalid for kSelfHelpPolls
+// polls is computed by the whole wave with `agg_of(q, head, value)` (uniform
+// q), as in wave_lookback_sh, so no tile waits on a tile that is not resident.
+// Segmented combination is not commutative: each window is folded in order
+// from the farthest tile to the nearest.
 template <typename T, int OP, typename AggFn>
 __device__ inline void wave_lookback_seg_sh(const TileStatus<T>& st, uint32_t tile,
                                             uint32_t& head_out, T& carry, AggFn agg_of) {

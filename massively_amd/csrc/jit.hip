@@ -547,8 +547,8 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
       src += "); }\n};\n}  // namespace msv\n";
     }
     src += fmt("extern \"C\" __global__ void msv_user_identity(msv::UT* o) { *o = msv::Combine<msv::UT, %d>::identity(); }\n", code);
-    exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d>", T, code, kReduceBlock, kReduceRows));
-    exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d>", T, code, kScanBlock, kScanRowsSmall));
+    exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, true>", T, code, kReduceBlock, kReduceRows));
+    exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, false>", T, code, kScanBlock, kScanRowsSmall));
     exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, true, %d, %d, %d>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
     exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, false, %d, %d, %d>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
     if (lazy && builtin && !dtype_is_float(dtype)) {
@@ -559,8 +559,8 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
                               excl ? "true" : "false", (int)SCAN_LOOKBACK, kScanBlock, rows));
     }
     if (lazy) {
-      exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, msv::LazySrc>", T, code, kReduceBlock, kReduceRows));
-      exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, msv::LazySrc>", T, code, kScanBlock, kScanRowsSmall));
+      exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, true, msv::LazySrc>", T, code, kReduceBlock, kReduceRows));
+      exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, false, msv::LazySrc>", T, code, kScanBlock, kScanRowsSmall));
       exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, true, %d, %d, %d, msv::LazySrc>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
       exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, false, %d, %d, %d, msv::LazySrc>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
     }

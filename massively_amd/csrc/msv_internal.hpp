@@ -97,6 +97,10 @@ struct msv_executor {
   int64_t opt_seg_launch = MSV_SEG_LAUNCH_PERSISTENT;
   int64_t opt_autotune = 1;
   int64_t opt_segsort_config = -1;
+  // Resident blocks per CU of each batched job-kernel family (0 = not yet
+  // queried). Per executor, so per device, and touched only by the thread
+  // that submits on this executor.
+  int batch_per_cu[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // HIP graph capture (msv_executor_begin_capture): while capturing, call
   // scratch and pinned staging are kept for the graph's lifetime.
   bool capturing = false;

@@ -45,7 +45,7 @@ static msv_status reduce_impl(Ctx& c, const T* in, uint32_t n, T init, T* d_resu
     const bool last = tiles == 1;
     T* dst = last ? d_result : (T*)c.scratch((size_t)tiles * sizeof(T));
     if (!dst) return MSV_ERR_OUT_OF_MEMORY;
-    hipLaunchKernelGGL((reduce_tile_kernel<T, OP, kReduceBlock, kReduceRows>), dim3(tiles),
+    hipLaunchKernelGGL((reduce_tile_kernel<T, OP, kReduceBlock, kReduceRows, true>), dim3(tiles),
                        dim3(kReduceBlock), 0, c.stream(), src, m, aligned, dst, last, init);
     MSV_HIP(hipGetLastError());
     if (last) return MSV_OK;

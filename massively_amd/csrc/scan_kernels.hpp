@@ -235,8 +235,10 @@ __device__ inline T tree_fold(T (&v)[N]) {
 // Reduces one tile of BLOCK*ROWS vectors per block into partials[blockIdx.x]
 // with a fixed tree: thread (pairwise over ROWS*V) -> wave (DPP) -> block
 // (pairwise over waves). With `apply_init`, block 0 writes op(init, total)
-// (used when one block covers the whole input).
-template <typename T, int OP, int BLOCK, int ROWS>
+// (used when one block covers the whole input). NT: non-temporal loads, for
+// an input read once (reduce); the float scans' partials pass keeps ordinary
+// loads, since their prefixed pass reads the same input again right after.
+template <typename T, int OP, int BLOCK, int ROWS, bool NT = false>
 __global__ __launch_bounds__(BLOCK) void reduce_tile_kernel(const T* __restrict__ in, uint32_t n,
                                                             bool aligned, T* __restrict__ partials,
                                                             bool apply_init, T init) {
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(BLOCK) void reduce_tile_kernel(const T* __restrict_
   const size_t base = (size_t)blockIdx.x * TILE;
   const bool vec_ok = aligned && (base + TILE <= n) && src_tile_ok(in, base, base + TILE);
   T x[ROWS][V];
-  load_tile<T, BLOCK, ROWS, true>(in, base, n, vec_ok, x, C::identity());  // streamed once
+  load_tile<T, BLOCK, ROWS, NT>(in, base, n, vec_ok, x, C::identity());
   T flat[ROWS * V];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r)
@@ -271,8 +273,10 @@ __global__ __launch_bounds__(BLOCK) void reduce_tile_kernel(const T* __restrict_
 // Order-preserving tile reduction, for user operators that need not be
 // commutative: row by row, each row in index order (the thread's vector, then
 // lanes, then waves), and then the rows in order. (reduce_tile_kernel folds a
-// thread's rows first, which interleaves rows that lie far apart.)
-template <typename T, int OP, int BLOCK, int ROWS, typename Src = const T* __restrict__>
+// thread's rows first, which interleaves rows that lie far apart.) NT as for
+// reduce_tile_kernel (pointer sources only).
+template <typename T, int OP, int BLOCK, int ROWS, bool NT = false,
+          typename Src = const T* __restrict__>
 __global__ __launch_bounds__(BLOCK) void reduce_tile_ordered_kernel(Src in,
                                                                     uint32_t n, bool aligned,
                                                                     T* __restrict__ partials,
@@ -286,7 +290,7 @@ __global__ __launch_bounds__(BLOCK) void reduce_tile_ordered_kernel(Src in,
   const size_t base = (size_t)blockIdx.x * TILE;
   const bool vec_ok = aligned && (base + TILE <= n) && src_tile_ok(in, base, base + TILE);
   T x[ROWS][V];
-  load_tile<T, BLOCK, ROWS, true>(in, base, n, vec_ok, x, C::identity());  // streamed once
+  load_tile<T, BLOCK, ROWS, NT>(in, base, n, vec_ok, x, C::identity());
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
     T v = x[r][0];

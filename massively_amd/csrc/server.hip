@@ -48,6 +48,7 @@ struct Request {
   void* d_out2 = nullptr;
   uint32_t out_len = 0;
   uint32_t count_index = 0;  // copy_where: index in the batch's pinned counts
+  uint32_t readers = 0;      // msv_server_result copies in flight (release waits for 0)
 };
 
 struct Slot {
@@ -459,9 +460,10 @@ msv_status msv_server_result(msv_server* s, uint64_t ticket, void* out, uint64_t
   }
   void *src = nullptr, *src2 = nullptr;
   uint64_t need = 0, need2 = 0;
+  Request* q = nullptr;
   {
     std::lock_guard<std::mutex> g(s->mu);
-    Request* q = find(s, ticket);
+    q = find(s, ticket);
     if (!q || q->state != 2) {
       set_error(MSV_ERR_INVALID_ARGUMENT, ticket, 0, "ticket is not a finished request");
       return MSV_ERR_INVALID_ARGUMENT;
@@ -476,14 +478,26 @@ msv_status msv_server_result(msv_server* s, uint64_t ticket, void* out, uint64_t
       src = q->d_out;
       need = (uint64_t)q->out_len * es;
     }
+    if ((need && (!out || out_bytes < need)) || (out2 && src2 && out2_bytes < need2)) {
+      set_error(MSV_ERR_OUTPUT_TOO_SHORT, need, out_bytes, "output buffer too short");
+      return MSV_ERR_OUTPUT_TOO_SHORT;
+    }
+    // The outputs stay allocated while this copy runs: msv_server_release of
+    // the same ticket waits until no reader is left.
+    ++q->readers;
   }
-  if ((need && (!out || out_bytes < need)) || (out2 && src2 && out2_bytes < need2)) {
-    set_error(MSV_ERR_OUTPUT_TOO_SHORT, need, out_bytes, "output buffer too short");
-    return MSV_ERR_OUTPUT_TOO_SHORT;
+  hipError_t e = hipSetDevice(s->cfg.device);
+  if (e == hipSuccess && need) e = hipMemcpy(out, src, need, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && out2 && src2 && need2) e = hipMemcpy(out2, src2, need2, hipMemcpyDeviceToHost);
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    --q->readers;
   }
-  MSV_HIP(hipSetDevice(s->cfg.device));
-  if (need) MSV_HIP(hipMemcpy(out, src, need, hipMemcpyDeviceToHost));
-  if (out2 && src2 && need2) MSV_HIP(hipMemcpy(out2, src2, need2, hipMemcpyDeviceToHost));
+  s->cv_done.notify_all();
+  if (e != hipSuccess) {
+    set_error(MSV_ERR_LAUNCH, 0, 0, "copying the result: %s", hipGetErrorString(e));
+    return MSV_ERR_LAUNCH;
+  }
   return MSV_OK;
 }
 
@@ -494,9 +508,18 @@ msv_status msv_server_release(msv_server* s, uint64_t ticket) {
     return MSV_ERR_INVALID_ARGUMENT;
   }
   {
-    std::lock_guard<std::mutex> g(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
     Request* q = find(s, ticket);
     if (!q || q->state < 2) {
+      set_error(MSV_ERR_INVALID_ARGUMENT, ticket, 0, "ticket is not a finished request");
+      return MSV_ERR_INVALID_ARGUMENT;
+    }
+    s->cv_done.wait(lk, [&] {
+      Request* r = find(s, ticket);
+      return !r || r->readers == 0;
+    });
+    q = find(s, ticket);  // a concurrent release of the same ticket may have won
+    if (!q) {
       set_error(MSV_ERR_INVALID_ARGUMENT, ticket, 0, "ticket is not a finished request");
       return MSV_ERR_INVALID_ARGUMENT;
     }

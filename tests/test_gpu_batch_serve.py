@@ -240,3 +240,50 @@ def test_server_budget_admission():
             got = _fetch(srv, t)
             x = _host_input(serve.Request("inclusive_scan", np.uint32, n, seed=i))
             assert got.tobytes() == oracle.inclusive_scan(x, "add").tobytes()
+
+
+def test_server_large_requests_alone_equals_mix():
+    """Requests of 2^21..2^26 elements (the upper half of the served stream's
+    size range): the same bytes alone and in a mix with small requests, and
+    each checked against the oracle."""
+    big = [serve.Request("sort_by_key", np.uint32, 1 << 21, seed=501, op=m.op.Less),
+           serve.Request("sort_by_key", np.uint64, 1 << 22, seed=502, op=m.op.Less),
+           serve.Request("sort_by_key", np.float32, 1 << 23, seed=503, op=m.op.Less),
+           serve.Request("inclusive_scan", np.uint32, 1 << 26, seed=504, op=m.op.Add),
+           serve.Request("inclusive_scan", np.float32, 1 << 24, seed=505, op=m.op.Add),
+           serve.Request("reduce", np.float32, 1 << 26, seed=506, op=m.op.Add),
+           serve.Request("reduce", np.uint64, 1 << 25, seed=507, op=m.op.Add),
+           serve.Request("copy_where", np.uint64, 1 << 26, seed=508, pred=m.op.Lt(1 << 63)),
+           serve.Request("copy_where", np.float32, 1 << 21, seed=509, pred=m.op.Lt(0.75))]
+    small = [r for r in _requests() if r.n <= (1 << 17)][:40]
+    with serve.Server(streams=4, max_batch=64, mode="batched", hbm_budget=16 << 30) as srv:
+        alone = [_fetch(srv, srv.submit(r)) for r in big]
+        mix = big[:]
+        for i, r in enumerate(small):
+            mix.insert((i * 7) % (len(mix) + 1), r)
+        tickets = [srv.submit(r) for r in mix]
+        mixed = {id(r): _fetch(srv, t) for r, t in zip(mix, tickets)}
+    for r, a in zip(big, alone):
+        assert _same(a, mixed[id(r)]), (r.kind, r.dtype, r.n)
+        x = _host_input(r)
+        if r.kind == "sort_by_key":
+            perm = oracle.sort_perm(x, "less")
+            assert a[0].tobytes() == x[perm].tobytes()
+            assert a[1].tobytes() == perm.astype(np.uint32).tobytes()
+        elif r.kind == "copy_where":
+            assert a.tobytes() == x[x < r.pred.a].tobytes()
+        elif r.kind == "inclusive_scan":
+            if x.dtype.kind == "f":
+                exact = np.cumsum(x.astype(np.float64))
+                mag = np.cumsum(np.abs(x.astype(np.float64)))
+                # The device-wide float scan's bound (tests/test_gpu_scan.py).
+                lv = np.ceil(np.log2(x.size))
+                assert np.all(np.abs(a.astype(np.float64) - exact) <= (lv + 16) * EPS32 * mag)
+            else:
+                assert a.tobytes() == oracle.inclusive_scan(x, "add").tobytes()
+        else:
+            if x.dtype.kind == "f":
+                s, mag = oracle.sum_f32_f64(x)
+                assert abs(float(a) - s) <= oracle.f32_sum_bound(x.size, mag)
+            else:
+                assert a == oracle.reduce(x, 0, "add")
