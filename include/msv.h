@@ -201,6 +201,19 @@ void* msv_vec_device_len_ptr(const msv_vec* v);
 /* ---- vector:: scans and reductions ----------------------------------- */
 /* vector::inclusive_scan (api/algorithm/scan.rs:96-113; core/scan.rs:1125-1156). */
 msv_status msv_inclusive_scan(msv_executor* ex, msv_slice input, msv_op op, msv_vec** out);
+/* No counterpart in the reference: msv_inclusive_scan into the caller's
+ * `output` (same dtype, at least len(input) elements, not overlapping the
+ * input) that allocates nothing (no hipMalloc/hipFree, no allocating library
+ * call), for callers that cannot allocate: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation. Its look-back status words
+ * and float tile partials live in a static 4 MiB array per GPU, so inputs
+ * whose scan needs more are rejected with MSV_ERR_LENGTH_TOO_LARGE (u32/i32:
+ * any length; u64/i64 and f32: up to about 2^31 elements; f64: up to about
+ * 2^29). Results equal msv_inclusive_scan's bit for bit. Every call on one
+ * GPU shares that array: calls that overlap on one GPU (on different
+ * executors or streams) race and must be ordered by the caller. */
+msv_status msv_inclusive_scan_noalloc(msv_executor* ex, msv_slice input, msv_op op,
+                                      msv_slice output);
 /* vector::exclusive_scan (api/algorithm/scan.rs:216-257; core/scan.rs:1209-1233).
  * `init` points to one host element of the input dtype. */
 msv_status msv_exclusive_scan(msv_executor* ex, msv_slice input, const void* init, msv_op op,

@@ -184,6 +184,7 @@ PROTOTYPES = {
     "msv_vec_device_ptr": (_P, [_P]),
     "msv_vec_device_len_ptr": (_P, [_P]),
     "msv_inclusive_scan": (_I, [_P, _S, _I, _PP]),
+    "msv_inclusive_scan_noalloc": (_I, [_P, _S, _I, _S]),
     "msv_exclusive_scan": (_I, [_P, _S, _P, _I, _PP]),
     "msv_reduce": (_I, [_P, _S, _P, _I, _P]),
     "msv_copy_where": (_I, [_P, _S, _S, _PP]),

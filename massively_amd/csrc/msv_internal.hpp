@@ -169,11 +169,30 @@ class Ctx {
   // Lookback state sized for `words` status words, and a fresh generation
   // for one launch.
   msv_status lookback(size_t words, uint64_t** counter, uint64_t** status, uint32_t* gen);
+  // Arena mode (entry points that allocate nothing): scratch() carves
+  // 256-byte-aligned pieces from the device region [base, base + bytes) and
+  // fails with MSV_ERR_LENGTH_TOO_LARGE when it is exhausted; lookback()
+  // takes its words from the same region with generation `gen` (while
+  // capturing: cleared by a kernel node and generation 1).
+  void use_arena(void* base, size_t bytes, uint32_t gen) {
+    arena_ = (char*)base;
+    arena_bytes_ = bytes;
+    arena_used_ = 0;
+    arena_gen_ = gen;
+  }
 
  private:
   msv_executor* ex_;
   std::vector<void*> scratch_;
+  char* arena_ = nullptr;
+  size_t arena_bytes_ = 0;
+  size_t arena_used_ = 0;
+  uint32_t arena_gen_ = 0;
 };
+
+// Arena bytes one scratch() request of `bytes` takes (scratch() keeps every
+// piece 256-byte aligned).
+inline size_t arena_round(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
 
 // ---- kernel module entry points (implemented in the .hip modules) ----------
 // All take device pointers and lengths already validated by the API layer.
@@ -214,6 +233,9 @@ msv_status runs_scan_run(Ctx& c, msv_dtype kdt, const void* keys, msv_dtype vdt,
 // unique_by_key: the values at run heads, compacted; the count goes to *out_len.
 msv_status unique_runs_run(Ctx& c, msv_dtype kdt, const void* keys, size_t vsize,
                            const void* vals, uint32_t n, void* out, uint32_t* out_len);
+// Once per device, with `device` current: resolves the device address of the
+// static scratch array of the no-allocation entry points and clears it.
+msv_status noalloc_setup(int device, hipStream_t stream);
 // Enqueues a batch of validated-on-entry jobs (msv_run_jobs).
 msv_status run_jobs(Ctx& c, const msv_job* jobs, uint32_t count);
 // Stable sort of every segment of offsets[0..nseg] on its own (mode 1 Less,

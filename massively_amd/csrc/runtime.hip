@@ -196,6 +196,18 @@ Ctx::~Ctx() {
 }
 
 void* Ctx::scratch(size_t bytes) {
+  if (arena_) {
+    const size_t need = arena_round(bytes);
+    if (need > arena_bytes_ - arena_used_) {
+      set_error(MSV_ERR_LENGTH_TOO_LARGE, arena_used_ + need, arena_bytes_,
+                "static scratch exhausted: %zu bytes needed, %zu available", arena_used_ + need,
+                arena_bytes_);
+      return nullptr;
+    }
+    void* p = arena_ + arena_used_;
+    arena_used_ += need;
+    return p;
+  }
   void* p = ex_->allocator.alloc(bytes);
   if (p) (ex_->capturing ? ex_->capture_keep : scratch_).push_back(p);
   return p;
@@ -221,6 +233,22 @@ static __global__ void zero_u64_kernel(uint64_t* p, size_t n) {
 
 msv_status Ctx::lookback(size_t words, uint64_t** counter, uint64_t** status, uint32_t* gen) {
   msv_executor* ex = ex_;
+  if (arena_) {
+    uint64_t* p = (uint64_t*)scratch((words + 1) * 8);
+    if (!p) return MSV_ERR_LENGTH_TOO_LARGE;
+    *counter = p;
+    *status = p + 1;
+    if (ex->capturing) {
+      // Replays reuse one generation, so the words are cleared before each.
+      hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)((words + 1 + 255) / 256)), dim3(256), 0,
+                         ex->stream, p, words + 1);
+      MSV_HIP(hipGetLastError());
+      *gen = 1;
+    } else {
+      *gen = arena_gen_;
+    }
+    return MSV_OK;
+  }
   if (ex->capturing) {
     // A captured launch gets its own counter and status words, cleared by a
     // kernel node before it on every replay, and a fixed generation.
@@ -312,6 +340,14 @@ msv_status msv_executor_create(int32_t device, msv_executor** out) {
     set_error(MSV_ERR_LAUNCH, 0, 0, "executor creation failed: %s", hipGetErrorString(e));
     delete ex;
     return MSV_ERR_LAUNCH;
+  }
+  // Per-device set-up of the static scratch of the no-allocation entry points.
+  const msv_status ns = noalloc_setup(device, ex->stream);
+  if (ns != MSV_OK) {
+    (void)hipStreamDestroy(ex->stream);
+    (void)hipFree(ex->d_counter);
+    delete ex;
+    return ns;
   }
   *out = ex;
   return MSV_OK;

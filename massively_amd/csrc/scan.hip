@@ -209,6 +209,84 @@ static bool op_valid(msv_dtype dt, int op) {
   return !(op == MSV_OP_XOR && dtype_is_float(dt));
 }
 
+// ---------------------------------------------------------------------------
+// Inclusive scan without allocation. Its look-back status words (integer
+// scans) and tile partials (float scans) live in `scratch`, a static array of
+// kNoAllocScratchBytes per device; the scan runs the same host code and
+// kernels as msv_inclusive_scan through a Ctx in arena mode.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = (size_t)4 << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) uint64_t scratch[kNoAllocScratchBytes / 8];
+
+struct NoAllocDevice {
+  std::mutex mu;
+  uint64_t* ptr = nullptr;  // device address of `scratch` on this device
+  // Generation of the next uncaptured call. Generation 1 belongs to captured
+  // calls, which clear their words first, so uncaptured ones start at 2.
+  uint32_t gen = 2;
+};
+static NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+static __global__ void noalloc_clear_kernel(uint64_t* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+static msv_status noalloc_clear(uint64_t* p, hipStream_t s) {
+  hipLaunchKernelGGL(noalloc_clear_kernel, dim3(256), dim3(256), 0, s, p,
+                     kNoAllocScratchBytes / 8);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+msv_status noalloc_setup(int device, hipStream_t stream) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return MSV_OK;  // entry point reports it
+  NoAllocDevice& d = g_noalloc[device];
+  std::lock_guard<std::mutex> lk(d.mu);
+  if (d.ptr) return MSV_OK;
+  void* p = nullptr;
+  MSV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  MSV_TRY(noalloc_clear((uint64_t*)p, stream));
+  MSV_HIP(hipStreamSynchronize(stream));
+  d.ptr = (uint64_t*)p;
+  return MSV_OK;
+}
+
+// Arena bytes the inclusive scan of n elements of T takes (the pieces
+// float_scan_impl and int_scan_rows request, in the order they request them).
+template <typename T>
+static size_t scan_arena_bytes(uint32_t n) {
+  if (n == 0) return 0;
+  if constexpr (IsFloat<T>::value) {
+    constexpr uint32_t TILE = (uint32_t)kScanBlock * kScanRowsSmall * (16 / sizeof(T));
+    size_t total = 0;
+    for (uint32_t m = n;;) {
+      const uint32_t tiles = div_up(m, TILE);
+      if (tiles <= 1) return total;
+      total += 2 * arena_round((size_t)tiles * sizeof(T));
+      m = tiles;
+    }
+  } else {
+    const uint32_t rows = n >= kScanLargeMin ? scan_rows_large<T>() : kScanRowsSmall;
+    const uint32_t tile = (uint32_t)kScanBlock * rows * (16 / sizeof(T));
+    return arena_round(((size_t)div_up(n, tile) * TileStatus<T>::kWords + 1) * 8);
+  }
+}
+
+static size_t scan_arena_bytes(msv_dtype dt, uint32_t n) {
+  switch (dt) {
+    case MSV_U32: return scan_arena_bytes<uint32_t>(n);
+    case MSV_I32: return scan_arena_bytes<int32_t>(n);
+    case MSV_F32: return scan_arena_bytes<float>(n);
+    case MSV_U64: return scan_arena_bytes<uint64_t>(n);
+    case MSV_I64: return scan_arena_bytes<int64_t>(n);
+    case MSV_F64: return scan_arena_bytes<double>(n);
+  }
+  return 0;
+}
+
 }  // namespace msv
 
 using namespace msv;
@@ -232,6 +310,64 @@ msv_status msv_inclusive_scan(msv_executor* ex, msv_slice input, msv_op op, msv_
     *out = nullptr;
   }
   return s;
+}
+
+msv_status msv_inclusive_scan_noalloc(msv_executor* ex, msv_slice input, msv_op op,
+                                      msv_slice output) {
+  clear_error();
+  View in, out;
+  MSV_TRY(resolve(ex, input, &in));
+  MSV_TRY(resolve(ex, output, &out));
+  if (!op_valid(in.dtype, op)) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "operator %d is not defined for dtype %d", (int)op,
+              (int)in.dtype);
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  if (out.dtype != in.dtype) {
+    set_error(MSV_ERR_INVALID_ARGUMENT, (uint64_t)in.dtype, (uint64_t)out.dtype,
+              "output dtype %d differs from input dtype %d", (int)out.dtype, (int)in.dtype);
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  if (out.len < in.len) {
+    set_error(MSV_ERR_OUTPUT_TOO_SHORT, in.len, out.len, "output of %u elements for %u inputs",
+              out.len, in.len);
+    return MSV_ERR_OUTPUT_TOO_SHORT;
+  }
+  const size_t es = dtype_size(in.dtype);
+  const uintptr_t ib = (uintptr_t)in.ptr, ob = (uintptr_t)out.ptr;
+  if (in.len && ib < ob + (size_t)in.len * es && ob < ib + (size_t)in.len * es) {
+    // A look-back tile may re-read a predecessor's input after it was scanned.
+    set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "input and output ranges overlap");
+    return MSV_ERR_INVALID_ARGUMENT;
+  }
+  const size_t need = scan_arena_bytes(in.dtype, in.len);
+  if (need > kNoAllocScratchBytes) {
+    set_error(MSV_ERR_LENGTH_TOO_LARGE, in.len, 0,
+              "a scan of %u elements of dtype %d needs %zu bytes of static scratch; the maximum "
+              "is %zu", in.len, (int)in.dtype, need, kNoAllocScratchBytes);
+    return MSV_ERR_LENGTH_TOO_LARGE;
+  }
+  if (ex->device < 0 || ex->device >= kNoAllocMaxDevices || !g_noalloc[ex->device].ptr) {
+    set_error(MSV_ERR_UNSUPPORTED, (uint64_t)ex->device, 0,
+              "no static scratch on device %d", ex->device);
+    return MSV_ERR_UNSUPPORTED;
+  }
+  if (in.len == 0) return MSV_OK;
+  MSV_HIP(hipSetDevice(ex->device));
+  NoAllocDevice& d = g_noalloc[ex->device];
+  uint32_t gen = 1;
+  if (!ex->capturing) {
+    std::lock_guard<std::mutex> lk(d.mu);
+    // Generations are 29 bits; before one would repeat, the array is cleared.
+    if (d.gen >= (1u << 29) - 1) {
+      MSV_TRY(noalloc_clear(d.ptr, ex->stream));
+      d.gen = 2;
+    }
+    gen = d.gen++;
+  }
+  Ctx c(ex);
+  c.use_arena(d.ptr, kNoAllocScratchBytes, gen);
+  return scan_run(c, in.dtype, op, false, in.ptr, out.ptr, in.len, nullptr);
 }
 
 msv_status msv_exclusive_scan(msv_executor* ex, msv_slice input, const void* init, msv_op op,
