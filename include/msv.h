@@ -345,13 +345,18 @@ msv_status msv_seg_take(msv_executor* ex, msv_slice values, const msv_segmentati
  *   MSV_USER_REDUCTION: `expr` of `a` and `b` (associative; the scans and the
  *     reduce combine in index order, so it need not be commutative) and
  *     `identity`, an expression of the identity element;
- *   MSV_USER_PREDICATE: `expr` of `x`, nonzero = keep (`identity` unused).
+ *   MSV_USER_PREDICATE: `expr` of `x`, nonzero = keep (`identity` unused);
+ *   MSV_USER_LESS: a BinaryPredicateOp used as Less (core/ordering/mod.rs:53-56),
+ *     `expr` of `a` and `b`, nonzero = a comes strictly before b (a strict
+ *     weak ordering; `identity` unused);
+ *   MSV_USER_EQUAL: a BinaryPredicateOp used as Equal, `expr` of `a` and `b`,
+ *     nonzero = a and b are equal keys (`identity` unused).
  * The code runs on the GPU with the caller's privileges, as the reference's
  * user operators do. A compile error returns MSV_ERR_INVALID_ARGUMENT with
  * the compiler's first diagnostic in msv_last_error. An operator belongs to
  * the executor it was created on and to one dtype. */
 typedef struct msv_user_op msv_user_op;
-enum { MSV_USER_REDUCTION = 1, MSV_USER_PREDICATE = 2 };
+enum { MSV_USER_REDUCTION = 1, MSV_USER_PREDICATE = 2, MSV_USER_LESS = 4, MSV_USER_EQUAL = 5 };
 msv_status msv_user_op_create(msv_executor* ex, int32_t kind, msv_dtype dtype, const char* expr,
                               const char* identity, msv_user_op** out);
 msv_status msv_user_op_destroy(msv_user_op* op);
@@ -366,6 +371,49 @@ msv_status msv_exclusive_scan_user(msv_executor* ex, msv_slice input, const void
                                    const msv_user_op* op, msv_vec** out);
 msv_status msv_copy_where_user(msv_executor* ex, msv_slice input, const msv_user_op* pred,
                                msv_vec** out);
+/* The ordering algorithms with a user Less (MSV_USER_LESS of the key dtype),
+ * with the outputs and errors of their built-in forms. Every form is stable:
+ * equal keys keep their input order, and merge and the set operations take
+ * the left input first.
+ *   sort (api/algorithm/ordering.rs:64-76), sort_by_key (by_key.rs:193-250;
+ *     returns the permuted values, `out_keys` nullable);
+ *   merge / merge_by_key (merge.rs:109-282; core/merge.rs:24-214);
+ *   set operations (set.rs:44-195; core/set.rs:48-207): two keys are equal
+ *     when neither is less than the other;
+ *   ForEachSegment(Sort(less)) (seg/mod.rs:379-425): `values` nullable .vec. */
+msv_status msv_sort_user(msv_executor* ex, msv_slice input, const msv_user_op* less,
+                         msv_vec** out);
+msv_status msv_sort_by_key_user(msv_executor* ex, msv_slice keys, msv_slice values,
+                                const msv_user_op* less, msv_vec** out_values,
+                                msv_vec** out_keys);
+msv_status msv_merge_user(msv_executor* ex, msv_slice left, msv_slice right,
+                          const msv_user_op* less, msv_vec** out);
+msv_status msv_merge_by_key_user(msv_executor* ex, msv_slice left_keys, msv_slice left_values,
+                                 msv_slice right_keys, msv_slice right_values,
+                                 const msv_user_op* less, msv_vec** out_keys,
+                                 msv_vec** out_values);
+msv_status msv_set_op_user(msv_executor* ex, msv_set_kind kind, msv_slice left, msv_slice right,
+                           const msv_user_op* less, msv_vec** out);
+msv_status msv_seg_sort_user(msv_executor* ex, msv_slice keys, msv_slice values,
+                             const msv_segmentation* seg, const msv_user_op* less,
+                             msv_vec** out_keys, msv_vec** out_values);
+/* The by-key family and seg::Unique with a user Equal (MSV_USER_EQUAL of the
+ * key dtype; by_key.rs:359-811, seg/mod.rs:597-640): a run is a maximal range
+ * of adjacent keys with equal(k[i-1], k[i]); outputs and errors as in the
+ * built-in forms. */
+msv_status msv_inclusive_scan_by_key_user(msv_executor* ex, msv_slice keys, msv_slice values,
+                                          const msv_user_op* equal, msv_op op, msv_vec** out);
+msv_status msv_exclusive_scan_by_key_user(msv_executor* ex, msv_slice keys, msv_slice values,
+                                          const msv_user_op* equal, const void* init,
+                                          msv_op op, msv_vec** out);
+msv_status msv_reduce_by_key_user(msv_executor* ex, msv_slice keys, msv_slice values,
+                                  const msv_user_op* equal, const void* init, msv_op op,
+                                  msv_vec** out_keys, msv_vec** out_values);
+msv_status msv_unique_by_key_user(msv_executor* ex, msv_slice keys, msv_slice values,
+                                  const msv_user_op* equal, msv_vec** out_values);
+msv_status msv_seg_unique_user(msv_executor* ex, msv_slice values, const msv_segmentation* seg,
+                               const msv_user_op* equal, msv_vec** out_values,
+                               msv_segmentation** out_seg);
 
 /* ---- lazy inputs, fused into the consuming kernel (compiled at run time) ---
  * The reference's lazy inputs (api/lazy.rs:664-920: constant, counting, stride,

@@ -212,6 +212,17 @@ PROTOTYPES = {
     "msv_radix_sort_by_key": (_I, [_P, _S, _S, _PP, _PP]),
     "msv_sort_by_key": (_I, [_P, _S, _S, _I, _PP, _PP]),
     "msv_sort": (_I, [_P, _S, _I, _PP]),
+    "msv_sort_user": (_I, [_P, _S, _P, _PP]),
+    "msv_sort_by_key_user": (_I, [_P, _S, _S, _P, _PP, _PP]),
+    "msv_merge_user": (_I, [_P, _S, _S, _P, _PP]),
+    "msv_merge_by_key_user": (_I, [_P, _S, _S, _S, _S, _P, _PP, _PP]),
+    "msv_set_op_user": (_I, [_P, _I, _S, _S, _P, _PP]),
+    "msv_seg_sort_user": (_I, [_P, _S, _S, _P, _P, _PP, _PP]),
+    "msv_inclusive_scan_by_key_user": (_I, [_P, _S, _S, _P, _I, _PP]),
+    "msv_exclusive_scan_by_key_user": (_I, [_P, _S, _S, _P, _P, _I, _PP]),
+    "msv_reduce_by_key_user": (_I, [_P, _S, _S, _P, _P, _I, _PP, _PP]),
+    "msv_unique_by_key_user": (_I, [_P, _S, _S, _P, _PP]),
+    "msv_seg_unique_user": (_I, [_P, _S, _P, _P, _PP, _PP]),
     "msv_merge": (_I, [_P, _S, _S, _I, _PP]),
     "msv_merge_by_key": (_I, [_P, _S, _S, _S, _S, _I, _PP, _PP]),
     "msv_set_op": (_I, [_P, _I, _S, _S, _I, _PP]),

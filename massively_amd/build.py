@@ -46,6 +46,7 @@ JIT_HEADERS = [("device_common.hpp", CSRC / "device_common.hpp"),
                ("scan_kernels.hpp", CSRC / "scan_kernels.hpp"),
                ("predicate.hpp", CSRC / "predicate.hpp"),
                ("select_kernels.hpp", CSRC / "select_kernels.hpp"),
+               ("merge_kernels.hpp", CSRC / "merge_kernels.hpp"),
                ("../../include/msv.h", PKG.parent / "include" / "msv.h")]
 
 

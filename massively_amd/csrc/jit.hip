@@ -31,11 +31,7 @@
 // fixed-shape (prefixed) passes of the float scans, which keep the order of
 // combination; copy_where uses the one-pass compaction with the predicate on
 // the input itself; map writes the lazy input out.
-#include <hip/hiprtc.h>
-
-#include <string>
-
-#include "msv_internal.hpp"
+#include "jit_internal.hpp"
 #include "scan_kernels.hpp"
 #include "select_kernels.hpp"
 #include "jit_headers.inc"
@@ -59,33 +55,9 @@ struct LazyArgsHost {
 static_assert(sizeof(LazyArgsHost) == 288, "LazyArgs layout");
 }  // namespace msv
 
-struct msv_user_op {
-  int32_t kind;
-  msv_dtype dtype;
-  uint64_t owner_id;
-  int device;
-  bool lazy = false;         // compiled with a read source
-  bool builtin = false;      // built-in reduction code / built-in predicates
-  hipModule_t module = nullptr;
-  // reduction over a pointer input: reduce (512 x 16), tile partials (256 x 4),
-  // exclusive / inclusive prefixed scans
-  hipFunction_t f_reduce = nullptr, f_partials = nullptr, f_scan_ex = nullptr, f_scan_in = nullptr;
-  // the same kernels reading the lazy source
-  hipFunction_t f_reduce_lz = nullptr, f_partials_lz = nullptr, f_scan_ex_lz = nullptr,
-                f_scan_in_lz = nullptr;
-  // built-in integer operator over a lazy input: the one-pass look-back scan
-  // [inclusive, exclusive] x [short tiles, long tiles]
-  hipFunction_t f_lb[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-  uint64_t identity = 0;  // bits of the identity, for host-side padding values
-  // predicate: compaction kernels for short and long inputs
-  hipFunction_t f_sel_small = nullptr, f_sel_large = nullptr;
-  // map: writes the lazy input out
-  hipFunction_t f_map = nullptr;
-};
-
 using namespace msv;
 
-static const char* c_type(msv_dtype dt) {
+const char* msv::jit_c_type(msv_dtype dt) {
   switch (dt) {
     case MSV_U32: return "uint32_t";
     case MSV_I32: return "int32_t";
@@ -96,7 +68,7 @@ static const char* c_type(msv_dtype dt) {
   }
 }
 
-static std::string fmt(const char* f, ...) {
+std::string msv::jit_fmt(const char* f, ...) {
   va_list ap, ap2;
   va_start(ap, f);
   va_copy(ap2, ap);
@@ -108,10 +80,8 @@ static std::string fmt(const char* f, ...) {
   return out;
 }
 
-// Compiles `src` for gfx950 and gives the code object and the symbol of each
-// name expression. Needs no device.
-static msv_status jit_compile(const std::string& src, const std::vector<std::string>& exprs,
-                              std::vector<char>& code, std::vector<std::string>& lowered) {
+msv_status msv::jit_compile(const std::string& src, const std::vector<std::string>& exprs,
+                            std::vector<char>& code, std::vector<std::string>& lowered) {
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "msv_user_op.hip", kJitHeaderCount,
                           kJitHeaderBodies, kJitHeaderNames) != HIPRTC_SUCCESS) {
@@ -154,10 +124,8 @@ static msv_status jit_compile(const std::string& src, const std::vector<std::str
   return MSV_OK;
 }
 
-// Compiles `src`, loads the code object and looks up each name expression.
-static msv_status jit_build(msv_user_op* op, const std::string& src,
-                            const std::vector<std::string>& exprs,
-                            std::vector<hipFunction_t>& fns) {
+msv_status msv::jit_build(msv_user_op* op, const std::string& src,
+                          const std::vector<std::string>& exprs, std::vector<hipFunction_t>& fns) {
   std::vector<char> code;
   std::vector<std::string> lowered;
   MSV_TRY(jit_compile(src, exprs, code, lowered));
@@ -172,8 +140,8 @@ static msv_status jit_build(msv_user_op* op, const std::string& src,
   return MSV_OK;
 }
 
-static msv_status launch(hipFunction_t f, uint32_t grid, uint32_t block, hipStream_t s,
-                         void** args) {
+msv_status msv::jit_launch(hipFunction_t f, uint32_t grid, uint32_t block, hipStream_t s,
+                           void** args) {
   MSV_HIP(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, s, args, nullptr));
   return MSV_OK;
 }
@@ -191,8 +159,8 @@ static Scalar scalar_of(const void* p, size_t size) {
   return v;
 }
 
-static msv_status check_op(msv_executor* ex, const msv_user_op* op, int32_t kind, msv_dtype dt,
-                           bool lazy) {
+msv_status msv::jit_check_op(msv_executor* ex, const msv_user_op* op, int32_t kind,
+                             msv_dtype dt, bool lazy) {
   if (!op || op->kind != kind) {
     set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null user operator or wrong kind");
     return MSV_ERR_INVALID_ARGUMENT;
@@ -260,7 +228,7 @@ static msv_status user_scan(Ctx& c, const msv_user_op* op, Input in, void* out, 
     if (!partials || !pre) return MSV_ERR_OUT_OF_MEMORY;
     bool no = false;
     void* a[] = {in.arg(), (void*)&n, (void*)&aligned, (void*)&partials, (void*)&no, (void*)&ident};
-    MSV_TRY(launch(in.lz ? op->f_partials_lz : op->f_partials, ntiles, kScanBlock, c.stream(), a));
+    MSV_TRY(jit_launch(in.lz ? op->f_partials_lz : op->f_partials, ntiles, kScanBlock, c.stream(), a));
     Input p;
     p.ptr = partials;
     MSV_TRY(user_scan(c, op, p, pre, ntiles, true, ident));
@@ -272,7 +240,7 @@ static msv_status user_scan(Ctx& c, const msv_user_op* op, Input in, void* out, 
                (void*)&aligned, (void*)&status, (void*)&gen, (void*)&prefixes};
   hipFunction_t f = exclusive ? (in.lz ? op->f_scan_ex_lz : op->f_scan_ex)
                               : (in.lz ? op->f_scan_in_lz : op->f_scan_in);
-  return launch(f, ntiles, kScanBlock, c.stream(), a);
+  return jit_launch(f, ntiles, kScanBlock, c.stream(), a);
 }
 
 // Scan of a slice (`input`) or of a lazy input (`lz`).
@@ -285,7 +253,7 @@ static msv_status user_scan_api(msv_executor* ex, const msv_slice* input, const 
   LazyArgsHost h;
   Input src;
   if (lz) {
-    if (!op) return check_op(ex, op, MSV_USER_REDUCTION, MSV_U32, true);
+    if (!op) return jit_check_op(ex, op, MSV_USER_REDUCTION, MSV_U32, true);
     MSV_TRY(resolve_lazy(ex, lz, &h));
     in.len = lz->len;
     in.dtype = op->dtype;
@@ -294,7 +262,7 @@ static msv_status user_scan_api(msv_executor* ex, const msv_slice* input, const 
     MSV_TRY(resolve(ex, *input, &in));
     src.ptr = in.ptr;
   }
-  MSV_TRY(check_op(ex, op, MSV_USER_REDUCTION, in.dtype, lz != nullptr));
+  MSV_TRY(jit_check_op(ex, op, MSV_USER_REDUCTION, in.dtype, lz != nullptr));
   if (!out || (exclusive && !init)) {
     set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null output or init");
     return MSV_ERR_INVALID_ARGUMENT;
@@ -321,7 +289,7 @@ static msv_status user_scan_api(msv_executor* ex, const msv_slice* input, const 
       const void* prefixes = nullptr;
       void* a[] = {(void*)&h,        (void*)&o,      (void*)&n,   (void*)&iv,
                    (void*)&aligned, (void*)&status, (void*)&gen, (void*)&prefixes};
-      s = launch(op->f_lb[exclusive][large], ntiles, kScanBlock, c.stream(), a);
+      s = jit_launch(op->f_lb[exclusive][large], ntiles, kScanBlock, c.stream(), a);
     }
   } else {
     s = user_scan(c, op, src, (*out)->ptr, in.len, exclusive, iv);
@@ -341,14 +309,14 @@ static msv_status user_reduce_api(msv_executor* ex, const msv_slice* input,
   View in;
   LazyArgsHost h;
   if (lz) {
-    if (!op) return check_op(ex, op, MSV_USER_REDUCTION, MSV_U32, true);
+    if (!op) return jit_check_op(ex, op, MSV_USER_REDUCTION, MSV_U32, true);
     MSV_TRY(resolve_lazy(ex, lz, &h));
     in.len = lz->len;
     in.dtype = op->dtype;
   } else {
     MSV_TRY(resolve(ex, *input, &in));
   }
-  MSV_TRY(check_op(ex, op, MSV_USER_REDUCTION, in.dtype, lz != nullptr));
+  MSV_TRY(jit_check_op(ex, op, MSV_USER_REDUCTION, in.dtype, lz != nullptr));
   if (!init || !out_host) {
     set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null init or output");
     return MSV_ERR_INVALID_ARGUMENT;
@@ -372,7 +340,7 @@ static msv_status user_reduce_api(msv_executor* ex, const msv_slice* input,
     void* dst = last ? d_result : c.scratch((size_t)tiles * es);
     if (!dst) return MSV_ERR_OUT_OF_MEMORY;
     void* a[] = {src.arg(), (void*)&m, (void*)&aligned, (void*)&dst, (void*)&last, (void*)&iv};
-    MSV_TRY(launch(src.lz ? op->f_reduce_lz : op->f_reduce, tiles, kReduceBlock, c.stream(), a));
+    MSV_TRY(jit_launch(src.lz ? op->f_reduce_lz : op->f_reduce, tiles, kReduceBlock, c.stream(), a));
     if (last) break;
     src = Input();
     src.ptr = dst;
@@ -411,14 +379,14 @@ static msv_status user_select_api(msv_executor* ex, const msv_slice* input,
   View in;
   LazyArgsHost h;
   if (lz) {
-    if (!pred) return check_op(ex, pred, MSV_USER_PREDICATE, MSV_U32, true);
+    if (!pred) return jit_check_op(ex, pred, MSV_USER_PREDICATE, MSV_U32, true);
     MSV_TRY(resolve_lazy(ex, lz, &h));
     in.len = lz->len;
     in.dtype = pred->dtype;
   } else {
     MSV_TRY(resolve(ex, *input, &in));
   }
-  MSV_TRY(check_op(ex, pred, MSV_USER_PREDICATE, in.dtype, lz != nullptr));
+  MSV_TRY(jit_check_op(ex, pred, MSV_USER_PREDICATE, in.dtype, lz != nullptr));
   if (!out) {
     set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null output");
     return MSV_ERR_INVALID_ARGUMENT;
@@ -455,7 +423,7 @@ static msv_status user_select_api(msv_executor* ex, const msv_slice* input,
                    (void*)&stencil, (void*)pa.bytes, (void*)&invert,
                    (void*)&n,       (void*)&aligned, (void*)&op_,     (void*)&out_len,
                    (void*)&status,  (void*)&gen,    (void*)&ntiles,   (void*)&part_total};
-      s = launch(large ? pred->f_sel_large : pred->f_sel_small, ntiles, kSelBlock, c.stream(), a);
+      s = jit_launch(large ? pred->f_sel_large : pred->f_sel_small, ntiles, kSelBlock, c.stream(), a);
     }
   }
   if (s != MSV_OK) {
@@ -486,7 +454,7 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
   }
   *builtin_out = builtin;
   const bool lazy = read_src != nullptr;
-  const char* T = c_type(dtype);
+  const char* T = jit_c_type(dtype);
   const bool wide = dtype_size(dtype) == 8;
   // hiprtc has the device math functions but not <math.h>'s constants.
   src = "#include \"device_common.hpp\"\n#include \"scan_kernels.hpp\"\n"
@@ -497,7 +465,7 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
   src += T;
   src += ";\n}  // namespace msv\n";
   if (lazy) {
-    src += fmt("namespace msv {\nstruct LazyArgs {\n  const void* p[%d];\n  uint32_t n[%d];\n"
+    src += jit_fmt("namespace msv {\nstruct LazyArgs {\n  const void* p[%d];\n  uint32_t n[%d];\n"
                "  uint64_t q[%d];\n};\nstatic_assert(sizeof(LazyArgs) == 288, \"layout\");\n",
                MSV_LAZY_SLOTS, MSV_LAZY_SLOTS, MSV_LAZY_PARAMS);
     src += "#define MSV_SLOT(k, D, i) ((size_t)(i) < (size_t)la.n[k] ? "
@@ -539,30 +507,30 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
   if (kind == MSV_USER_REDUCTION) {
     const int code = builtin ? builtin_op : kOpUser;
     if (!builtin) {
-      src += fmt("namespace msv {\ntemplate <> struct Combine<UT, %d> {\n", kOpUser);
+      src += jit_fmt("namespace msv {\ntemplate <> struct Combine<UT, %d> {\n", kOpUser);
       src += "  __host__ __device__ static inline UT apply(UT a, UT b) { return (UT)(";
       src += expr;
       src += "); }\n  __host__ __device__ static inline UT identity() { return (UT)(";
       src += identity;
       src += "); }\n};\n}  // namespace msv\n";
     }
-    src += fmt("extern \"C\" __global__ void msv_user_identity(msv::UT* o) { *o = msv::Combine<msv::UT, %d>::identity(); }\n", code);
-    exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, true>", T, code, kReduceBlock, kReduceRows));
-    exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, false>", T, code, kScanBlock, kScanRowsSmall));
-    exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, true, %d, %d, %d>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
-    exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, false, %d, %d, %d>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
+    src += jit_fmt("extern \"C\" __global__ void msv_user_identity(msv::UT* o) { *o = msv::Combine<msv::UT, %d>::identity(); }\n", code);
+    exprs.push_back(jit_fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, true>", T, code, kReduceBlock, kReduceRows));
+    exprs.push_back(jit_fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, false>", T, code, kScanBlock, kScanRowsSmall));
+    exprs.push_back(jit_fmt("msv::scan_tile_kernel<%s, %d, true, %d, %d, %d>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
+    exprs.push_back(jit_fmt("msv::scan_tile_kernel<%s, %d, false, %d, %d, %d>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
     if (lazy && builtin && !dtype_is_float(dtype)) {
       const int large = lazy_scan_rows_large(dtype_size(dtype));
       for (int excl = 0; excl < 2; ++excl)
         for (int rows : {kScanRowsSmall, large})
-          exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, %s, %d, %d, %d, msv::LazySrc>", T, code,
+          exprs.push_back(jit_fmt("msv::scan_tile_kernel<%s, %d, %s, %d, %d, %d, msv::LazySrc>", T, code,
                               excl ? "true" : "false", (int)SCAN_LOOKBACK, kScanBlock, rows));
     }
     if (lazy) {
-      exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, true, msv::LazySrc>", T, code, kReduceBlock, kReduceRows));
-      exprs.push_back(fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, false, msv::LazySrc>", T, code, kScanBlock, kScanRowsSmall));
-      exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, true, %d, %d, %d, msv::LazySrc>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
-      exprs.push_back(fmt("msv::scan_tile_kernel<%s, %d, false, %d, %d, %d, msv::LazySrc>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
+      exprs.push_back(jit_fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, true, msv::LazySrc>", T, code, kReduceBlock, kReduceRows));
+      exprs.push_back(jit_fmt("msv::reduce_tile_ordered_kernel<%s, %d, %d, %d, false, msv::LazySrc>", T, code, kScanBlock, kScanRowsSmall));
+      exprs.push_back(jit_fmt("msv::scan_tile_kernel<%s, %d, true, %d, %d, %d, msv::LazySrc>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
+      exprs.push_back(jit_fmt("msv::scan_tile_kernel<%s, %d, false, %d, %d, %d, msv::LazySrc>", T, code, (int)SCAN_PREFIXED, kScanBlock, kScanRowsSmall));
     }
   } else if (kind == MSV_USER_PREDICATE) {
     std::string P = "msv::Pred<msv::UT>";
@@ -574,11 +542,11 @@ static msv_status gen_source(int32_t kind, msv_dtype dtype, const char* read_src
     }
     const char* U = wide ? "uint64_t" : "uint32_t";
     const char* S = lazy ? ", msv::LazySrcBits" : "";
-    exprs.push_back(fmt("msv::select_kernel<%s, %s, 2, %d, false, %s%s>", U, T, kSelRowsSmall, P.c_str(), S));
-    exprs.push_back(fmt("msv::select_kernel<%s, %s, 2, %d, false, %s%s>", U, T,
+    exprs.push_back(jit_fmt("msv::select_kernel<%s, %s, 2, %d, false, %s%s>", U, T, kSelRowsSmall, P.c_str(), S));
+    exprs.push_back(jit_fmt("msv::select_kernel<%s, %s, 2, %d, false, %s%s>", U, T,
                         wide ? sel_rows_large<uint64_t>() : sel_rows_large<uint32_t>(), P.c_str(), S));
   } else {
-    src += fmt("extern \"C\" __global__ __launch_bounds__(%d) void msv_lazy_map(msv::LazySrc s,\n"
+    src += jit_fmt("extern \"C\" __global__ __launch_bounds__(%d) void msv_lazy_map(msv::LazySrc s,\n"
                "    msv::UT* __restrict__ out, uint32_t n) {\n"
                "  constexpr int V = 16 / sizeof(msv::UT);\n"
                "  const size_t i0 = ((size_t)blockIdx.x * %d + threadIdx.x) * V;\n"
@@ -674,6 +642,13 @@ msv_status msv_lazy_op_create(msv_executor* ex, int32_t kind, msv_dtype dtype,
 msv_status msv_user_op_create(msv_executor* ex, int32_t kind, msv_dtype dtype, const char* expr,
                               const char* identity, msv_user_op** out) {
   clear_error();
+  if (kind == MSV_USER_LESS || kind == MSV_USER_EQUAL) {
+    if (!ex || !out) {
+      set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null executor or output");
+      return MSV_ERR_INVALID_ARGUMENT;
+    }
+    return cmp_op_create(ex, kind, dtype, expr, out);
+  }
   if (!expr || (kind != MSV_USER_REDUCTION && kind != MSV_USER_PREDICATE)) {
     set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad user operator kind, dtype or expression");
     return MSV_ERR_INVALID_ARGUMENT;
@@ -687,7 +662,15 @@ msv_status msv_lazy_op_check(int32_t kind, msv_dtype dtype, const char* read_src
   std::string src;
   std::vector<std::string> exprs;
   bool builtin = false;
-  MSV_TRY(gen_source(kind, dtype, read_src, builtin_op, expr, identity, &builtin, src, exprs));
+  if (kind == MSV_USER_LESS || kind == MSV_USER_EQUAL) {
+    if (read_src) {
+      set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "a comparator takes no read source");
+      return MSV_ERR_INVALID_ARGUMENT;
+    }
+    MSV_TRY(cmp_gen_source(kind, dtype, expr, src, exprs));
+  } else {
+    MSV_TRY(gen_source(kind, dtype, read_src, builtin_op, expr, identity, &builtin, src, exprs));
+  }
   std::vector<char> code;
   std::vector<std::string> lowered;
   return jit_compile(src, exprs, code, lowered);
@@ -752,10 +735,10 @@ msv_status msv_map_lazy(msv_executor* ex, const msv_lazy_args* input, const msv_
                         msv_vec** out) {
   clear_error();
   MSV_TRY(no_capture(ex, "a lazy input"));
-  if (!op) return check_op(ex, op, MSV_LAZY_MAP, MSV_U32, true);
+  if (!op) return jit_check_op(ex, op, MSV_LAZY_MAP, MSV_U32, true);
   LazyArgsHost h;
   MSV_TRY(resolve_lazy(ex, input, &h));
-  MSV_TRY(check_op(ex, op, MSV_LAZY_MAP, op->dtype, true));
+  MSV_TRY(jit_check_op(ex, op, MSV_LAZY_MAP, op->dtype, true));
   if (!out) {
     set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "null output");
     return MSV_ERR_INVALID_ARGUMENT;
@@ -767,7 +750,7 @@ msv_status msv_map_lazy(msv_executor* ex, const msv_lazy_args* input, const msv_
   const uint32_t blocks = (uint32_t)(((uint64_t)n + per_block - 1) / per_block);
   void* o = (*out)->ptr;
   void* a[] = {(void*)&h, (void*)&o, (void*)&n};
-  msv_status s = launch(op->f_map, blocks, kMapBlock, ex->stream, a);
+  msv_status s = jit_launch(op->f_map, blocks, kMapBlock, ex->stream, a);
   if (s != MSV_OK) {
     msv_vec_release(*out);
     *out = nullptr;

@@ -64,7 +64,14 @@ class _Greater(BinaryPredicateOp):
     code, name = 1, "Greater"
 
 
-Less, Greater = _Less(), _Greater()
+class _Equal(BinaryPredicateOp):
+    """The built-in Equal of the by-key family and seg::Unique: numeric ==
+    in the key dtype (-0.0 == +0.0; a NaN equals nothing)."""
+
+    code, name = -1, "Equal"
+
+
+Less, Greater, Equal = _Less(), _Greater(), _Equal()
 
 
 @dataclass(frozen=True)
@@ -172,6 +179,34 @@ class UserPredicate(_UserOp):
     applied by ``vector.copy_where_if(exec, x, x, pred)``."""
 
     kind = 2
+
+    def __init__(self, expr: str):
+        _UserOp.__init__(self, expr, None)
+
+
+class UserLess(_UserOp, BinaryPredicateOp):
+    """A BinaryPredicateOp used as Less (massively/src/core/ordering/mod.rs:53-56):
+    a C++ expression of `a` and `b` over the key type, nonzero when a comes
+    strictly before b, e.g. UserLess("fabs(a) > fabs(b)") (descending by
+    magnitude). It must be a strict weak ordering. Accepted by sort,
+    sort_by_key, merge, merge_by_key, the set operations and seg.Sort; every
+    one of them is stable under it."""
+
+    kind = 4
+    name = "UserLess"
+
+    def __init__(self, expr: str):
+        _UserOp.__init__(self, expr, None)
+
+
+class UserEqual(_UserOp, BinaryPredicateOp):
+    """A BinaryPredicateOp used as Equal: a C++ expression of `a` and `b` over
+    the key type, nonzero when they are equal keys, e.g. UserEqual("a / 10 ==
+    b / 10"). Accepted by the by-key family and seg.Unique; a run is a maximal
+    range of adjacent keys with equal(k[i-1], k[i])."""
+
+    kind = 5
+    name = "UserEqual"
 
     def __init__(self, expr: str):
         _UserOp.__init__(self, expr, None)

@@ -19,7 +19,7 @@ import numpy as np
 
 from . import _lib as L
 from .error import check
-from .op import BinaryPredicateOp, PredicateOp, ReductionOp
+from .op import BinaryPredicateOp, PredicateOp, ReductionOp, UserEqual, UserLess
 from .runtime import DeviceSlice, DeviceVec, Executor, NULL_SLICE, as_slice
 
 _lib = L.lib
@@ -118,7 +118,8 @@ class Filter:
 
 @dataclass(frozen=True)
 class Unique:
-    """Unique(Equal): the built-in numeric equality."""
+    """Unique(equal): the built-in numeric equality (None or op.Equal) or a
+    UserEqual."""
 
     equal: object = None
 
@@ -143,8 +144,13 @@ class ForEachSegment:
             else:
                 keys, payload, pc = as_slice(values), None, NULL_SLICE
             hk, hv = C.c_void_p(), C.c_void_p()
-            check(_lib().msv_seg_sort(exec.handle, keys._c(), pc, seg.handle, alg.less.code,
-                                      C.byref(hk), C.byref(hv) if payload is not None else None))
+            if isinstance(alg.less, UserLess):
+                check(_lib().msv_seg_sort_user(exec.handle, keys._c(), pc, seg.handle,
+                                               alg.less.handle(exec, keys.dtype), C.byref(hk),
+                                               C.byref(hv) if payload is not None else None))
+            else:
+                check(_lib().msv_seg_sort(exec.handle, keys._c(), pc, seg.handle, alg.less.code,
+                                          C.byref(hk), C.byref(hv) if payload is not None else None))
             k = DeviceVec(exec, hk)
             out = (k, DeviceVec(exec, hv)) if payload is not None else k
             return SegmentIterator(out, seg)
@@ -172,6 +178,10 @@ class ForEachSegment:
                 p = alg.pred.c_struct(s.dtype)
                 check(_lib().msv_seg_filter(exec.handle, s._c(), seg.handle, C.byref(p),
                                             C.byref(h), C.byref(hs)))
+            elif isinstance(alg, Unique) and isinstance(alg.equal, UserEqual):
+                check(_lib().msv_seg_unique_user(exec.handle, s._c(), seg.handle,
+                                                 alg.equal.handle(exec, s.dtype), C.byref(h),
+                                                 C.byref(hs)))
             elif isinstance(alg, Unique):
                 check(_lib().msv_seg_unique(exec.handle, s._c(), seg.handle, C.byref(h),
                                             C.byref(hs)))

@@ -12,8 +12,8 @@ import numpy as np
 
 from . import _lib as L
 from .error import LengthMismatch, check
-from .op import (BinaryPredicateOp, PredicateOp, ReductionOp, UnaryOp, UserPredicate,
-                 UserReductionOp)
+from .op import (BinaryPredicateOp, PredicateOp, ReductionOp, UnaryOp, UserEqual, UserLess,
+                 UserPredicate, UserReductionOp)
 from . import lazy
 from .runtime import DeviceSlice, DeviceVec, Executor, as_slice
 
@@ -297,8 +297,14 @@ def sort_by_key(exec: Executor, keys, values, less: BinaryPredicateOp, return_ke
         res = gather(exec, tuple(cols), perm)
         return (res, sk) if return_keys else res
     hv, hk = C.c_void_p(), C.c_void_p()
-    check(_lib().msv_sort_by_key(exec.handle, _in(exec, keys)._c(), _in(exec, values)._c(),
-                                 less.code, C.byref(hv), C.byref(hk) if return_keys else None))
+    k = _in(exec, keys)
+    if isinstance(less, UserLess):
+        check(_lib().msv_sort_by_key_user(exec.handle, k._c(), _in(exec, values)._c(),
+                                          less.handle(exec, k.dtype), C.byref(hv),
+                                          C.byref(hk) if return_keys else None))
+    else:
+        check(_lib().msv_sort_by_key(exec.handle, k._c(), _in(exec, values)._c(),
+                                     less.code, C.byref(hv), C.byref(hk) if return_keys else None))
     v = _out(exec, hv)
     return (v, _out(exec, hk)) if return_keys else v
 
@@ -306,15 +312,23 @@ def sort_by_key(exec: Executor, keys, values, less: BinaryPredicateOp, return_ke
 def sort(exec: Executor, input, less: BinaryPredicateOp) -> DeviceVec:
     """massively/src/api/algorithm/ordering.rs:64-76."""
     h = C.c_void_p()
-    check(_lib().msv_sort(exec.handle, _in(exec, input)._c(), less.code, C.byref(h)))
+    x = _in(exec, input)
+    if isinstance(less, UserLess):
+        check(_lib().msv_sort_user(exec.handle, x._c(), less.handle(exec, x.dtype), C.byref(h)))
+    else:
+        check(_lib().msv_sort(exec.handle, x._c(), less.code, C.byref(h)))
     return _out(exec, h)
 
 
 def merge(exec: Executor, left, right, less: BinaryPredicateOp) -> DeviceVec:
     """massively/src/api/algorithm/merge.rs:109-190: stable, ties take left."""
     h = C.c_void_p()
-    check(_lib().msv_merge(exec.handle, _in(exec, left)._c(), _in(exec, right)._c(), less.code,
-                           C.byref(h)))
+    a, b = _in(exec, left), _in(exec, right)
+    if isinstance(less, UserLess):
+        check(_lib().msv_merge_user(exec.handle, a._c(), b._c(), less.handle(exec, a.dtype),
+                                    C.byref(h)))
+    else:
+        check(_lib().msv_merge(exec.handle, a._c(), b._c(), less.code, C.byref(h)))
     return _out(exec, h)
 
 
@@ -341,17 +355,25 @@ def merge_by_key(exec: Executor, left_keys, left_values, right_keys, right_value
             outs.append(gather(exec, both, pos))
         return keys, tuple(outs)
     hk, hv = C.c_void_p(), C.c_void_p()
-    check(_lib().msv_merge_by_key(
-        exec.handle, _in(exec, left_keys)._c(), _in(exec, left_values)._c(),
-        _in(exec, right_keys)._c(), _in(exec, right_values)._c(), less.code,
-        C.byref(hk), C.byref(hv)))
+    lk = _in(exec, left_keys)
+    args = (exec.handle, lk._c(), _in(exec, left_values)._c(), _in(exec, right_keys)._c(),
+            _in(exec, right_values)._c())
+    if isinstance(less, UserLess):
+        check(_lib().msv_merge_by_key_user(*args, less.handle(exec, lk.dtype), C.byref(hk),
+                                           C.byref(hv)))
+    else:
+        check(_lib().msv_merge_by_key(*args, less.code, C.byref(hk), C.byref(hv)))
     return _out(exec, hk), _out(exec, hv)
 
 
 def _set(kind: int, exec: Executor, left, right, less: BinaryPredicateOp) -> DeviceVec:
     h = C.c_void_p()
-    check(_lib().msv_set_op(exec.handle, kind, _in(exec, left)._c(), _in(exec, right)._c(),
-                            less.code, C.byref(h)))
+    a, b = _in(exec, left), _in(exec, right)
+    if isinstance(less, UserLess):
+        check(_lib().msv_set_op_user(exec.handle, kind, a._c(), b._c(),
+                                     less.handle(exec, a.dtype), C.byref(h)))
+    else:
+        check(_lib().msv_set_op(exec.handle, kind, a._c(), b._c(), less.code, C.byref(h)))
     return _out(exec, h)
 
 
@@ -370,65 +392,108 @@ def set_difference(exec: Executor, left, right, less: BinaryPredicateOp) -> Devi
     return _set(2, exec, left, right, less)
 
 
-# ---- by-key family (built-in Equal: numeric == on the key dtype) -------------
-def inclusive_scan_by_key(exec: Executor, keys, values, op: ReductionOp) -> DeviceVec:
-    """massively/src/api/algorithm/by_key.rs:399-427 with the built-in Equal.
+# ---- by-key family -------------------------------------------------------------
+# The reference's argument order is (keys, values, equal, [init,] op); the
+# equality may be left out for the built-in Equal (numeric == on the key
+# dtype). A UserEqual runs the same algorithms on its own runs.
+def _equal_and_rest(args, n_rest):
+    if len(args) == n_rest + 1 and isinstance(args[0], BinaryPredicateOp):
+        return args[0], args[1:]
+    if len(args) == n_rest:
+        return None, args
+    raise TypeError(f"expected [equal,] and {n_rest} more arguments, got {len(args)}")
+
+
+def _user_equal(equal):
+    return equal if isinstance(equal, UserEqual) else None
+
+
+def inclusive_scan_by_key(exec: Executor, keys, values, *args) -> DeviceVec:
+    """massively/src/api/algorithm/by_key.rs:399-427: ``(keys, values, [equal,] op)``.
     Zipped values: op applies to each column (a row-wise operator made of one
     built-in operator per component)."""
+    equal, (op,) = _equal_and_rest(args, 1)
     cols = _cols(values)
     if cols is not None:
         _same_lengths(exec, cols)
-        return tuple(inclusive_scan_by_key(exec, keys, c, op) for c in cols)
+        return tuple(inclusive_scan_by_key(exec, keys, c, *args) for c in cols)
     h = C.c_void_p()
-    check(_lib().msv_inclusive_scan_by_key(exec.handle, _in(exec, keys)._c(), _in(exec, values)._c(),
-                                           op.code, C.byref(h)))
+    k, v = _in(exec, keys), _in(exec, values)
+    ue = _user_equal(equal)
+    if ue is not None:
+        check(_lib().msv_inclusive_scan_by_key_user(exec.handle, k._c(), v._c(),
+                                                    ue.handle(exec, k.dtype), op.code, C.byref(h)))
+    else:
+        check(_lib().msv_inclusive_scan_by_key(exec.handle, k._c(), v._c(), op.code, C.byref(h)))
     return _out(exec, h)
 
 
-def exclusive_scan_by_key(exec: Executor, keys, values, init, op: ReductionOp) -> DeviceVec:
-    """massively/src/api/algorithm/by_key.rs:497-546 with the built-in Equal.
+def exclusive_scan_by_key(exec: Executor, keys, values, *args) -> DeviceVec:
+    """massively/src/api/algorithm/by_key.rs:497-546: ``(keys, values, [equal,] init, op)``.
     Zipped values: per column, ``init`` a tuple of one value per column."""
+    equal, (init, op) = _equal_and_rest(args, 2)
     cols = _cols(values)
     if cols is not None:
         _same_lengths(exec, cols)
         inits = init if isinstance(init, (tuple, list)) else (init,) * len(cols)
-        return tuple(exclusive_scan_by_key(exec, keys, c, i0, op) for c, i0 in zip(cols, inits))
-    v = _in(exec, values)
+        pre = (equal,) if equal is not None else ()
+        return tuple(exclusive_scan_by_key(exec, keys, c, *pre, i0, op) for c, i0 in zip(cols, inits))
+    k, v = _in(exec, keys), _in(exec, values)
     i = _scalar(init, v.dtype)
     h = C.c_void_p()
-    check(_lib().msv_exclusive_scan_by_key(exec.handle, _in(exec, keys)._c(), v._c(),
-                                           i.ctypes.data_as(C.c_void_p), op.code, C.byref(h)))
+    ue = _user_equal(equal)
+    if ue is not None:
+        check(_lib().msv_exclusive_scan_by_key_user(exec.handle, k._c(), v._c(),
+                                                    ue.handle(exec, k.dtype),
+                                                    i.ctypes.data_as(C.c_void_p), op.code, C.byref(h)))
+    else:
+        check(_lib().msv_exclusive_scan_by_key(exec.handle, k._c(), v._c(),
+                                               i.ctypes.data_as(C.c_void_p), op.code, C.byref(h)))
     return _out(exec, h)
 
 
-def reduce_by_key(exec: Executor, keys, values, init, op: ReductionOp):
-    """massively/src/api/algorithm/by_key.rs:618-682: (run keys, init (+) run values),
-    both with a device-resident length. Zipped values: per column (``init`` a
-    tuple of one value per column), with the run keys once."""
+def reduce_by_key(exec: Executor, keys, values, *args):
+    """massively/src/api/algorithm/by_key.rs:618-682: ``(keys, values, [equal,] init, op)``
+    gives (run keys, init (+) run values), both with a device-resident length.
+    Zipped values: per column (``init`` a tuple of one value per column), with
+    the run keys once."""
+    equal, (init, op) = _equal_and_rest(args, 2)
     cols = _cols(values)
     if cols is not None:
         _same_lengths(exec, cols)
         inits = init if isinstance(init, (tuple, list)) else (init,) * len(cols)
-        pairs = [reduce_by_key(exec, keys, c, i0, op) for c, i0 in zip(cols, inits)]
+        pre = (equal,) if equal is not None else ()
+        pairs = [reduce_by_key(exec, keys, c, *pre, i0, op) for c, i0 in zip(cols, inits)]
         return pairs[0][0], tuple(v for _, v in pairs)
-    v = _in(exec, values)
+    k, v = _in(exec, keys), _in(exec, values)
     i = _scalar(init, v.dtype)
     hk, hv = C.c_void_p(), C.c_void_p()
-    check(_lib().msv_reduce_by_key(exec.handle, _in(exec, keys)._c(), v._c(),
-                                   i.ctypes.data_as(C.c_void_p), op.code, C.byref(hk), C.byref(hv)))
+    ue = _user_equal(equal)
+    if ue is not None:
+        check(_lib().msv_reduce_by_key_user(exec.handle, k._c(), v._c(), ue.handle(exec, k.dtype),
+                                            i.ctypes.data_as(C.c_void_p), op.code, C.byref(hk),
+                                            C.byref(hv)))
+    else:
+        check(_lib().msv_reduce_by_key(exec.handle, k._c(), v._c(), i.ctypes.data_as(C.c_void_p),
+                                       op.code, C.byref(hk), C.byref(hv)))
     return _out(exec, hk), _out(exec, hv)
 
 
-def unique_by_key(exec: Executor, keys, values) -> DeviceVec:
+def unique_by_key(exec: Executor, keys, values, equal: BinaryPredicateOp | None = None) -> DeviceVec:
     """massively/src/api/algorithm/by_key.rs:759-789: the first value of each key run
     (zipped values: per column)."""
     cols = _cols(values)
     if cols is not None:
         _same_lengths(exec, cols)
-        return tuple(unique_by_key(exec, keys, c) for c in cols)
+        return tuple(unique_by_key(exec, keys, c, equal) for c in cols)
     h = C.c_void_p()
-    check(_lib().msv_unique_by_key(exec.handle, _in(exec, keys)._c(), _in(exec, values)._c(),
-                                   C.byref(h)))
+    k, v = _in(exec, keys), _in(exec, values)
+    ue = _user_equal(equal)
+    if ue is not None:
+        check(_lib().msv_unique_by_key_user(exec.handle, k._c(), v._c(), ue.handle(exec, k.dtype),
+                                            C.byref(h)))
+    else:
+        check(_lib().msv_unique_by_key(exec.handle, k._c(), v._c(), C.byref(h)))
     return _out(exec, h)
 
 

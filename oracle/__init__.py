@@ -64,6 +64,8 @@ def lib():
         L.msvo_merge_src.argtypes = [I, I, P, U64, P, U64, P]
         L.msvo_set_op.argtypes = [I, I, I, P, U64, P, U64, P]
         L.msvo_set_op.restype = U64
+        L.msvo_set_op_src.argtypes = [I, I, I, P, U64, P, U64, P]
+        L.msvo_set_op_src.restype = U64
         L.msvo_seg_inclusive_scan.argtypes = [I, I, P, P, P, U64]
         L.msvo_seg_exclusive_scan.argtypes = [I, I, P, P, P, U64, P]
         L.msvo_seg_reduce.argtypes = [I, I, P, P, U64, P, P]
@@ -240,6 +242,16 @@ def set_op(kind: str, left, right, mode: str = "less") -> np.ndarray:
     k = lib().msvo_set_op(DT[a.dtype], ORD[mode], SET_KIND[kind], _p(a), a.size, _p(b), b.size,
                           _p(out))
     return out[:k].copy()
+
+
+def set_op_src(kind: str, left, right, mode: str = "less") -> np.ndarray:
+    """Source of each output of set_op: an index into left, or len(left) + an
+    index into right (the same walk, for orders given by derived keys)."""
+    a, b = _c(left), _c(right)
+    src = np.empty(a.size + b.size, dtype=np.uint32)
+    k = lib().msvo_set_op_src(DT[a.dtype], ORD[mode], SET_KIND[kind], _p(a), a.size, _p(b), b.size,
+                              _p(src))
+    return src[:k].copy()
 
 
 def _offs(offsets) -> np.ndarray:

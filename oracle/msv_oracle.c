@@ -332,6 +332,32 @@ uint64_t msvo_set_op(int dt, int mode, int kind, const void* left, uint64_t nl, 
   return k;
 }
 
+/* The same walk (vector.rs:595-664) giving the source of each output: an
+ * index into left, or nl + an index into right. Returns the output length. */
+uint64_t msvo_set_op_src(int dt, int mode, int kind, const void* left, uint64_t nl,
+                         const void* right, uint64_t nr, uint32_t* src) {
+  const size_t es = dt_size(dt);
+  uint64_t l = 0, r = 0, k = 0;
+  while (l < nl && r < nr) {
+    uint64_t wl = msvo_order_word(dt, (const char*)left + l * es, mode);
+    uint64_t wr = msvo_order_word(dt, (const char*)right + r * es, mode);
+    if (wl < wr) {
+      if (kind != 1) src[k++] = (uint32_t)l;
+      ++l;
+    } else if (wr < wl) {
+      if (kind == 0) src[k++] = (uint32_t)(nl + r);
+      ++r;
+    } else {
+      if (kind != 2) src[k++] = (uint32_t)l;
+      ++l;
+      ++r;
+    }
+  }
+  if (kind != 1) while (l < nl) src[k++] = (uint32_t)l++;
+  if (kind == 0) while (r < nr) src[k++] = (uint32_t)(nl + r++);
+  return k;
+}
+
 /* ---- seg (seg.rs:18-39, 48-88, 155-169) ------------------------------------- */
 int msvo_seg_inclusive_scan(int dt, int op, const void* in, void* out, const uint32_t* offsets,
                             uint64_t nseg) {

@@ -140,6 +140,19 @@ def test_user_ops_on_device_vectors_compile(dtype):
     lazy.check_compiles("predicate", dtype, None, UserPredicate("x > 1"))
 
 
+@pytest.mark.parametrize("dtype", [np.uint32, np.int64, np.float32])
+def test_user_comparators_compile(dtype):
+    """msv_user_op_create's comparator kernels (block sort, merge passes,
+    merge, set walk; run heads) under hiprtc, without a device."""
+    lib = L.load()
+    dt = L.DTYPE_OF_NP[np.dtype(dtype)]
+    assert lib.msv_lazy_op_check(4, dt, None, -1, b"a < b", None) == 0, L.last_error().message
+    assert lib.msv_lazy_op_check(5, dt, None, -1, b"a == b", None) == 0, L.last_error().message
+    assert lib.msv_lazy_op_check(4, dt, None, -1, b"a <", None) == L.ERR_INVALID_ARGUMENT
+    assert b"does not compile" in L.last_error().message
+    assert lib.msv_lazy_op_check(4, dt, b"x", -1, b"a < b", None) == L.ERR_INVALID_ARGUMENT
+
+
 def test_with_table_lowering():
     t = lazy.map(lazy.with_table(_slot(np.uint32, 7), _slot(np.float64, 3)),
                  UnaryOp("table.at(x) * 2.0", np.float64))

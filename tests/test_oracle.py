@@ -238,6 +238,41 @@ def test_set_ops_match_multiset_counts(kind, na, nb, hi):
     assert oracle.set_op(kind, a, b).tolist() == _multiset(kind, a, b)
 
 
+def _rank_rule_src(kind, a, b):
+    """Sources of a set operation by the reference's rank rules
+    (core/set.rs:18-46): with m = count of a key on the left and n on the
+    right, and r the rank of an element inside its equal run, intersection
+    keeps left elements with r < n, difference left elements with r >= n, and
+    union all left elements plus right elements with r >= m, in merge order."""
+    ca, cb = Counter(a.tolist()), Counter(b.tolist())
+    ra, rb = Counter(), Counter()
+    out = []  # (key, side, position)
+    for i, k in enumerate(a.tolist()):
+        r = ra[k]
+        ra[k] += 1
+        n = cb.get(k, 0)
+        if kind == "union" or (kind == "intersection" and r < n) or (kind == "difference" and r >= n):
+            out.append((k, 0, i))
+    if kind == "union":
+        for j, k in enumerate(b.tolist()):
+            r = rb[k]
+            rb[k] += 1
+            if r >= ca.get(k, 0):
+                out.append((k, 1, len(a) + j))
+    out.sort()
+    return [p for _, _, p in out]
+
+
+@pytest.mark.parametrize("kind", ["union", "intersection", "difference"])
+@pytest.mark.parametrize("na,nb,hi", [(0, 7, 4), (7, 0, 4), (300, 200, 12), (1500, 2500, 900)])
+def test_set_op_src_follows_rank_rules(kind, na, nb, hi):
+    a = np.sort(rand(np.uint32, na, 0, hi))
+    b = np.sort(rand(np.uint32, nb, 0, hi))
+    src = oracle.set_op_src(kind, a, b)
+    assert src.tolist() == _rank_rule_src(kind, a, b)
+    assert np.array_equal(np.concatenate([a, b])[src], oracle.set_op(kind, a, b))
+
+
 def _offsets(lengths):
     return np.concatenate([[0], np.cumsum(lengths)]).astype(np.uint32)
 
