@@ -137,11 +137,16 @@ msv_status msv_executor_trim(msv_executor* ex);
  *    and keeps the fastest for the process; 0 uses the built-in default.
  *  MSV_OPT_SEGSORT_CONFIG: -1 (default) = as above, else force candidate k
  *    (0 <= k < msv_segsort_config_count()); MSV_ERR_UNSUPPORTED if candidate
- *    k does not fit the element width. */
+ *    k does not fit the element width.
+ *  MSV_OPT_RADIX_ONESWEEP: the radix sorts' schedule: 1 = one histogram pass
+ *    and one look-back pass per digit (onesweep); 0 = a counting upsweep, a
+ *    scan of the counts and a downsweep per digit (reduce-then-scan); -1
+ *    (default) = onesweep for 4-byte keys, reduce-then-scan for 8-byte keys. */
 typedef enum msv_option {
   MSV_OPT_SEG_LAUNCH = 1,
   MSV_OPT_AUTOTUNE = 2,
-  MSV_OPT_SEGSORT_CONFIG = 3
+  MSV_OPT_SEGSORT_CONFIG = 3,
+  MSV_OPT_RADIX_ONESWEEP = 4
 } msv_option;
 #define MSV_SEG_LAUNCH_PERSISTENT 0
 #define MSV_SEG_LAUNCH_PER_CLASS 1

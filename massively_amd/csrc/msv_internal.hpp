@@ -97,6 +97,7 @@ struct msv_executor {
   int64_t opt_seg_launch = MSV_SEG_LAUNCH_PERSISTENT;
   int64_t opt_autotune = 1;
   int64_t opt_segsort_config = -1;
+  int64_t opt_radix_onesweep = -1;
   // Resident blocks per CU of each batched job-kernel family (0 = not yet
   // queried). Per executor, so per device, and touched only by the thread
   // that submits on this executor.

@@ -56,11 +56,42 @@ static msv_status radix_sort_impl(Ctx& c, const void* keys, const void* vals, ui
   const size_t elem_bytes = (size_t)n * EW * 4;
   void* aos0 = c.scratch(elem_bytes);
   void* aos1 = c.scratch(elem_bytes);
-  uint64_t* orand = (uint64_t*)c.scratch(sizeof(uint64_t) * 2 * ntiles);
   RadixPass* table = (RadixPass*)c.scratch(sizeof(RadixPass) * P);
+  if (!aos0 || !aos1 || !table) return MSV_ERR_OUT_OF_MEMORY;
+  // Schedule: onesweep for 4-byte keys, reduce-then-scan for 8-byte keys
+  // (faster for each on MI355X at 2^28 u32 / 2^27 u64 keys), unless the
+  // executor option forces one.
+  const int64_t sched = c.ex()->opt_radix_onesweep;
+  if (sched == 1 || (sched < 0 && sizeof(K) == 4)) {
+    // Onesweep: histograms of every digit position in one read of the keys,
+    // then one look-back pass per digit position that needs one.
+    uint32_t* hist = (uint32_t*)c.scratch(sizeof(uint32_t) * (kRadixBins * P + P));
+    uint32_t* gbase = (uint32_t*)c.scratch(sizeof(uint32_t) * kRadixBins * P);
+    if (!hist || !gbase) return MSV_ERR_OUT_OF_MEMORY;
+    uint32_t* tile_counters = hist + kRadixBins * P;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(div_up(kRadixBins * P, 256)), dim3(256), 0,
+                       c.stream(), hist, (uint32_t)(kRadixBins * P), 0u);
+    const uint32_t hist_blocks = min(div_up(n, 256u * 16u), (uint32_t)c.num_cus() * 8u);
+    hipLaunchKernelGGL((radix_hist_kernel<K, MODE, 256>), dim3(hist_blocks), dim3(256), 0,
+                       c.stream(), (const K*)keys, n, hist);
+    hipLaunchKernelGGL((radix_onesweep_plan_kernel<P>), dim3(1), dim3(kRadixBins), 0, c.stream(),
+                       hist, n, table, gbase, tile_counters, keys, vals, out_k, out_v, aos0, aos1);
+    MSV_HIP(hipGetLastError());
+    for (int p = 0; p < P; ++p) {
+      uint64_t *counter, *status;
+      uint32_t gen;
+      MSV_TRY(c.lookback((size_t)ntiles * kRadixBins, &counter, &status, &gen));
+      hipLaunchKernelGGL((radix_onesweep_kernel<K, MODE, EW, BLOCK, IPT>), dim3(ntiles),
+                         dim3(BLOCK), 0, c.stream(), table, p, gbase, tile_counters, status, gen,
+                         n);
+      MSV_HIP(hipGetLastError());
+    }
+    return MSV_OK;
+  }
+  uint64_t* orand = (uint64_t*)c.scratch(sizeof(uint64_t) * 2 * ntiles);
   uint32_t* counts = (uint32_t*)c.scratch(sizeof(uint32_t) * kRadixBins * (size_t)ntiles);
   uint32_t* offs = (uint32_t*)c.scratch(sizeof(uint32_t) * kRadixBins * (size_t)ntiles);
-  if (!aos0 || !aos1 || !orand || !table || !counts || !offs) return MSV_ERR_OUT_OF_MEMORY;
+  if (!orand || !counts || !offs) return MSV_ERR_OUT_OF_MEMORY;
 
   // Pass 0's counts and the per-tile OR / AND of the order words, then the
   // plan (which passes run, and where each reads and writes).

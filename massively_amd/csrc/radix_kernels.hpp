@@ -7,6 +7,7 @@
 namespace msv {
 
 constexpr int kRadixIpt = 16;     // keys per thread in a pass tile
+constexpr int kLookbackBatch = 4;  // predecessors a onesweep look-back step reads
 constexpr int kRadixBins = 256;   // 8-bit digits
 
 // Element moved by the passes: the key (1 or 2 dwords) followed by the value
@@ -387,6 +388,291 @@ __global__ __launch_bounds__(BLOCK) void radix_downsweep_kernel(
     const uint32_t tstart = woff + incl - cnt;
     s_tile_start[tid] = tstart;
     s_goff[tid] = goff - tstart;
+  }
+  __syncthreads();
+
+  // Reorder elements into digit-sorted runs in LDS, then write the runs out.
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (dig[j] < 0x100u) s_el[s_tile_start[dig[j]] + s_hist[wid][dig[j]] + rank[j]] = el[j];
+  __syncthreads();
+  const uint32_t tile_n = (uint32_t)min((size_t)TILE, (size_t)n - (size_t)tile * TILE);
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    const uint32_t i = tid + (uint32_t)q * BLOCK;
+    if (i < tile_n) {
+      const Elem<EW> e = s_el[i];
+      const uint32_t d = (uint32_t)(order_bits<K, MODE>(elem_key<K, EW>(e)) >> shift) & 0xffu;
+      store_elem<K, EW>(rp, (size_t)s_goff[d] + i, e);
+    }
+  }
+}
+
+
+// ---- onesweep: one histogram pass, then one pass per digit ---------------------
+// 5. Global digit histograms of every digit position in one read of the keys:
+//    hist[p * 256 + d] = number of keys whose digit p is d. Each thread reads
+//    16 keys per step with 16-byte loads (all in flight before the first is
+//    counted), each wave counts into its own LDS histogram, then the block adds
+//    its counts to the global histogram. A wave whose lanes all hold the same digit (constant bytes are
+//    common: small keys, a constant top byte) adds its count once instead of
+//    64 times to one bank.
+template <typename K, int MODE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void radix_hist_kernel(const K* __restrict__ keys, uint32_t n,
+                                                          uint32_t* __restrict__ hist) {
+  using U = typename Bits<K>::U;
+  constexpr int P = sizeof(K);
+  constexpr int KPT = 16;                   // keys per thread per step
+  constexpr int V = 16 / sizeof(K);         // keys per 16-byte load
+  constexpr int NW = BLOCK / kWave;
+  __shared__ uint32_t s_hw[NW][P * kRadixBins];  // per-wave histograms
+  const uint32_t tid = threadIdx.x;
+  uint32_t* s_h = s_hw[tid / kWave];
+  for (uint32_t i = tid; i < (uint32_t)(NW * P * kRadixBins); i += BLOCK) (&s_hw[0][0])[i] = 0;
+  __syncthreads();
+  const bool aligned = ((uintptr_t)keys & 15u) == 0;
+  const size_t step = (size_t)gridDim.x * BLOCK * KPT;
+  for (size_t b0 = (size_t)blockIdx.x * BLOCK * KPT; b0 < n; b0 += step) {
+    U w[KPT];
+    bool ok[KPT];
+    // Row-striped: load r of thread t covers keys b0 + (r * BLOCK + t) * V ...
+    if (aligned && b0 + (size_t)BLOCK * KPT <= n) {
+#pragma unroll
+      for (int r = 0; r < KPT / V; ++r) {
+        const uint4 q = *reinterpret_cast<const uint4*>(keys + b0 + ((size_t)r * BLOCK + tid) * V);
+        const uint32_t qq[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+          K key;
+          if constexpr (sizeof(K) == 4) key = bit_cast<K>(qq[k]);
+          else key = bit_cast<K>(((uint64_t)qq[2 * k + 1] << 32) | qq[2 * k]);
+          w[r * V + k] = order_bits<K, MODE>(key);
+          ok[r * V + k] = true;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < KPT / V; ++r)
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+          const size_t i = b0 + ((size_t)r * BLOCK + tid) * V + k;
+          ok[r * V + k] = i < n;
+          w[r * V + k] = i < n ? order_bits<K, MODE>(keys[i]) : (U)0;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+      const uint64_t valid = ballot(ok[j]);
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const uint32_t d = (uint32_t)(w[j] >> (8 * p)) & 0xffu;
+        const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+        if (ballot(ok[j] && d != d0) == 0) {
+          if (__lane_id() == 0 && valid) atomicAdd(&s_h[p * kRadixBins + d0], (uint32_t)__builtin_popcountll(valid));
+        } else if (ok[j]) {
+          atomicAdd(&s_h[p * kRadixBins + d], 1u);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < (uint32_t)(P * kRadixBins); i += BLOCK) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) c += s_hw[w][i];
+    if (c) atomicAdd(&hist[i], c);
+  }
+}
+
+// 6. Onesweep plan (one block of 256 threads): the exclusive scan of every
+//    digit position's histogram (the global start of each digit in that
+//    pass), which positions need a pass (a position whose histogram puts all
+//    n keys in one bin is skipped), the pass table (as the reduce-then-scan
+//    plan) and zeroed tile counters.
+template <int P>
+__global__ __launch_bounds__(kRadixBins) void radix_onesweep_plan_kernel(
+    const uint32_t* __restrict__ hist, uint32_t n, RadixPass* __restrict__ table,
+    uint32_t* __restrict__ gbase, uint32_t* __restrict__ tile_counters, const void* in_k,
+    const void* in_v, void* out_k, void* out_v, void* aos0, void* aos1) {
+  __shared__ uint32_t s_trivial[P];
+  __shared__ uint32_t s_wsum[kRadixBins / kWave];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
+  if (t < (uint32_t)P) {
+    s_trivial[t] = 0;
+    tile_counters[t] = 0;
+  }
+  __syncthreads();
+  for (int p = 0; p < P; ++p) {
+    const uint32_t c = hist[p * kRadixBins + t];
+    if (c == n) s_trivial[p] = 1;
+    const uint32_t incl = wave_inclusive_scan<uint32_t, OP_ADD>(c);
+    if (lane == 63) s_wsum[wid] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (uint32_t k = 0; k < wid; ++k) woff += s_wsum[k];
+    gbase[p * kRadixBins + t] = woff + incl - c;
+    __syncthreads();
+  }
+  if (t == 0) {
+    int active = 0;
+    for (int p = 0; p < P; ++p) active += s_trivial[p] ? 0 : 1;
+    // All digits constant (all keys equal): one pass still moves the data.
+    const bool force_last = active == 0;
+    if (force_last) active = 1;
+    int j = 0;
+    for (int p = 0; p < P; ++p) {
+      RadixPass rp;
+      rp.shift = 8u * (uint32_t)p;
+      rp.active = (!s_trivial[p] || (force_last && p == P - 1)) ? 1u : 0u;
+      rp.in_aos = rp.out_aos = 0;
+      rp.src_k = rp.src_v = nullptr;
+      rp.dst_k = rp.dst_v = nullptr;
+      if (rp.active) {
+        rp.in_aos = j > 0;
+        rp.out_aos = j < active - 1;
+        rp.src_k = j == 0 ? in_k : (((j - 1) & 1) ? aos1 : aos0);
+        rp.src_v = j == 0 ? in_v : nullptr;
+        rp.dst_k = j == active - 1 ? out_k : ((j & 1) ? aos1 : aos0);
+        rp.dst_v = j == active - 1 ? out_v : nullptr;
+        ++j;
+      }
+      table[p] = rp;
+    }
+  }
+}
+
+// 7. Onesweep pass: a block claims the next tile id from the pass's counter
+//    (so every predecessor of a tile is already running), ranks its keys as
+//    the downsweep does, publishes its per-digit counts as an aggregate, and
+//    one thread per digit walks back over the predecessors' statuses (adding
+//    aggregates until it meets an inclusive prefix), publishes its own
+//    inclusive prefix and places the digit's run at
+//    global start + prefix. A tile publishes its aggregate before it waits on
+//    anything, so every walk ends.
+template <typename K, int MODE, int EW, int BLOCK, int IPT = kRadixIpt>
+__global__ __launch_bounds__(BLOCK) void radix_onesweep_kernel(
+    const RadixPass* __restrict__ table, int pass, const uint32_t* __restrict__ gbase,
+    uint32_t* __restrict__ tile_counters, uint64_t* __restrict__ status, uint32_t gen,
+    uint32_t n) {
+  constexpr int NW = BLOCK / kWave;
+  constexpr uint32_t TILE = (uint32_t)BLOCK * IPT;
+  __shared__ Elem<EW> s_el[TILE];
+  __shared__ uint32_t s_hist[NW][kRadixBins];     // per-wave counts, then wave offsets
+  __shared__ uint32_t s_tile_start[kRadixBins];   // in-tile start of each digit
+  __shared__ uint32_t s_goff[kRadixBins];         // global position - in-tile position
+  __shared__ uint32_t s_wsum[kRadixBins / kWave];
+  __shared__ uint32_t s_cnt[kRadixBins];          // the tile's digit counts (early)
+  __shared__ uint32_t s_tile;
+
+  const RadixPass rp = table[pass];
+  if (!rp.active) return;
+
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
+  const bool digit_thread = tid < (uint32_t)kRadixBins;
+  for (uint32_t i = tid; i < NW * kRadixBins; i += BLOCK) (&s_hist[0][0])[i] = 0;
+  if (digit_thread) s_cnt[tid] = 0;
+  if (tid == 0) s_tile = atomicAdd(tile_counters + pass, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const size_t base = (size_t)tile * TILE + (size_t)wid * (kWave * IPT);
+  const uint32_t shift = rp.shift;
+
+  Elem<EW> el[IPT];
+  uint32_t dig[IPT];
+  uint32_t rank[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const size_t i = base + (size_t)j * kWave + lane;
+    if (i < n) {
+      el[j] = load_elem<K, EW>(rp, i);
+      dig[j] = (uint32_t)(order_bits<K, MODE>(elem_key<K, EW>(el[j])) >> shift) & 0xffu;
+    } else {
+      dig[j] = 0x100u;
+    }
+  }
+  uint32_t gstart = 0;
+  if (digit_thread) gstart = gbase[pass * kRadixBins + tid];
+
+  // The tile's digit counts first (LDS atomics), published at once as its
+  // aggregate: successors walking back meet an aggregate instead of waiting
+  // for this tile's ranking.
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (dig[j] < 0x100u) atomicAdd(&s_cnt[dig[j]], 1u);
+  __syncthreads();
+  uint64_t* my_status = status + (size_t)tile * kRadixBins + tid;
+  uint32_t cnt = 0;
+  if (digit_thread) {
+    cnt = s_cnt[tid];
+    st_agent(my_status, pack_status(gen, tile == 0 ? kFlagInc : kFlagAgg, 0, cnt));
+  }
+
+  // Wave-level multi-split, as in the downsweep.
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t d = dig[j];
+    const bool valid = d < 0x100u;
+    const uint64_t peers = match_digit<8>(d, valid);
+    uint32_t before = 0;
+    if (valid) before = s_hist[wid][d];
+    const uint32_t below = (uint32_t)__builtin_popcountll(peers & lt);
+    if (valid && below == 0) s_hist[wid][d] = before + (uint32_t)__builtin_popcountll(peers);
+    rank[j] = before + below;
+  }
+  __syncthreads();
+
+  // Per digit (thread t <-> digit t): wave offsets (their sum is cnt).
+  if (digit_thread) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t c = s_hist[w][tid];
+      s_hist[w][tid] = acc;
+      acc += c;
+    }
+  }
+  uint32_t incl = 0;
+  if (digit_thread) {
+    incl = wave_inclusive_scan<uint32_t, OP_ADD>(cnt);
+    if (lane == 63) s_wsum[wid] = incl;
+  }
+  __syncthreads();
+  if (digit_thread) {
+    uint32_t woff = 0;
+    for (uint32_t k = 0; k < wid; ++k) woff += s_wsum[k];
+    const uint32_t tstart = woff + incl - cnt;
+    s_tile_start[tid] = tstart;
+    // Look back: the number of keys with this digit in all earlier tiles,
+    // reading kLookbackBatch predecessors per step (loads in flight
+    // together); an unpublished predecessor is polled again from there.
+    uint32_t prefix = 0;
+    if (tile > 0) {
+      constexpr int B = kLookbackBatch;
+      int64_t q = (int64_t)tile - 1;  // nearest predecessor not yet added
+      for (;;) {
+        uint64_t w[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+          w[k] = q - k >= 0 ? ld_agent(status + (size_t)(q - k) * kRadixBins + tid) : 0ull;
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+          if (done || used < k) continue;
+          const uint32_t f = status_flag(w[k], gen);
+          if (f == 0) continue;  // not published yet (tile 0 always is)
+          prefix += (uint32_t)w[k];
+          used = k + 1;
+          done = f == kFlagInc;
+        }
+        if (done) break;
+        q -= used;
+        if (used == 0) __builtin_amdgcn_s_sleep(1);
+      }
+      st_agent(my_status, pack_status(gen, kFlagInc, 0, prefix + cnt));
+    }
+    s_goff[tid] = gstart + prefix - tstart;
   }
   __syncthreads();
 

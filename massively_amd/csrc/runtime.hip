@@ -489,6 +489,10 @@ msv_status msv_executor_set_option(msv_executor* ex, msv_option opt, int64_t val
       if (value < -1 || value >= (int64_t)msv_segsort_config_count()) break;
       ex->opt_segsort_config = value;
       return MSV_OK;
+    case MSV_OPT_RADIX_ONESWEEP:
+      if (value < -1 || value > 1) break;
+      ex->opt_radix_onesweep = value;
+      return MSV_OK;
   }
   set_error(MSV_ERR_INVALID_ARGUMENT, (uint64_t)opt, (uint64_t)value, "bad option %d = %lld",
             (int)opt, (long long)value);
@@ -505,6 +509,7 @@ msv_status msv_executor_get_option(msv_executor* ex, msv_option opt, int64_t* va
     case MSV_OPT_SEG_LAUNCH: *value = ex->opt_seg_launch; return MSV_OK;
     case MSV_OPT_AUTOTUNE: *value = ex->opt_autotune; return MSV_OK;
     case MSV_OPT_SEGSORT_CONFIG: *value = ex->opt_segsort_config; return MSV_OK;
+    case MSV_OPT_RADIX_ONESWEEP: *value = ex->opt_radix_onesweep; return MSV_OK;
   }
   set_error(MSV_ERR_INVALID_ARGUMENT, (uint64_t)opt, 0, "bad option %d", (int)opt);
   return MSV_ERR_INVALID_ARGUMENT;

@@ -1,4 +1,4 @@
-"""GPU parity: radix_sort_by_key / sort_by_key / sort (onesweep radix), bit-exact.
+"""GPU parity: radix_sort_by_key / sort_by_key / sort (LSD radix), bit-exact.
 
 Stable ascending order in integer / IEEE total order (radix_sort_by_key) or by
 the built-in comparators (Less/Greater; -0.0 == +0.0). NaN keys are excluded
@@ -92,6 +92,28 @@ def test_pass_skipping_constant_digits(exec_):
     lowconst = (RNG.integers(0, 2**24, n, dtype=np.uint32) << np.uint32(8)) | np.uint32(7)
     out = exec_.to_host(m.vector.radix_sort_by_key(exec_, exec_.to_device(lowconst), exec_.to_device(vals)))
     assert np.array_equal(out, oracle.radix_sort_by_key(lowconst, vals)[1])
+
+
+@pytest.mark.parametrize("onesweep", [1, 0])
+def test_radix_paths_agree(exec_, onesweep):
+    """Both radix schedules (onesweep: histogram pass + look-back pass per
+    digit; reduce-then-scan: upsweep + count scan + downsweep per digit)
+    against the oracle, across tile counts and key / value widths."""
+    exec_.set_option(m.Executor.OPT_RADIX_ONESWEEP, onesweep)
+    try:
+        for dtype in (np.uint32, np.float32, np.int64, np.float64):
+            for n in [1, 4097, 65537, 1_000_003]:
+                keys = rand_keys(dtype, n, 1000 if n > 70000 else None)
+                for vals in (np.arange(n, dtype=np.uint32), RNG.integers(0, 2**63, n, dtype=np.uint64)):
+                    v, k = m.vector.radix_sort_by_key(exec_, exec_.to_device(keys), exec_.to_device(vals),
+                                                      return_keys=True)
+                    ok, ov = oracle.radix_sort_by_key(keys, vals)
+                    assert exec_.to_host(v).tobytes() == ov.tobytes(), (dtype, n, onesweep)
+                    assert exec_.to_host(k).tobytes() == ok.tobytes(), (dtype, n, onesweep)
+                s = exec_.to_host(m.vector.sort(exec_, exec_.to_device(keys), m.op.Greater))
+                assert s.tobytes() == oracle.sort(keys, "greater").tobytes(), (dtype, n, onesweep)
+    finally:
+        exec_.set_option(m.Executor.OPT_RADIX_ONESWEEP, -1)
 
 
 def test_regression_pairs(exec_):
