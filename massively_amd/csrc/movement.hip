@@ -7,6 +7,8 @@
 // dwordx4 accesses and each thread keeps 16 independent random accesses in
 // flight. The reference leaves out-of-range indices undefined; here a gather
 // reads zero for them and a scatter drops them, so no index can fault.
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "msv_internal.hpp"
 
@@ -85,6 +87,227 @@ __global__ __launch_bounds__(kMoveBlock) void scatter_kernel(const T* __restrict
     }
 }
 
+// ---- binned scatter ----------------------------------------------------------
+// A scatter into an output far larger than the caches writes one partial
+// cache line per element (a 4-byte write costs a whole line of HBM traffic).
+// The binned form partitions the (index, value) pairs by destination window
+// (256 windows over the output; one look-back pass like a radix digit pass,
+// unordered inside a window: ranks come from LDS atomics), then writes them
+// window by window. Workgroups go to the 8 XCDs round-robin, and each XCD has
+// its own L2, so the partition orders the windows XCD by XCD (window w goes to
+// group w % 8) and the write pass keeps every block on its own group's pairs
+// (block b writes group b % 8): each L2 then merges the writes of the one or
+// two windows its blocks have in flight into whole lines before they reach
+// HBM.
+constexpr int kBinBlock = 512;
+constexpr uint32_t kBinBuckets = 256;  // one look-back thread per window
+constexpr uint32_t kXcds = 8;
+// Pairs per thread: 16 for 4-byte values, 8 for 8-byte values (the tile's
+// pairs are staged in LDS: 64 or 48 KiB).
+template <typename T>
+constexpr int bin_ipt() { return sizeof(T) == 4 ? 16 : 8; }
+
+// Partition position of window w: XCD group w % 8 first, then w / 8.
+__device__ __forceinline__ uint32_t bin_slot(uint32_t w) {
+  return (w % kXcds) * (kBinBuckets / kXcds) + w / kXcds;
+}
+
+__global__ void bin_fill_kernel(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// hist[slot] = number of indices < olen whose window (index >> shift) has
+// partition position `slot`. Four 16-byte index loads per thread per step.
+__global__ __launch_bounds__(1024) void bin_hist_kernel(const uint32_t* __restrict__ idx,
+                                                        uint32_t n, uint32_t olen, uint32_t shift,
+                                                        uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_h[kBinBuckets];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < kBinBuckets; i += blockDim.x) s_h[i] = 0;
+  __syncthreads();
+  const bool aligned = ((uintptr_t)idx & 15u) == 0;
+  const size_t step = (size_t)gridDim.x * blockDim.x * 16;
+  for (size_t b0 = (size_t)blockIdx.x * blockDim.x * 16; b0 < n; b0 += step) {
+    uint32_t d[16];
+    if (aligned && b0 + (size_t)blockDim.x * 16 <= n) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint4 q = *reinterpret_cast<const uint4*>(idx + b0 + ((size_t)r * blockDim.x + tid) * 4);
+        d[4 * r] = q.x; d[4 * r + 1] = q.y; d[4 * r + 2] = q.z; d[4 * r + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const size_t i = b0 + ((size_t)r * blockDim.x + tid) * 4 + k;
+          d[4 * r + k] = i < n ? idx[i] : 0xffffffffu;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (d[j] < olen) atomicAdd(&s_h[bin_slot(d[j] >> shift)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < kBinBuckets; i += blockDim.x)
+    if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// base[s] = exclusive scan of hist, base[512] = total; zeroes the tile counter.
+__global__ __launch_bounds__(kBinBuckets) void bin_plan_kernel(const uint32_t* __restrict__ hist,
+                                                               uint32_t* __restrict__ base,
+                                                               uint32_t* __restrict__ counter) {
+  __shared__ uint32_t s_w[kBinBuckets / kWave];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
+  const uint32_t c = hist[t];
+  const uint32_t incl = wave_inclusive_scan<uint32_t, OP_ADD>(c);
+  if (lane == 63) s_w[wid] = incl;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (uint32_t w = 0; w < kBinBuckets / kWave; ++w) {
+    off += w < wid ? s_w[w] : 0u;
+    tot += s_w[w];
+  }
+  base[t] = off + incl - c;
+  if (t == 0) {
+    base[kBinBuckets] = tot;
+    *counter = 0;
+  }
+}
+
+// Partitions the pairs of one tile by window: a block claims the next tile
+// id, ranks its pairs inside their windows with LDS atomics, publishes its
+// window counts at once, looks back over earlier tiles for each window's
+// prefix (one thread per window, four predecessors per step), and writes each
+// window's run of pairs, compacted through LDS, to base[slot] + prefix.
+template <typename T>
+__global__ __launch_bounds__(kBinBlock) void bin_partition_kernel(
+    const uint32_t* __restrict__ idx, const T* __restrict__ vals, uint32_t n, uint32_t olen,
+    uint32_t shift, const uint32_t* __restrict__ base, uint32_t* __restrict__ counter,
+    uint64_t* __restrict__ status, uint32_t gen, uint32_t* __restrict__ out_idx,
+    T* __restrict__ out_vals) {
+  constexpr int IPT = bin_ipt<T>();
+  constexpr uint32_t TILE = (uint32_t)kBinBlock * IPT;
+  __shared__ uint32_t s_idx[TILE];
+  __shared__ T s_val[TILE];
+  __shared__ uint32_t s_cnt[kBinBuckets];
+  __shared__ uint32_t s_start[kBinBuckets];
+  __shared__ uint32_t s_goff[kBinBuckets];
+  __shared__ uint32_t s_w[kBinBuckets / kWave];
+  __shared__ uint32_t s_tile;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+  const bool window_thread = tid < kBinBuckets;
+  if (window_thread) s_cnt[tid] = 0;
+  if (tid == 0) s_tile = atomicAdd(counter, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const size_t t0 = (size_t)tile * TILE;
+  uint32_t d[IPT], slot[IPT], rank[IPT];
+  T v[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const size_t i = t0 + (size_t)j * kBinBlock + tid;
+    d[j] = i < n ? idx[i] : 0xffffffffu;
+    v[j] = i < n ? vals[i] : T();
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    slot[j] = d[j] < olen ? bin_slot(d[j] >> shift) : kBinBuckets;
+    rank[j] = slot[j] < kBinBuckets ? atomicAdd(&s_cnt[slot[j]], 1u) : 0u;
+  }
+  __syncthreads();
+  uint64_t* my = status + (size_t)tile * kBinBuckets + tid;
+  uint32_t cnt = 0, incl = 0;
+  if (window_thread) {
+    cnt = s_cnt[tid];
+    st_agent(my, pack_status(gen, tile == 0 ? kFlagInc : kFlagAgg, 0, cnt));
+    incl = wave_inclusive_scan<uint32_t, OP_ADD>(cnt);
+    if (lane == 63) s_w[wid] = incl;
+  }
+  __syncthreads();
+  if (window_thread) {
+    uint32_t off = 0;
+    for (uint32_t w = 0; w < wid; ++w) off += s_w[w];
+    const uint32_t start = off + incl - cnt;
+    s_start[tid] = start;
+    uint32_t prefix = 0;
+    if (tile > 0) {
+      int64_t q = (int64_t)tile - 1;
+      for (;;) {
+        uint64_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          w[k] = q - k >= 0 ? ld_agent(status + (size_t)(q - k) * kBinBuckets + tid) : 0ull;
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (done || used < k) continue;
+          const uint32_t f = status_flag(w[k], gen);
+          if (f == 0) continue;
+          prefix += (uint32_t)w[k];
+          used = k + 1;
+          done = f == kFlagInc;
+        }
+        if (done) break;
+        q -= used;
+        if (used == 0) __builtin_amdgcn_s_sleep(1);
+      }
+      st_agent(my, pack_status(gen, kFlagInc, 0, prefix + cnt));
+    }
+    s_goff[tid] = base[tid] + prefix - start;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    if (slot[j] < kBinBuckets) {
+      const uint32_t pos = s_start[slot[j]] + rank[j];
+      s_idx[pos] = d[j];
+      s_val[pos] = v[j];
+    }
+  }
+  __syncthreads();
+  uint32_t total = 0;
+  for (uint32_t w = 0; w < kBinBuckets / kWave; ++w) total += s_w[w];
+  for (uint32_t i = tid; i < total; i += kBinBlock) {
+    const uint32_t dst = s_idx[i];
+    const uint32_t o = s_goff[bin_slot(dst >> shift)] + i;
+    out_idx[o] = dst;
+    out_vals[o] = s_val[i];
+  }
+}
+
+// Writes the partitioned pairs: block b walks the tiles of XCD group b % 8
+// (the pairs of windows w % 8 == b % 8, in window order), grid-strided over
+// the blocks of that group.
+template <typename T>
+__global__ __launch_bounds__(kMoveBlock) void bin_scatter_kernel(const uint32_t* __restrict__ pidx,
+                                                                 const T* __restrict__ pvals,
+                                                                 const uint32_t* __restrict__ base,
+                                                                 T* __restrict__ out) {
+  const uint32_t g = blockIdx.x % kXcds;
+  const uint32_t per_group = gridDim.x / kXcds;
+  const size_t g0 = base[g * (kBinBuckets / kXcds)];
+  const size_t g1 = base[(g + 1) * (kBinBuckets / kXcds)];
+  for (size_t t0 = g0 + (size_t)(blockIdx.x / kXcds) * kMoveTile; t0 < g1;
+       t0 += (size_t)per_group * kMoveTile) {
+    uint32_t dd[kMoveRows * 4];
+    T vv[kMoveRows * 4];
+#pragma unroll
+    for (int r = 0; r < kMoveRows * 4; ++r) {
+      // The pair stream is read once: non-temporal, so the L2 keeps the
+      // output lines being filled.
+      const size_t i = t0 + (size_t)r * kMoveBlock + threadIdx.x;
+      dd[r] = i < g1 ? __builtin_nontemporal_load(pidx + i) : 0xffffffffu;
+      vv[r] = i < g1 ? __builtin_nontemporal_load(pvals + i) : T();
+    }
+#pragma unroll
+    for (int r = 0; r < kMoveRows * 4; ++r)
+      if (dd[r] != 0xffffffffu) out[dd[r]] = vv[r];
+  }
+}
+
 // ---- seeded streams (massively/src/api/util/random.rs:12-22,331-396) ------
 __host__ __device__ inline uint32_t pcg_hash32(uint32_t input) {
   const uint32_t state = input * 747796405u + 2891336453u;
@@ -134,9 +357,52 @@ msv_status gather_run(Ctx& c, msv_dtype dt, const void* values, uint32_t vlen,
   return MSV_OK;
 }
 
+template <typename T>
+static msv_status binned_scatter(Ctx& c, const T* values, const uint32_t* idx, uint32_t n, T* out,
+                                 uint32_t olen) {
+  uint32_t shift = 0;
+  while (((uint64_t)(olen - 1) >> shift) >= kBinBuckets) ++shift;
+  const uint32_t ntiles = div_up(n, (uint32_t)kBinBlock * bin_ipt<T>());
+  uint32_t* hist = (uint32_t*)c.scratch(sizeof(uint32_t) * (2 * kBinBuckets + 2));
+  uint32_t* pidx = (uint32_t*)c.scratch(sizeof(uint32_t) * (size_t)n);
+  T* pvals = (T*)c.scratch(sizeof(T) * (size_t)n);
+  if (!hist || !pidx || !pvals) return MSV_ERR_OUT_OF_MEMORY;
+  uint32_t* base = hist + kBinBuckets;       // kBinBuckets + 1 words
+  uint32_t* counter = base + kBinBuckets + 1;
+  uint64_t *lb_counter, *status;
+  uint32_t gen;
+  MSV_TRY(c.lookback((size_t)ntiles * kBinBuckets, &lb_counter, &status, &gen));
+  hipLaunchKernelGGL(bin_fill_kernel, dim3(div_up(kBinBuckets, 256)), dim3(256), 0, c.stream(),
+                     hist, kBinBuckets);
+  const uint32_t hist_blocks =
+      std::min<uint32_t>(div_up(n, 1024u * 16u), (uint32_t)c.num_cus() * 4u);
+  hipLaunchKernelGGL(bin_hist_kernel, dim3(hist_blocks), dim3(1024), 0, c.stream(), idx, n, olen,
+                     shift, hist);
+  hipLaunchKernelGGL(bin_plan_kernel, dim3(1), dim3(kBinBuckets), 0, c.stream(), hist, base,
+                     counter);
+  hipLaunchKernelGGL(bin_partition_kernel<T>, dim3(ntiles), dim3(kBinBlock), 0, c.stream(), idx,
+                     values, n, olen, shift, base, counter, status, gen, pidx, pvals);
+  // A multiple of 8 blocks, a few per CU: blocks stay on their XCD's group.
+  const uint32_t grid = kXcds * std::max<uint32_t>(1u, std::min<uint32_t>(
+                                    div_up(n, kMoveTile * kXcds), (uint32_t)c.num_cus() * 4u / kXcds));
+  hipLaunchKernelGGL(bin_scatter_kernel<T>, dim3(grid), dim3(kMoveBlock), 0, c.stream(), pidx,
+                     pvals, base, out);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+// Outputs from this size up take the binned form (the direct scatter's
+// partial-line writes stay in the caches below it).
+constexpr uint64_t kBinScatterMinBytes = 64ull << 20;
+
 msv_status scatter_run(Ctx& c, msv_dtype dt, const void* values, const uint32_t* idx,
                        uint32_t n, void* out, uint32_t olen) {
   if (n == 0) return MSV_OK;
+  if ((uint64_t)olen * dtype_size(dt) >= kBinScatterMinBytes && n >= (1u << 20)) {
+    if (dtype_size(dt) == 4)
+      return binned_scatter<uint32_t>(c, (const uint32_t*)values, idx, n, (uint32_t*)out, olen);
+    return binned_scatter<uint64_t>(c, (const uint64_t*)values, idx, n, (uint64_t*)out, olen);
+  }
   const uint32_t blocks = div_up(n, kMoveTile);
   if (dtype_size(dt) == 4)
     hipLaunchKernelGGL(scatter_kernel<uint32_t>, dim3(blocks), dim3(kMoveBlock), 0, c.stream(),
