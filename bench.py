@@ -175,6 +175,12 @@ def w_sort(c):
     vals = c.ex.alloc(np.uint32, n)
     c.mr.iota_u32(c.ex, vals)
     out["u64_uniform_2^27"] = _sort_case(c, "sort_by_key_u64_uniform_2^27", keys, vals, n, 8, 4, 8)
+    # The onesweep schedule on the same input (the default for 8-byte keys
+    # is reduce-then-scan), for comparison.
+    c.ex.set_option(m.Executor.OPT_RADIX_ONESWEEP, 1)
+    out["u64_uniform_2^27_onesweep"] = _sort_case(c, "sort_by_key_u64_uniform_2^27_onesweep", keys, vals,
+                                                  n, 8, 4, 8)
+    c.ex.set_option(m.Executor.OPT_RADIX_ONESWEEP, -1)
     ref = c.ex.to_host(m.vector.radix_sort_by_key(c.ex, keys, vals))
     out["rocprim_u64_uniform_2^27"] = rocprim_sort_case(c, "sort_pairs_u64_u32_2^27", keys, vals, n, 8,
                                                         check=ref)
@@ -200,6 +206,12 @@ def w_sort(c):
     v28 = c.ex.alloc(np.uint32, n28)
     c.mr.iota_u32(c.ex, v28)
     out["u32_2^28"] = _sort_case(c, "sort_by_key_u32_2^28", k28, v28, n28, 4, 4, 4)
+    # The reduce-then-scan schedule (upsweep + count scan + downsweep per
+    # digit), for comparison with the default onesweep schedule.
+    c.ex.set_option(m.Executor.OPT_RADIX_ONESWEEP, 0)
+    out["u32_2^28_reduce_then_scan"] = _sort_case(c, "sort_by_key_u32_2^28_reduce_then_scan", k28, v28,
+                                                  n28, 4, 4, 4)
+    c.ex.set_option(m.Executor.OPT_RADIX_ONESWEEP, -1)
     out["rocprim_u32_2^28"] = rocprim_sort_case(c, "sort_pairs_u32_u32_2^28", k28, v28, n28, 4)
     out["reference_schedule_proxy_u32_2^28"] = refproxy_case(c, k28, v28, n28, reps=2)
     del k28, v28
@@ -543,7 +555,8 @@ def served_trace(np, jobs=500, seed=10, log2_max=26):
 
 def run_served(np, mode, reqs, arrivals_us=None):
     """Submits the requests (at the given arrival times, or all at once),
-    returns (makespan_us, latencies_us)."""
+    returns (makespan_us, latencies_us, service_us): latency is submit to
+    done, service is admit to done."""
     from massively_amd import serve
 
     srv = serve.Server(streams=4, max_batch=256, mode=mode, hbm_budget=64 << 30)
@@ -560,57 +573,93 @@ def run_served(np, mode, reqs, arrivals_us=None):
                 while srv.now_us() < due:
                     pass
             tickets.append(srv.submit(r))
-        lat, last = [], t0
+        lat, svc, last = [], [], t0
         for t in tickets:
             info = srv.wait(t)
             lat.append(info["done_us"] - info["submit_us"])
+            svc.append(info["done_us"] - info["admit_us"])
             last = max(last, info["done_us"])
             srv.release(t)
-        return last - t0, lat
+        return last - t0, lat, svc
     finally:
         srv.close()
 
 
+SIZE_BANDS = [(10, 14), (14, 18), (18, 22), (22, 27)]
+
+
+def band_percentiles(np, reqs, lat):
+    """p50 / p99 latency (us) per size band [2^lo, 2^hi) of the request length."""
+    out = {}
+    n = np.array([r.n for r in reqs])
+    lat = np.asarray(lat)
+    for lo, hi in SIZE_BANDS:
+        m = (n >= (1 << lo)) & (n < (1 << hi))
+        if m.any():
+            out[f"2^{lo}..2^{hi}"] = dict(jobs=int(m.sum()),
+                                         p50_us=round(float(np.percentile(lat[m], 50)), 1),
+                                         p99_us=round(float(np.percentile(lat[m], 99)), 1))
+    return out
+
+
 def w_served(c):
     """Served mixed stream: capacity (all requests at once) and Poisson
-    arrivals at 80% of the one-launch-per-job capacity, for continuous
-    batching vs one request at a time through the device-wide kernels."""
+    arrivals at 80% of the serial capacity, for continuous batching against
+    two baselines: one request at a time through the device-wide kernels
+    ("serial") and one request per launch sequence on the same four streams
+    ("per_job"). The Amdahl bound on batching's gain over serial comes from the
+    serial service times: batching can remove at most the time of the
+    requests the batched kernels take (all requests of at most 2^22
+    elements), never that of the larger ones."""
     np = c.np
     reqs, bytes_ = served_trace(np)
     total_bytes = float(sum(bytes_))
     out = {"jobs": len(reqs), "algorithmic_bytes": total_bytes}
     cap = {}
-    for mode in ("batched", "serial"):
-        span, lat = run_served(np, mode, reqs)
+    modes = ("batched", "per_job", "serial")
+    for mode in modes:
+        span, lat, svc = run_served(np, mode, reqs)
         cap[mode] = len(reqs) / (span * 1e-6)
         out[f"capacity_{mode}"] = dict(makespan_ms=round(span / 1e3, 3), jobs_per_s=round(cap[mode], 1),
                                        gbs=round(total_bytes / (span * 1e-6) / 1e9, 1))
         log(f"[bench] served capacity {mode}: {span / 1e3:.2f} ms for {len(reqs)} jobs = "
             f"{cap[mode]:.0f} jobs/s, {total_bytes / (span * 1e-6) / 1e9:.1f} GB/s")
+        if mode == "serial":
+            svc = np.asarray(svc)
+            big = np.array([r.n > (1 << 22) for r in reqs])
+            out["serial_service_ms"] = {"all": round(float(svc.sum()) / 1e3, 3),
+                                        "over_2^22": round(float(svc[big].sum()) / 1e3, 3)}
+            out["amdahl_bound_vs_serial"] = round(float(svc.sum() / max(svc[big].sum(), 1e-9)), 3)
+            log(f"[bench] served serial service: {svc.sum() / 1e3:.2f} ms in all, "
+                f"{svc[big].sum() / 1e3:.2f} ms for requests over 2^22 "
+                f"(bound on the batched gain {out['amdahl_bound_vs_serial']:.2f}x)")
     rate = 0.8 * cap["serial"]
     rng = np.random.default_rng(10)
     arrivals = np.cumsum(rng.exponential(1e6 / rate, len(reqs)))
     out["poisson_rate_jobs_per_s"] = round(rate, 1)
-    for mode in ("batched", "serial"):
-        span, lat = run_served(np, mode, reqs, arrivals)
+    for mode in modes:
+        span, lat, _ = run_served(np, mode, reqs, arrivals)
         p50, p99 = np.percentile(lat, 50), np.percentile(lat, 99)
         out[f"poisson_{mode}"] = dict(p50_us=round(float(p50), 1), p99_us=round(float(p99), 1),
                                       mean_us=round(float(np.mean(lat)), 1),
-                                      jobs_per_s=round(len(reqs) / (span * 1e-6), 1))
+                                      jobs_per_s=round(len(reqs) / (span * 1e-6), 1),
+                                      by_size=band_percentiles(np, reqs, lat))
         log(f"[bench] served poisson {mode} @ {rate:.0f} jobs/s: p50 {p50:.0f} us, p99 {p99:.0f} us")
     out["capacity_ratio_batched_over_serial"] = round(cap["batched"] / cap["serial"], 3)
+    out["capacity_ratio_batched_over_per_job"] = round(cap["batched"] / cap["per_job"], 3)
     # The launch-bound part of the stream on its own: the same mix with sizes
     # 2^10..2^16 only (a separate, labelled measurement).
     sreqs, sbytes = served_trace(np, jobs=2000, seed=11, log2_max=16)
     scap = {}
-    for mode in ("batched", "serial"):
-        span, lat = run_served(np, mode, sreqs)
+    for mode in modes:
+        span, lat, _ = run_served(np, mode, sreqs)
         scap[mode] = len(sreqs) / (span * 1e-6)
         out[f"small_jobs_capacity_{mode}"] = dict(jobs=len(sreqs), makespan_ms=round(span / 1e3, 3),
                                                   jobs_per_s=round(scap[mode], 1),
                                                   gbs=round(sum(sbytes) / (span * 1e-6) / 1e9, 2))
         log(f"[bench] served small-jobs capacity {mode}: {scap[mode]:.0f} jobs/s")
     out["small_jobs_capacity_ratio"] = round(scap["batched"] / scap["serial"], 3)
+    out["small_jobs_capacity_ratio_over_per_job"] = round(scap["batched"] / scap["per_job"], 3)
     return out
 
 

@@ -546,20 +546,26 @@ msv_status msv_run_jobs(msv_executor* ex, const msv_job* jobs, uint32_t count);
  * admitted requests are batched continuously: every batch goes to the next
  * free stream as soon as one is free, so requests that arrive while others
  * run overlap with them on the device. Outputs stay device-resident until
- * fetched and released. MSV_SERVE_SERIAL runs one request at a time through
- * the device-wide kernels (one launch sequence per job), the form the batched
- * mode is measured against. Results are the same in both modes for integer
+ * fetched and released. Small requests are not held behind large ones: at
+ * most half of the streams run large requests (those too big for the batched
+ * kernels) at once, and small requests behind a large head that cannot start
+ * yet are admitted past it (up to 1024 positions). Two forms the batched mode
+ * is measured against: MSV_SERVE_SERIAL runs one request at a time through
+ * the device-wide kernels (one launch sequence per job); MSV_SERVE_PER_JOB
+ * runs one request per launch sequence through the device-wide kernels on
+ * every stream, in arrival order. Results are the same in all modes for integer
  * and sort jobs, and the same for one request alone or in any mix within a
  * mode (floating-point scan/reduce of jobs small enough for the batched
  * kernels follow that kernel's fixed order). */
 typedef struct msv_server msv_server;
 #define MSV_SERVE_BATCHED 0u
 #define MSV_SERVE_SERIAL 1u
+#define MSV_SERVE_PER_JOB 2u
 typedef struct msv_server_config {
   int32_t device;
   uint32_t streams;    /* streams (executors) batches rotate over, 1..16 */
   uint32_t max_batch;  /* requests per batch, >= 1 */
-  uint32_t mode;       /* MSV_SERVE_BATCHED or MSV_SERVE_SERIAL */
+  uint32_t mode;       /* MSV_SERVE_BATCHED, MSV_SERVE_SERIAL or MSV_SERVE_PER_JOB */
   uint64_t hbm_budget; /* bytes of device memory admitted requests may hold */
 } msv_server_config;
 typedef struct msv_request {

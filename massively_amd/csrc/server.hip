@@ -15,8 +15,17 @@
 //     msv_run_jobs call, the copy_where counts copied to pinned memory, an
 //     event) and returns to step 1 without waiting.
 // While batches run, new arrivals are admitted onto the other streams, so
-// their work overlaps on the device. MSV_SERVE_SERIAL instead runs one request
-// at a time, through the device-wide kernels.
+// their work overlaps on the device. A large request owns the device by
+// itself, so at most half of the streams run large requests at once; small
+// requests behind a large head that cannot be admitted (that cap, or the
+// budget) are taken from up to kServeBypass positions behind it, so they do
+// not wait for the large ones (bounded bypass: the head keeps its place and
+// is admitted as soon as a large slot and the budget allow).
+// MSV_SERVE_SERIAL instead runs one request at a time through the device-wide
+// kernels, and MSV_SERVE_PER_JOB runs one request per launch through the
+// device-wide kernels on every stream (concurrency without batching): the two
+// forms the batched mode is measured against.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -31,6 +40,7 @@ namespace msv {
 constexpr uint32_t kServeSmallStream = 1u << 17;  // the batched kernels' limits
 constexpr uint32_t kServeSmallSort = 1u << 15;
 constexpr uint32_t kServeMaxStreams = 16;
+constexpr size_t kServeBypass = 1024;  // pending positions a small batch may look past
 
 struct Request {
   uint64_t id = 0;
@@ -179,7 +189,7 @@ static msv_status launch_batch(msv_server* s, int k) {
     j.out2 = q->d_out2;
     jobs.push_back(j);
   }
-  if (s->cfg.mode == MSV_SERVE_SERIAL) {
+  if (s->cfg.mode != MSV_SERVE_BATCHED) {
     MSV_TRY(run_jobs_plain(c, jobs.data(), (uint32_t)jobs.size()));
   } else {
     MSV_TRY(run_jobs(c, jobs.data(), (uint32_t)jobs.size()));
@@ -257,18 +267,46 @@ static void scheduler(msv_server* s) {
         }
       if (k < 0) break;
       Slot& sl = s->slots[k];
-      const uint32_t cap = s->cfg.mode == MSV_SERVE_SERIAL ? 1u : s->cfg.max_batch;
-      while (!s->pending.empty() && sl.reqs.size() < cap) {
+      const bool batched = s->cfg.mode == MSV_SERVE_BATCHED;
+      const uint32_t cap = batched ? s->cfg.max_batch : 1u;
+      if (!batched) {
+        // One request per launch, in arrival order.
         Request* q = s->pending.front();
-        const bool small = is_small(q->r);
-        if (!sl.reqs.empty() && (!small || !is_small(sl.reqs[0]->r))) break;
-        if (s->used + q->bytes > s->cfg.hbm_budget) break;
-        s->used += q->bytes;
-        s->pending.pop_front();
-        sl.reqs.push_back(q);
-        if (!small) break;
+        if (s->used + q->bytes <= s->cfg.hbm_budget) {
+          s->used += q->bytes;
+          s->pending.pop_front();
+          sl.reqs.push_back(q);
+        }
+      } else {
+        uint32_t large_busy = 0;
+        for (const Slot& o : s->slots) large_busy += (o.busy && !o.reqs.empty() && !is_small(o.reqs[0]->r)) ? 1u : 0u;
+        const uint32_t large_cap = std::max<uint32_t>(1u, (uint32_t)s->slots.size() / 2);
+        Request* head = s->pending.front();
+        if (!is_small(head->r) && large_busy < large_cap &&
+            s->used + head->bytes <= s->cfg.hbm_budget) {
+          s->used += head->bytes;
+          s->pending.pop_front();
+          sl.reqs.push_back(head);
+        } else {
+          // A batch of small requests in arrival order, looking past large
+          // ones (which keep their places) for at most kServeBypass positions.
+          size_t i = 0;
+          const bool head_small = is_small(head->r);
+          while (i < s->pending.size() && i < kServeBypass && sl.reqs.size() < cap) {
+            Request* q = s->pending[i];
+            if (!is_small(q->r)) {
+              ++i;
+              continue;
+            }
+            if (s->used + q->bytes > s->cfg.hbm_budget) break;
+            s->used += q->bytes;
+            s->pending.erase(s->pending.begin() + (ptrdiff_t)i);
+            sl.reqs.push_back(q);
+          }
+          (void)head_small;
+        }
       }
-      if (sl.reqs.empty()) break;  // the head does not fit the budget yet
+      if (sl.reqs.empty()) break;  // nothing admissible now
       const double t = now_us(s);
       for (Request* q : sl.reqs) {
         q->state = 1;
@@ -323,13 +361,14 @@ extern "C" {
 msv_status msv_server_create(const msv_server_config* cfg, msv_server** out) {
   clear_error();
   if (!cfg || !out || cfg->streams == 0 || cfg->streams > kServeMaxStreams ||
-      cfg->max_batch == 0 || cfg->mode > MSV_SERVE_SERIAL || cfg->hbm_budget == 0) {
+      cfg->max_batch == 0 || cfg->mode > MSV_SERVE_PER_JOB || cfg->hbm_budget == 0) {
     set_error(MSV_ERR_INVALID_ARGUMENT, 0, 0, "bad server configuration");
     return MSV_ERR_INVALID_ARGUMENT;
   }
   auto* s = new msv_server();
   s->cfg = *cfg;
   if (s->cfg.mode == MSV_SERVE_SERIAL) s->cfg.streams = 1;
+  if (s->cfg.mode == MSV_SERVE_PER_JOB) s->cfg.max_batch = 1;
   s->t0 = std::chrono::steady_clock::now();
   s->slots.resize(s->cfg.streams);
   for (Slot& sl : s->slots) {

@@ -6,8 +6,11 @@ one persistent launch per job family). ``Server`` is the request loop of
 ``msv_server_*``: requests are admitted in arrival order by an HBM budget,
 their inputs are created on the device, admitted requests are batched
 continuously over several streams, and outputs stay device-resident until
-fetched. ``Server(mode="serial")`` runs one request at a time through the
-device-wide kernels (the one-launch-per-job form).
+fetched. Two forms the batched mode is measured against:
+``Server(mode="serial")`` runs one request at a time through the device-wide
+kernels, and ``Server(mode="per_job")`` runs one request per launch sequence
+through the device-wide kernels on every stream (concurrency without
+batching).
 """
 from __future__ import annotations
 
@@ -120,13 +123,17 @@ class Request:
 
 
 class Server:
-    """``msv_server``: continuous batching ("batched") or one request at a
-    time through the device-wide kernels ("serial")."""
+    """``msv_server``: continuous batching ("batched"), one request at a time
+    through the device-wide kernels ("serial"), or one request per launch
+    sequence on every stream ("per_job")."""
 
     def __init__(self, device: int = 0, streams: int = 4, max_batch: int = 256,
                  mode: str = "batched", hbm_budget: int = 64 << 30):
         self._lib = L.lib()
-        cfg = L.MsvServerConfig(device, streams, max_batch, 0 if mode == "batched" else 1, hbm_budget)
+        modes = {"batched": 0, "serial": 1, "per_job": 2}
+        if mode not in modes:
+            raise ValueError(f"mode must be one of {sorted(modes)}")
+        cfg = L.MsvServerConfig(device, streams, max_batch, modes[mode], hbm_budget)
         h = C.c_void_p()
         check(self._lib.msv_server_create(C.byref(cfg), C.byref(h)))
         self._h = h

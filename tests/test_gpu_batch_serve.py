@@ -191,7 +191,7 @@ def _same(a, b):
     return np.array([a]).tobytes() == np.array([b]).tobytes()
 
 
-@pytest.mark.parametrize("mode", ["batched", "serial"])
+@pytest.mark.parametrize("mode", ["batched", "serial", "per_job"])
 def test_server_alone_equals_mix(mode):
     reqs = _requests()
     with serve.Server(streams=4, max_batch=64, mode=mode, hbm_budget=8 << 30) as srv:
@@ -217,15 +217,15 @@ def test_server_alone_equals_mix(mode):
                 assert a == oracle.reduce(x, 0, "add")
 
 
-def test_server_serial_and_batched_agree_on_exact_jobs():
+def test_server_modes_agree_on_exact_jobs():
     reqs = [r for r in _requests() if r.kind in ("sort_by_key", "copy_where") or r.dtype != np.float32]
     out = {}
-    for mode in ("batched", "serial"):
+    for mode in ("batched", "serial", "per_job"):
         with serve.Server(streams=3, max_batch=32, mode=mode, hbm_budget=8 << 30) as srv:
             ts = [srv.submit(r) for r in reqs]
             out[mode] = [_fetch(srv, t) for t in ts]
-    for r, a, b in zip(reqs, out["batched"], out["serial"]):
-        assert _same(a, b), (r.kind, r.dtype, r.n)
+    for r, a, b, c in zip(reqs, out["batched"], out["serial"], out["per_job"]):
+        assert _same(a, b) and _same(b, c), (r.kind, r.dtype, r.n)
 
 
 def test_server_budget_admission():
