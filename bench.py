@@ -364,6 +364,13 @@ def w_seg(c):
         log(f"[bench]   tuned {d['tuned_config']}, candidates (block x ipt x digit bits): {cands}")
         out["sort_u32_f32" + tag] = d
     c.ex.set_option(c.ex.OPT_SEG_LAUNCH, c.ex.SEG_LAUNCH_PERSISTENT)
+    # The short segments packed into tile-sized windows instead of sorted
+    # inside one wave each (the default), for comparison.
+    c.ex.set_option(c.ex.OPT_SEGSORT_WAVE, 0)
+    ms = timed(c.ex, run_sort, c.steps, 1)
+    out["sort_u32_f32_windows"] = summarize("seg_sort_u32key_f32val_2^26_in_2^20_windows", ms,
+                                            16.0 * n + 4.0 * (S + 1), n)
+    c.ex.set_option(c.ex.OPT_SEGSORT_WAVE, 1)
     lib = rocprim_lib()
     if lib is not None:
         # Vendor yardstick: rocPRIM segmented_radix_sort_pairs on the same

@@ -141,12 +141,18 @@ msv_status msv_executor_trim(msv_executor* ex);
  *  MSV_OPT_RADIX_ONESWEEP: the radix sorts' schedule: 1 = one histogram pass
  *    and one look-back pass per digit (onesweep); 0 = a counting upsweep, a
  *    scan of the counts and a downsweep per digit (reduce-then-scan); -1
- *    (default) = onesweep for 4-byte keys, reduce-then-scan for 8-byte keys. */
+ *    (default) = onesweep for 4-byte keys, reduce-then-scan for 8-byte keys.
+ *  MSV_OPT_SEGSORT_WAVE: 1 (default) seg::Sort sorts segments of up to 1024
+ *    elements inside one wave each (bitonic network in registers), longer
+ *    ones up to the tile size one block each with the tile shapes above;
+ *    0 packs the short segments into tile-sized windows instead. Either way
+ *    segments longer than max(tile, 4096) are sorted by many blocks each. */
 typedef enum msv_option {
   MSV_OPT_SEG_LAUNCH = 1,
   MSV_OPT_AUTOTUNE = 2,
   MSV_OPT_SEGSORT_CONFIG = 3,
-  MSV_OPT_RADIX_ONESWEEP = 4
+  MSV_OPT_RADIX_ONESWEEP = 4,
+  MSV_OPT_SEGSORT_WAVE = 5
 } msv_option;
 #define MSV_SEG_LAUNCH_PERSISTENT 0
 #define MSV_SEG_LAUNCH_PER_CLASS 1

@@ -83,6 +83,12 @@ struct msv_executor {
   int device;
   int num_cus;
   hipStream_t stream;
+  // Second stream for work one call runs beside its main work (created on
+  // first use, default priority), and the events that fork it from and join
+  // it back to `stream`.
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t fork_event = nullptr;
+  hipEvent_t join_event = nullptr;
   std::atomic<int> refs;
   bool destroyed;
   msv::Allocator allocator;
@@ -98,6 +104,7 @@ struct msv_executor {
   int64_t opt_autotune = 1;
   int64_t opt_segsort_config = -1;
   int64_t opt_radix_onesweep = -1;
+  int64_t opt_segsort_wave = 1;
   // Resident blocks per CU of each batched job-kernel family (0 = not yet
   // queried). Per executor, so per device, and touched only by the thread
   // that submits on this executor.
@@ -160,6 +167,13 @@ class Ctx {
   explicit Ctx(msv_executor* ex) : ex_(ex) {}
   ~Ctx();
   hipStream_t stream() const { return ex_->stream; }
+  // Makes the auxiliary stream wait for the work enqueued on stream() so far
+  // and returns it (nullptr + error on failure). Every fork is followed by a
+  // join before the call returns; scratch and look-back state the forked work
+  // uses are requested before the fork.
+  hipStream_t fork();
+  // Makes stream() wait for the work enqueued on the auxiliary stream.
+  msv_status join();
   int num_cus() const { return ex_->num_cus; }
   msv_executor* ex() const { return ex_; }
   // nullptr + error on failure.

@@ -100,6 +100,12 @@ void executor_release(msv_executor* ex) {
   if (ex->d_status) (void)hipFree(ex->d_status);
   if (ex->d_counter) (void)hipFree(ex->d_counter);
   ex->allocator.trim();
+  if (ex->aux_stream) {
+    (void)hipStreamSynchronize(ex->aux_stream);
+    (void)hipStreamDestroy(ex->aux_stream);
+  }
+  if (ex->fork_event) (void)hipEventDestroy(ex->fork_event);
+  if (ex->join_event) (void)hipEventDestroy(ex->join_event);
   (void)hipStreamDestroy(ex->stream);
   delete ex;
 }
@@ -188,6 +194,29 @@ msv_status new_vec(msv_executor* ex, msv_dtype dtype, uint32_t capacity, bool de
   v->d_len = dl;
   executor_retain(ex);
   *out = v;
+  return MSV_OK;
+}
+
+hipStream_t Ctx::fork() {
+  msv_executor* ex = ex_;
+  hipError_t e = hipSuccess;
+  if (!ex->aux_stream) e = hipStreamCreateWithFlags(&ex->aux_stream, hipStreamNonBlocking);
+  if (e == hipSuccess && !ex->fork_event)
+    e = hipEventCreateWithFlags(&ex->fork_event, hipEventDisableTiming);
+  if (e == hipSuccess && !ex->join_event)
+    e = hipEventCreateWithFlags(&ex->join_event, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(ex->fork_event, ex->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(ex->aux_stream, ex->fork_event, 0);
+  if (e != hipSuccess) {
+    set_error(MSV_ERR_LAUNCH, 0, 0, "stream fork failed: %s", hipGetErrorString(e));
+    return nullptr;
+  }
+  return ex->aux_stream;
+}
+
+msv_status Ctx::join() {
+  MSV_HIP(hipEventRecord(ex_->join_event, ex_->aux_stream));
+  MSV_HIP(hipStreamWaitEvent(ex_->stream, ex_->join_event, 0));
   return MSV_OK;
 }
 
@@ -493,6 +522,10 @@ msv_status msv_executor_set_option(msv_executor* ex, msv_option opt, int64_t val
       if (value < -1 || value > 1) break;
       ex->opt_radix_onesweep = value;
       return MSV_OK;
+    case MSV_OPT_SEGSORT_WAVE:
+      if (value != 0 && value != 1) break;
+      ex->opt_segsort_wave = value;
+      return MSV_OK;
   }
   set_error(MSV_ERR_INVALID_ARGUMENT, (uint64_t)opt, (uint64_t)value, "bad option %d = %lld",
             (int)opt, (long long)value);
@@ -510,6 +543,7 @@ msv_status msv_executor_get_option(msv_executor* ex, msv_option opt, int64_t* va
     case MSV_OPT_AUTOTUNE: *value = ex->opt_autotune; return MSV_OK;
     case MSV_OPT_SEGSORT_CONFIG: *value = ex->opt_segsort_config; return MSV_OK;
     case MSV_OPT_RADIX_ONESWEEP: *value = ex->opt_radix_onesweep; return MSV_OK;
+    case MSV_OPT_SEGSORT_WAVE: *value = ex->opt_segsort_wave; return MSV_OK;
   }
   set_error(MSV_ERR_INVALID_ARGUMENT, (uint64_t)opt, 0, "bad option %d", (int)opt);
   return MSV_ERR_INVALID_ARGUMENT;

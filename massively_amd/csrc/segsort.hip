@@ -9,10 +9,13 @@
 //    keeps the rest contiguous). The range is sorted in LDS by key and then
 //    by segment slot, which restores the segment grouping;
 //  * big segments (W < L <= TILE): one segment per tile in LDS;
-//  * huge segments (L > TILE): one block sorts the segment through global
-//    memory, a tile-sized chunk at a time per digit pass.
+//  * long segments (L > max(TILE, 4096)): many blocks per segment, an LSD
+//    radix sort over all long segments at once (segsort_huge.hpp); with a
+//    tile smaller than 4096, segments in between are sorted by one block
+//    through global memory, a tile-sized chunk at a time per digit pass.
 // A plan kernel finds every window's segments by binary search over the
-// offsets and appends the big and huge segments to a list. The sort then runs
+// offsets and appends the big segments to one list and the long ones to
+// another. The sort then runs
 // either as one persistent launch (blocks take items from a queue, list items
 // first because they are the longest) or as one launch per size class. The
 // tile shape (threads, keys per thread, digit width) is one of a few compiled
@@ -22,6 +25,8 @@
 #include <mutex>
 
 #include "msv_internal.hpp"
+#include "segsort_huge.hpp"
+#include "segsort_small.hpp"
 #include "tile_sort.hpp"
 
 namespace msv {
@@ -56,8 +61,10 @@ __device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t m, uint64
 }
 
 __global__ void segsort_plan_kernel(const uint32_t* __restrict__ offsets, uint32_t nseg,
-                                    uint32_t W, uint32_t nwin, uint2* __restrict__ win,
-                                    uint32_t* __restrict__ big, uint32_t* __restrict__ counters) {
+                                    uint32_t W, uint32_t nwin, uint32_t long_min,
+                                    uint2* __restrict__ win, uint32_t* __restrict__ big,
+                                    uint32_t* __restrict__ counters, uint32_t* __restrict__ longs,
+                                    uint32_t* __restrict__ long_count) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nwin) return;
   const uint64_t lo = (uint64_t)k * W;
@@ -65,8 +72,10 @@ __global__ void segsort_plan_kernel(const uint32_t* __restrict__ offsets, uint32
   uint32_t end = lower_bound_u32(offsets, nseg, lo + W);
   if (end > first) {
     const uint32_t last = end - 1;
-    if (offsets[last + 1] - offsets[last] > W) {
-      big[atomicAdd(&counters[0], 1u)] = last;
+    const uint32_t L = offsets[last + 1] - offsets[last];
+    if (L > W) {
+      if (L >= long_min) longs[atomicAdd(long_count, 1u)] = last;
+      else big[atomicAdd(&counters[0], 1u)] = last;
       end = last;
     }
   }
@@ -152,25 +161,219 @@ constexpr int kSegSortDefault = 1;
 // A tile of `tile` elements of EW dwords must fit the LDS budget of a block.
 constexpr bool segsort_fits(int ew, int tile) { return (size_t)tile * ew * 4 <= 65536; }
 
+// Long segments (listed in `longs`, their number in ctl[HUGE_NSEG] on the
+// device) sorted by many blocks each (segsort_huge.hpp). `long_min` is the
+// shortest length that can be listed, which bounds the lists' sizes. The
+// plan holds everything the launches need, set up on the host first (scratch
+// and one look-back generation per pass), so that the launches can go to a
+// forked stream.
+template <int KW>
+struct LongPlan {
+  static constexpr int P = KW * 4;
+  HugeArgs h;
+  uint64_t* status[P];
+  uint32_t gen[P];
+  uint32_t tiles_max = 0;
+  uint32_t nlong_max = 0;
+};
+
+template <int KW, int EW>
+static msv_status long_prepare(Ctx& c, const SegSortArgs& base, uint32_t n, uint32_t long_min,
+                               const uint32_t* longs, uint32_t* ctl, void* s0, void* s1,
+                               LongPlan<KW>* lp) {
+  constexpr int P = KW * 4;
+  lp->nlong_max = n / long_min;
+  if (lp->nlong_max == 0) return MSV_OK;
+  lp->tiles_max = div_up(n, kHugeTile) + lp->nlong_max;
+  uint32_t* tile_begin = (uint32_t*)c.scratch(sizeof(uint32_t) * lp->nlong_max);
+  uint32_t* tile_seg = (uint32_t*)c.scratch(sizeof(uint32_t) * lp->tiles_max);
+  uint32_t* hist = (uint32_t*)c.scratch(sizeof(uint32_t) * (size_t)lp->nlong_max * P * kRadixBins);
+  if (!tile_begin || !tile_seg || !hist) return MSV_ERR_OUT_OF_MEMORY;
+  HugeArgs& h = lp->h;
+  h.keys = base.keys;
+  h.vals = base.vals;
+  h.out_k = base.out_k;
+  h.out_v = base.out_v;
+  h.offsets = base.offsets;
+  h.list = longs;
+  h.ctl = ctl;
+  h.tile_begin = tile_begin;
+  h.tile_seg = tile_seg;
+  h.hist = hist;
+  h.s0 = s0;
+  h.s1 = s1;
+  h.om = base.om;
+  for (int p = 0; p < P; ++p) {
+    uint64_t* counter;
+    MSV_TRY(c.lookback((size_t)lp->tiles_max * kRadixBins, &counter, &lp->status[p], &lp->gen[p]));
+  }
+  return MSV_OK;
+}
+
+template <int KW, int EW>
+static msv_status long_launch(Ctx& c, hipStream_t st, const LongPlan<KW>& lp) {
+  constexpr int P = KW * 4;
+  if (lp.nlong_max == 0) return MSV_OK;
+  const HugeArgs& h = lp.h;
+  hipLaunchKernelGGL((huge_layout_kernel<P>), dim3(1), dim3(256), 0, st, h);
+  const uint32_t zgrid = std::min<uint32_t>(div_up(lp.nlong_max * P * kRadixBins, 256u * 8u), 1024u);
+  hipLaunchKernelGGL((huge_zero_kernel<P>), dim3(std::max(1u, zgrid)), dim3(256), 0, st, h);
+  const uint32_t ncu = (uint32_t)c.num_cus();
+  // About 8 consecutive tiles per histogram block (fewer when there are
+  // fewer tiles than blocks the device holds at once).
+  hipLaunchKernelGGL((huge_hist_kernel<KW, EW>),
+                     dim3(std::max(1u, std::min(div_up(lp.tiles_max, 8u), ncu * 4u))),
+                     dim3(kHugeBlock), 0, st, h);
+  hipLaunchKernelGGL((huge_starts_kernel<P>), dim3(std::min(lp.nlong_max, ncu)), dim3(kRadixBins),
+                     0, st, h);
+  MSV_HIP(hipGetLastError());
+  static int per_cu = 0;  // resident blocks per CU of this instantiation
+  if (per_cu == 0) {
+    int b = 0;
+    MSV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, huge_pass_kernel<KW, EW>,
+                                                         kHugeBlock, 0));
+    per_cu = std::max(1, b);
+  }
+  const uint32_t pgrid = std::min<uint32_t>(lp.tiles_max, (uint32_t)per_cu * ncu);
+  for (int p = 0; p < P; ++p) {
+    hipLaunchKernelGGL((huge_pass_kernel<KW, EW>), dim3(pgrid), dim3(kHugeBlock), 0, st, h, p,
+                       lp.status[p], lp.gen[p]);
+    MSV_HIP(hipGetLastError());
+  }
+  return MSV_OK;
+}
+
+// Shortest segment of the long path for a block tile of `tile` elements.
+constexpr uint32_t seg_long_min(uint32_t tile) {
+  return (tile > kHugeTile ? tile : kHugeTile) + 1u;
+}
+
+// Segments of up to 1024 elements sorted inside one wave
+// (segsort_small.hpp), those up to the long limit by the block path (one tile
+// per segment, or the one-block range sort), the long ones by the long path,
+// all from lists the classify and fill kernels build on the device.
+template <int KW, int EW, int BLOCK, int IPT, int DBITS>
+static msv_status segsort_wave_go(Ctx& c, const SegSortArgs& base, uint32_t n, uint32_t nseg,
+                                  const uint32_t* offsets) {
+  constexpr int VW = EW - KW;
+  if constexpr (!segsort_fits(EW, BLOCK * IPT)) {
+    set_error(MSV_ERR_UNSUPPORTED, 0, 0, "tile shape does not fit this element width");
+    return MSV_ERR_UNSUPPORTED;
+  } else {
+    constexpr uint32_t TILE = (uint32_t)BLOCK * IPT;
+    const uint32_t long_min = seg_long_min(TILE);
+    // ctl: [0, 13) class counts, [16, 29) cursors ([27] = the block path's
+    // count, [28] its queue, never a cursor), [32, 52) the long path's words.
+    uint32_t* ctl = (uint32_t*)c.scratch(sizeof(uint32_t) * 64);
+    uint2* list = (uint2*)c.scratch(sizeof(uint2) * (size_t)nseg);
+    uint32_t* big = (uint32_t*)c.scratch(sizeof(uint32_t) * (size_t)nseg);
+    const uint32_t nlong_max = n / long_min;
+    uint32_t* longs = (uint32_t*)c.scratch(sizeof(uint32_t) * std::max(1u, nlong_max));
+    if (!ctl || !list || !big || !longs) return MSV_ERR_OUT_OF_MEMORY;
+    uint32_t* cursors = ctl + 16;
+    uint32_t* hctl = ctl + 32;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, c.stream(), ctl, 64u, 0u);
+    const uint32_t sblocks = div_up(nseg, 256u * kClassifySpt);
+    hipLaunchKernelGGL(seg_classify_kernel, dim3(sblocks), dim3(256), 0, c.stream(), offsets, nseg,
+                       long_min, ctl);
+    hipLaunchKernelGGL(seg_fill_kernel, dim3(sblocks), dim3(256), 0, c.stream(), offsets, nseg,
+                       long_min, ctl, cursors, list, big, longs, hctl + HUGE_NSEG);
+    MSV_HIP(hipGetLastError());
+    WaveSortArgs w;
+    w.keys = base.keys;
+    w.vals = base.vals;
+    w.out_k = base.out_k;
+    w.out_v = base.out_v;
+    w.list = list;
+    w.counts = ctl;
+    w.om = base.om;
+    // Segments longer than 1024: the block path over its list (no windows)
+    // and the long path, on a forked stream beside the wave sorts (they have
+    // few, long work items; the wave sorts have many short ones).
+    const bool longer = (uint64_t)n > kWaveSortMaxLen;
+    SegSortArgs b = base;
+    LongPlan<KW> lp;
+    if (longer) {
+      void* s0 = c.scratch((size_t)n * EW * 4);
+      void* s1 = c.scratch((size_t)n * EW * 4);
+      if (!s0 || !s1) return MSV_ERR_OUT_OF_MEMORY;
+      b.win = nullptr;
+      b.big = big;
+      b.counters = cursors + kSegClassBlock;  // [0] block-path segments, [1] queue
+      b.s0 = s0;
+      b.s1 = s1;
+      b.nwin = 0;
+      b.klass = SEG_ITEMS_ALL;
+      MSV_TRY((long_prepare<KW, EW>(c, base, n, long_min, longs, hctl, s0, s1, &lp)));
+    }
+    auto kern = segsort_kernel<KW, EW, BLOCK, IPT, DBITS>;
+    static int per_cu = 0;  // resident blocks per CU of this instantiation
+    if (longer && per_cu == 0) {
+      int bpc = 0;
+      MSV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, BLOCK, 0));
+      per_cu = std::max(1, bpc);
+    }
+    hipStream_t aux = nullptr;
+    if (longer) {
+      aux = c.fork();
+      if (!aux) return MSV_ERR_LAUNCH;
+      const uint32_t most = (uint32_t)(n / (kWaveSortMaxLen + 1));  // segments longer than 1024
+      const uint32_t bgrid = std::min<uint32_t>(std::max(1u, most), (uint32_t)(per_cu * c.num_cus()));
+      hipLaunchKernelGGL(kern, dim3(bgrid), dim3(BLOCK), 0, aux, b);
+      MSV_HIP(hipGetLastError());
+      const msv_status ls = long_launch<KW, EW>(c, aux, lp);
+      if (ls != MSV_OK) {
+        (void)c.join();
+        return ls;
+      }
+    }
+    // Enough waves to fill the device; each walks its share of the units.
+    const uint32_t ncu = (uint32_t)c.num_cus();
+    const uint32_t sgrid = std::max(1u, std::min(div_up(nseg, kWaveSortWaves), ncu * 8u));
+    hipLaunchKernelGGL((wave_sort_large_kernel<KW, VW, 16>), dim3(sgrid), dim3(kWaveSortBlock), 0,
+                       c.stream(), w);
+    hipLaunchKernelGGL((wave_sort_large_kernel<KW, VW, 8>), dim3(sgrid), dim3(kWaveSortBlock), 0,
+                       c.stream(), w);
+    hipLaunchKernelGGL((wave_sort_large_kernel<KW, VW, 4>), dim3(sgrid), dim3(kWaveSortBlock), 0,
+                       c.stream(), w);
+    hipLaunchKernelGGL((wave_sort_large_kernel<KW, VW, 2>), dim3(sgrid), dim3(kWaveSortBlock), 0,
+                       c.stream(), w);
+    hipLaunchKernelGGL((wave_sort_small_kernel<KW, VW>), dim3(sgrid), dim3(kWaveSortBlock), 0,
+                       c.stream(), w);
+    const hipError_t le = hipGetLastError();
+    if (longer) MSV_TRY(c.join());
+    MSV_HIP(le);
+    return MSV_OK;
+  }
+}
+
 template <int KW, int EW, int BLOCK, int IPT, int DBITS>
 static msv_status segsort_go(Ctx& c, const SegSortArgs& base, uint32_t n, uint32_t nseg,
                              const uint32_t* offsets) {
+  if (c.ex()->opt_segsort_wave)
+    return segsort_wave_go<KW, EW, BLOCK, IPT, DBITS>(c, base, n, nseg, offsets);
   if constexpr (!segsort_fits(EW, BLOCK * IPT)) {
     set_error(MSV_ERR_UNSUPPORTED, 0, 0, "tile shape does not fit this element width");
     return MSV_ERR_UNSUPPORTED;
   } else {
     constexpr uint32_t TILE = (uint32_t)BLOCK * IPT;
     constexpr uint32_t W = TILE / 2;
+    const uint32_t long_min = seg_long_min(TILE);
     const uint32_t nwin = (uint32_t)(((uint64_t)n + W - 1) / W);
+    const uint32_t nlong_max = n / long_min;
     uint2* win = (uint2*)c.scratch(sizeof(uint2) * nwin);
     uint32_t* big = (uint32_t*)c.scratch(sizeof(uint32_t) * nwin);
-    uint32_t* counters = (uint32_t*)c.scratch(sizeof(uint32_t) * 2);
+    // [0] big segments, [1] queue, [32, 52) the long path's words.
+    uint32_t* counters = (uint32_t*)c.scratch(sizeof(uint32_t) * 64);
+    uint32_t* longs = (uint32_t*)c.scratch(sizeof(uint32_t) * std::max(1u, nlong_max));
     void* s0 = c.scratch((size_t)n * EW * 4);
     void* s1 = c.scratch((size_t)n * EW * 4);
-    if (!win || !big || !counters || !s0 || !s1) return MSV_ERR_OUT_OF_MEMORY;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, c.stream(), counters, 2u, 0u);
+    if (!win || !big || !counters || !longs || !s0 || !s1) return MSV_ERR_OUT_OF_MEMORY;
+    uint32_t* hctl = counters + 32;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, c.stream(), counters, 64u, 0u);
     hipLaunchKernelGGL(segsort_plan_kernel, dim3((nwin + 255) / 256), dim3(256), 0, c.stream(),
-                       offsets, nseg, W, nwin, win, big, counters);
+                       offsets, nseg, W, nwin, long_min, win, big, counters, longs,
+                       hctl + HUGE_NSEG);
     MSV_HIP(hipGetLastError());
     SegSortArgs a = base;
     a.win = win;
@@ -197,7 +400,9 @@ static msv_status segsort_go(Ctx& c, const SegSortArgs& base, uint32_t n, uint32
       hipLaunchKernelGGL(kern, dim3(std::max(1u, grid)), dim3(BLOCK), 0, c.stream(), a);
     }
     MSV_HIP(hipGetLastError());
-    return MSV_OK;
+    LongPlan<KW> lp;
+    MSV_TRY((long_prepare<KW, EW>(c, base, n, long_min, longs, hctl, s0, s1, &lp)));
+    return long_launch<KW, EW>(c, c.stream(), lp);
   }
 }
 

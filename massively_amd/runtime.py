@@ -69,7 +69,7 @@ class Executor:
         return GraphCapture(self)
 
     # Tuning options of the batching layer (results never depend on them).
-    OPT_SEG_LAUNCH, OPT_AUTOTUNE, OPT_SEGSORT_CONFIG, OPT_RADIX_ONESWEEP = 1, 2, 3, 4
+    OPT_SEG_LAUNCH, OPT_AUTOTUNE, OPT_SEGSORT_CONFIG, OPT_RADIX_ONESWEEP, OPT_SEGSORT_WAVE = 1, 2, 3, 4, 5
     SEG_LAUNCH_PERSISTENT, SEG_LAUNCH_PER_CLASS = 0, 1
 
     def set_option(self, opt: int, value: int) -> None:

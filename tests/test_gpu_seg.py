@@ -174,9 +174,12 @@ def test_batched_seg_config_2p26(exec_):
 
 
 def _mixed_lengths():
-    # Tiny and empty segments, segments around every candidate's half tile and
+    # Tiny and empty segments, every power-of-two size class boundary of the
+    # wave sorts (1 .. 1025), segments around every candidate's half tile and
     # tile (512 .. 8192), and segments longer than any tile.
     L = list(RNG.integers(0, 40, 2000))
+    for b in range(11):
+        L += [(1 << b) - 1, 1 << b, (1 << b) + 1]
     for t in (512, 1024, 2048, 4096, 8192):
         L += [t // 2 - 1, t // 2, t // 2 + 1, 0, t - 1, t, t + 1, 3]
     L += [20000, 1, 0, 65536, 2, 9000]
@@ -184,24 +187,30 @@ def _mixed_lengths():
     return L
 
 
+@pytest.mark.parametrize("wave", [1, 0])
 @pytest.mark.parametrize("launch", [m.Executor.SEG_LAUNCH_PERSISTENT, m.Executor.SEG_LAUNCH_PER_CLASS])
 @pytest.mark.parametrize("kdtype,vdtype", [(np.uint32, np.float32), (np.int32, None), (np.float64, np.uint64),
                                            (np.uint64, np.uint32)])
-def test_seg_sort_every_tile_config(exec_, launch, kdtype, vdtype):
-    """Every tile-shape candidate and both launch forms give the oracle's order."""
+def test_seg_sort_every_tile_config(exec_, wave, launch, kdtype, vdtype):
+    """Every tile-shape candidate and both launch forms give the oracle's order,
+    with short segments sorted inside a wave (wave=1) or packed into windows
+    (wave=0). Half the keys come from 5 values, so ties (stability) are common."""
     off = offsets_of(_mixed_lengths())
     n = int(off[-1])
     if np.dtype(kdtype).kind == "f":
         keys = np.round(RNG.standard_normal(n) * 50).astype(kdtype)
         keys[::7] = -0.0
+        keys[1::2] = np.round(keys[1::2] / 40)
     else:
         keys = RNG.integers(0, 1 << 20, n).astype(kdtype)
+        keys[1::2] = RNG.integers(0, 5, keys[1::2].size).astype(kdtype)
     vals = None if vdtype is None else RNG.integers(0, 2**32, n).astype(vdtype)
     sg = seg.Segmentation.from_offsets(exec_, exec_.to_device(off))
     dk = exec_.to_device(keys)
     cols = dk if vals is None else (dk, exec_.to_device(vals))
     ran = 0
     try:
+        exec_.set_option(m.Executor.OPT_SEGSORT_WAVE, wave)
         exec_.set_option(m.Executor.OPT_SEG_LAUNCH, launch)
         for k in range(len(m.segsort_configs())):
             exec_.set_option(m.Executor.OPT_SEGSORT_CONFIG, k)
@@ -219,7 +228,52 @@ def test_seg_sort_every_tile_config(exec_, launch, kdtype, vdtype):
     finally:
         exec_.set_option(m.Executor.OPT_SEGSORT_CONFIG, -1)
         exec_.set_option(m.Executor.OPT_SEG_LAUNCH, m.Executor.SEG_LAUNCH_PERSISTENT)
+        exec_.set_option(m.Executor.OPT_SEGSORT_WAVE, 1)
     assert ran >= 2
+
+
+@pytest.mark.parametrize("wave", [1, 0])
+@pytest.mark.parametrize("kdtype,vdtype", [(np.uint32, np.float32), (np.int64, None), (np.float64, np.uint64)])
+def test_seg_sort_long_segments(exec_, wave, kdtype, vdtype):
+    """Segments longer than a block's tile, sorted by many blocks each: many
+    long segments between short ones, one of 300000 elements, keys that vary
+    in the low byte only (the other digit passes are skipped), and a segment
+    set whose keys are all equal (every pass skipped but the data still moves)."""
+    L = []
+    for _ in range(40):
+        L += list(RNG.integers(0, 50, 5)) + [int(RNG.integers(4097, 30000))]
+    L += [300000, 7, 4096, 4097, 8192, 8193, 0]
+    off = offsets_of(L)
+    n = int(off[-1])
+    kinds = {"random": None, "low_byte": None, "equal": None}
+    for kind in kinds:
+        if np.dtype(kdtype).kind == "f":
+            keys = np.round(RNG.standard_normal(n) * 1e3).astype(kdtype)
+            keys[::5] = -0.0
+            keys[1::5] = 0.0
+            if kind == "low_byte":
+                keys = RNG.integers(0, 4, n).astype(kdtype)
+        else:
+            keys = RNG.integers(-(1 << 40) if np.dtype(kdtype).kind == "i" else 0, 1 << 31, n).astype(kdtype)
+            if kind == "low_byte":
+                keys = RNG.integers(0, 256, n).astype(kdtype)
+        if kind == "equal":
+            keys = np.full(n, 3, dtype=kdtype)
+        vals = None if vdtype is None else RNG.integers(0, 2**32, n).astype(vdtype)
+        sg = seg.Segmentation.from_offsets(exec_, exec_.to_device(off))
+        dk = exec_.to_device(keys)
+        cols = dk if vals is None else (dk, exec_.to_device(vals))
+        try:
+            exec_.set_option(m.Executor.OPT_SEGSORT_WAVE, wave)
+            for mode, cmp in (("less", m.op.Less), ("greater", m.op.Greater)):
+                out = seg.ForEachSegment(seg.Sort(cmp)).run(exec_, seg.SegmentIterator(cols, sg))
+                perm = oracle.seg_sort_perm(keys, off, mode)
+                got = out.values if vals is None else out.values[0]
+                assert exec_.to_host(got).tobytes() == keys[perm].tobytes(), (kind, mode)
+                if vals is not None:
+                    assert exec_.to_host(out.values[1]).tobytes() == vals[perm].tobytes(), (kind, mode)
+        finally:
+            exec_.set_option(m.Executor.OPT_SEGSORT_WAVE, 1)
 
 
 def test_seg_sort_config_2p26_autotuned(exec_):
