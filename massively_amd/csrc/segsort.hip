@@ -338,7 +338,11 @@ static msv_status segsort_wave_go(Ctx& c, const SegSortArgs& base, uint32_t n, u
                        c.stream(), w);
     hipLaunchKernelGGL((wave_sort_large_kernel<KW, VW, 2>), dim3(sgrid), dim3(kWaveSortBlock), 0,
                        c.stream(), w);
-    hipLaunchKernelGGL((wave_sort_small_kernel<KW, VW>), dim3(sgrid), dim3(kWaveSortBlock), 0,
+    hipLaunchKernelGGL((wave_sort_small_kernel<KW, VW, 6, 6>), dim3(sgrid), dim3(kWaveSortBlock), 0,
+                       c.stream(), w);
+    hipLaunchKernelGGL((wave_sort_small_kernel<KW, VW, 5, 5>), dim3(sgrid), dim3(kWaveSortBlock), 0,
+                       c.stream(), w);
+    hipLaunchKernelGGL((wave_sort_small_kernel<KW, VW, 0, 4>), dim3(sgrid), dim3(kWaveSortBlock), 0,
                        c.stream(), w);
     const hipError_t le = hipGetLastError();
     if (longer) MSV_TRY(c.join());

@@ -18,7 +18,8 @@
 // per block, one global add per class), a fill kernel lists them per class
 // as (start, length), the block path's segments (1024 < L <= long limit) and
 // the long ones by id; the one-slot classes run in one launch and the
-// multi-register classes one launch per class, longest class first. Wave w takes work
+// multi-register classes and classes 6 and 5 one launch per class, classes
+// 0..4 in one, longest class first. Wave w takes work
 // units w, w + W, w + 2W, ... (a unit is 64 >> c segments of a one-slot
 // class or one segment of a larger class) and loads the next unit's
 // descriptors before it sorts the current one.
@@ -194,7 +195,7 @@ template <int KW, int LG>
 __device__ __forceinline__ void bitonic_lanes(SortSlot<KW>& v) {
   SortSlot<KW>(&a)[1] = *reinterpret_cast<SortSlot<KW>(*)[1]>(&v);
   uint32_t e0 = __lane_id();
-  asm volatile("" : "+v"(e0));  // per call: keeps the direction masks out of enclosing loops
+  asm volatile("" : "+v"(e0));  // per call (see bitonic_wave)
   bitonic_sort<KW, 2, (1u << LG), 1>(a, e0 & ((1u << LG) - 1u));
 }
 
@@ -202,7 +203,9 @@ __device__ __forceinline__ void bitonic_lanes(SortSlot<KW>& v) {
 template <int KW, int R>
 __device__ __forceinline__ void bitonic_wave(SortSlot<KW> (&v)[R]) {
   uint32_t e0 = __lane_id();
-  asm volatile("" : "+v"(e0));  // per call: keeps the direction masks out of enclosing loops
+  // Per call: hoisted out of the unit loop, the direction masks of all stages
+  // would not fit the scalar registers (they spill to VGPR lanes).
+  asm volatile("" : "+v"(e0));
   bitonic_sort<KW, 2, 64u * R, R>(v, e0);
 }
 
@@ -459,10 +462,11 @@ __device__ __forceinline__ void wave_sort_one_direct(const WaveSortArgs& a,
   }
 }
 
-// Classes 0..6 in one launch, class 6 first. A wave's units are pipelined:
-// descriptors two units ahead, input elements one unit ahead, so the memory
-// latency of the next unit overlaps the current unit's network.
-template <int KW, int VW>
+// Classes CLO..CHI (within 0..6) in one launch, class CHI first. A wave's
+// units are pipelined: descriptors two units ahead, input elements one unit
+// ahead, so the memory latency of the next unit overlaps the current unit's
+// network.
+template <int KW, int VW, int CLO, int CHI>
 __global__ __launch_bounds__(kWaveSortBlock) void wave_sort_small_kernel(WaveSortArgs a) {
   __shared__ uint32_t units[7], starts[7], s_total;
   if (threadIdx.x == 0) {
@@ -472,7 +476,7 @@ __global__ __launch_bounds__(kWaveSortBlock) void wave_sort_small_kernel(WaveSor
       units[c] = (n + wave_sort_unit(c) - 1) / wave_sort_unit(c);
       starts[c] = seg_start;
       seg_start += n;
-      total += units[c];
+      if (c >= (uint32_t)CLO && c <= (uint32_t)CHI) total += units[c];
     }
     s_total = total;
   }
@@ -480,17 +484,19 @@ __global__ __launch_bounds__(kWaveSortBlock) void wave_sort_small_kernel(WaveSor
   const uint32_t total = s_total;
   const uint32_t lane = __lane_id();
   const uint32_t W = gridDim.x * kWaveSortWaves;
-  // Unit u counts from class 6 down: its class, and this lane's descriptor
+  // Unit u counts from class CHI down: its class, and this lane's descriptor
   // (length 0 past the unit's segments and past the last unit).
   auto locate = [&](uint32_t u, uint32_t* cls) -> uint2 {
     if (u >= total) {
-      *cls = 0;
+      *cls = CLO;
       return make_uint2(0u, 0u);
     }
-    uint32_t c = 6;
-    while (u >= units[c]) {
-      u -= units[c];
-      --c;
+    uint32_t c = CHI;
+    if constexpr (CLO < CHI) {
+      while (u >= units[c]) {
+        u -= units[c];
+        --c;
+      }
     }
     *cls = c;
     const uint32_t per = wave_sort_unit(c);
@@ -507,14 +513,18 @@ __global__ __launch_bounds__(kWaveSortBlock) void wave_sort_small_kernel(WaveSor
     const LaneElem<KW, VW> x1 = load_lane_elem<KW, VW>(a, c1, d1);
     uint32_t c2;
     const uint2 d2 = locate(u0 + 2 * W, &c2);
-    switch (c0) {
-      case 0: wave_sort_groups<KW, VW, 0>(a, d0, x0); break;
-      case 1: wave_sort_groups<KW, VW, 1>(a, d0, x0); break;
-      case 2: wave_sort_groups<KW, VW, 2>(a, d0, x0); break;
-      case 3: wave_sort_groups<KW, VW, 3>(a, d0, x0); break;
-      case 4: wave_sort_groups<KW, VW, 4>(a, d0, x0); break;
-      case 5: wave_sort_groups<KW, VW, 5>(a, d0, x0); break;
-      default: wave_sort_groups<KW, VW, 6>(a, d0, x0); break;
+    if constexpr (CLO == CHI) {
+      wave_sort_groups<KW, VW, CHI>(a, d0, x0);
+    } else {
+      switch (c0) {
+        case 0: wave_sort_groups<KW, VW, 0>(a, d0, x0); break;
+        case 1: wave_sort_groups<KW, VW, 1>(a, d0, x0); break;
+        case 2: wave_sort_groups<KW, VW, 2>(a, d0, x0); break;
+        case 3: wave_sort_groups<KW, VW, 3>(a, d0, x0); break;
+        case 4: wave_sort_groups<KW, VW, 4>(a, d0, x0); break;
+        case 5: wave_sort_groups<KW, VW, 5>(a, d0, x0); break;
+        default: wave_sort_groups<KW, VW, 6>(a, d0, x0); break;
+      }
     }
     u0 += W;
     c0 = c1;
