@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kCmpBlock) void seg_compact_kernel(
     if (s < nseg) {
       const uint32_t st = offsets[s];
       if (st < tile_end) {
-        more = true;
+        more = tid == kCmpBlock - 1;  // (sorted starts)
         atomicOr(&s_heads[(st - (uint32_t)base) >> 5], 1u << ((st - (uint32_t)base) & 31u));
       }
     }
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kCmpBlock) void seg_compact_kernel(
     if (s < nseg) {
       const uint32_t st = offsets[s];
       if (st < tile_end) {
-        more = true;
+        more = tid == kCmpBlock - 1;  // (sorted starts)
         const uint32_t j = st - (uint32_t)base;
         out_offsets[s] = prefix + s_wpre[j >> 5] +
                          (uint32_t)__builtin_popcount(s_keep[j >> 5] & ((1u << (j & 31u)) - 1u));

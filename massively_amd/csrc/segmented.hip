@@ -36,7 +36,13 @@ enum SegOut : int { SEG_INCL = 0, SEG_EXCL = 1, SEG_REDUCE = 2, SEG_RUNS = 3 };
 // occupancy. 512-thread blocks ran the seg reduce and reduce_by_key 3-7%
 // faster than 256-thread blocks with the same tiles.
 __host__ __device__ constexpr int seg_rows(int out) { return out == SEG_RUNS ? 2 : 4; }
-enum SegMode : int { SEG_LOOKBACK = 0, SEG_AGGREGATE = 1, SEG_PREFIXED = 2 };
+// SEG_LOCAL (float Reduce, one read of the values): the tile's segments are
+// combined from an empty carry; segments that start and end in the tile are
+// written, the tile's aggregate is stored, and the segment that enters the
+// tile from before and ends in it leaves (segment, partial up to its end)
+// for seg_span_fixup_kernel, which adds the carry from the scan of the tile
+// aggregates.
+enum SegMode : int { SEG_LOOKBACK = 0, SEG_AGGREGATE = 1, SEG_PREFIXED = 2, SEG_LOCAL = 3 };
 
 template <typename T, int OP>
 __device__ inline void seg_combine(uint32_t& f, T& v, uint32_t f2, T v2) {
@@ -68,6 +74,9 @@ struct RunsArgs {
   uint32_t* len_vals = nullptr;
   uint64_t* cstatus = nullptr;  // look-back form: run-count tile status
   uint32_t* tile_c = nullptr;   // fixed-shape form: tile run counts, then their exclusive prefixes
+  const uint32_t* tile_first = nullptr;  // offsets form: per tile, the last segment starting before it
+  uint32_t* span = nullptr;     // SEG_LOCAL: per tile, the entering segment that ends in it (or ~0)
+  void* lead = nullptr;         // SEG_LOCAL: per tile, that segment's partial within the tile
 };
 
 // V consecutive elements from p + i (16-byte aligned when full).
@@ -122,15 +131,26 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
   const size_t base = (size_t)tile * TILE;
   const uint32_t tile_end = (uint32_t)min((size_t)n, base + TILE);
   const KT* __restrict__ hk = (const KT*)ra.keys;
+  const bool vec_ok = aligned && (base + TILE <= n);
+  T x[kSegRows][V];
+  // The values first: their latency overlaps the search for the tile's
+  // segment starts below.
+#pragma unroll
+  for (int r = 0; r < kSegRows; ++r)
+    load_run<T, V>(in, base + ((uint32_t)r * kSegBlock + tid) * V, n, vec_ok, x[r]);
   uint32_t s0 = 0;
   if constexpr (!KEYS) {
     for (uint32_t i = tid; i < TILE / 32; i += kSegBlock) s_heads[i] = 0;
-    if (wid == 0) {
-      const uint32_t lb = wave_lower_bound(offsets, 0, nseg + 1, (uint32_t)base);
-      if (lane == 0) s_first = lb > 0 ? lb - 1 : 0;
+    if (ra.tile_first) {
+      s0 = ra.tile_first[tile];
+    } else {
+      if (wid == 0) {
+        const uint32_t lb = wave_lower_bound(offsets, 0, nseg + 1, (uint32_t)base);
+        if (lane == 0) s_first = lb > 0 ? lb - 1 : 0;
+      }
+      __syncthreads();
+      s0 = s_first;
     }
-    __syncthreads();
-    s0 = s_first;
     // Mark the segment starts that fall inside the tile.
     for (uint32_t c = 0;; c += kSegBlock) {
       const uint32_t s = s0 + c + tid;
@@ -138,7 +158,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
       if (s < nseg) {
         const uint32_t st = offsets[s];
         if (st < tile_end) {
-          more = true;
+          more = tid == kSegBlock - 1;  // (sorted starts)
           if (st >= base) atomicOr(&s_heads[(st - base) >> 5], 1u << ((st - base) & 31u));
         }
       }
@@ -146,16 +166,13 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
     }
   }
 
-  const bool vec_ok = aligned && (base + TILE <= n);
   const T id = C::identity();
-  T x[kSegRows][V];
   uint32_t hb[kSegRows];    // bit k: element k of the thread's vector in row r starts a segment
   uint32_t ends[kSegRows];  // SEG_RUNS: bit k = element k of the vector ends a run
 #pragma unroll
   for (int r = 0; r < kSegRows; ++r) {
     const uint32_t li = ((uint32_t)r * kSegBlock + tid) * V;
     const size_t gi = base + li;
-    load_run<T, V>(in, gi, n, vec_ok, x[r]);
     if constexpr (KEYS) {
       KT kv[V];
       load_run<KT, V>(hk, gi, n, vec_ok, kv);
@@ -283,6 +300,14 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
       if constexpr (RUNS) ra.tile_c[tile] = ac;
     }
     return;
+  } else if constexpr (MODE == SEG_LOCAL) {
+    static_assert(OUT == SEG_REDUCE, "the local form serves Reduce");
+    if (tid == 0) {
+      tile_f[tile] = af;
+      tile_v[tile] = av;
+      s_cf = 0;
+      s_cv = id;
+    }
   } else if constexpr (MODE == SEG_LOOKBACK) {
     if (wid == 0) {
       TileStatus<T> st{status, gen};
@@ -429,16 +454,34 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
       if (s < nseg) {
         const uint32_t st = offsets[s];
         if (st < tile_end) {
-          more = true;
+          more = tid == kSegBlock - 1;  // (sorted starts)
           const uint32_t en = offsets[s + 1];
           if (en == st) {
             if (st >= base) out[s] = init;
-          } else if (en - 1 >= base && en - 1 < tile_end) {
+          } else if (en - 1 >= base && en - 1 < tile_end && (MODE != SEG_LOCAL || st >= base)) {
             out[s] = C::apply(init, s_vals[en - 1 - base]);
           }
         }
       }
       if (!__syncthreads_or(more)) break;
+    }
+    if constexpr (MODE == SEG_LOCAL) {
+      // The segment holding element `base` when no segment starts there
+      // (s0: the last segment starting before the tile) and whether it ends
+      // in this tile.
+      if (tid == 0) {
+        uint32_t sp = ~0u;
+        T ld = id;
+        if (!(s_heads[0] & 1u)) {
+          const uint32_t en = offsets[s0 + 1];
+          if (en <= tile_end) {
+            sp = s0;
+            ld = s_vals[en - 1 - base];
+          }
+        }
+        ra.span[tile] = sp;
+        ((T*)ra.lead)[tile] = ld;
+      }
     }
     // Trailing empty segments (start == n) belong to the last tile.
     if (tile == ntiles - 1) {
@@ -510,6 +553,33 @@ __global__ __launch_bounds__(1024) void seg_pairs_exclusive_kernel(
   }
 }
 
+// first[t] = the last segment starting before tile t (the non-empty segment
+// holding element t * tile - 1), first[0] = 0: one thread per segment writes
+// the tiles whose start lies in (its start, its end].
+__global__ void seg_tile_first_kernel(const uint32_t* __restrict__ offsets, uint32_t nseg,
+                                      uint32_t tile, uint32_t ntiles, uint32_t* __restrict__ first) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) first[0] = 0;
+  if (s >= nseg) return;
+  const uint32_t lo = offsets[s], hi = offsets[s + 1];
+  if (lo == hi) return;
+  const uint32_t t1 = min(hi / tile, ntiles - 1);
+  for (uint32_t t = lo / tile + 1; t <= t1; ++t) first[t] = s;
+}
+
+// The segments that cross into a tile and end in it (SEG_LOCAL): init (+)
+// (carry from the earlier tiles (+) partial within the tile).
+template <typename T, int OP>
+__global__ void seg_span_fixup_kernel(const uint32_t* __restrict__ span, const T* __restrict__ lead,
+                                      const T* __restrict__ carry, uint32_t ntiles,
+                                      T* __restrict__ out, T init) {
+  using C = Combine<T, OP>;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const uint32_t s = span[t];
+  if (s != ~0u) out[s] = C::apply(init, C::apply(carry[t], lead[t]));
+}
+
 template <typename T>
 __global__ void fill_kernel(T* __restrict__ p, uint32_t n, T v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -558,6 +628,15 @@ static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
   else if (outk == SEG_EXCL) MSV_SEG_LAUNCH(MODE, SEG_EXCL, ST, TF, TV);   \
   else if constexpr (KEYS) MSV_SEG_LAUNCH(MODE, SEG_RUNS, ST, TF, TV);     \
   else MSV_SEG_LAUNCH(MODE, SEG_REDUCE, ST, TF, TV);
+  if constexpr (!KEYS) {
+    if (nseg > 0 && ntiles > 1) {
+      uint32_t* first = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
+      if (!first) return MSV_ERR_OUT_OF_MEMORY;
+      hipLaunchKernelGGL(seg_tile_first_kernel, dim3(div_up(nseg, 256u)), dim3(256), 0, c.stream(),
+                         offsets, nseg, TILE, ntiles, first);
+      ra.tile_first = first;
+    }
+  }
   uint32_t gen = 0;
   if constexpr (!IsFloat<T>::value) {
     uint64_t *counter, *status;
@@ -578,6 +657,28 @@ static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
       if (!acnt || !pcnt) return MSV_ERR_OUT_OF_MEMORY;
     }
     if (!af || !av || !pf || !pv) return MSV_ERR_OUT_OF_MEMORY;
+    if (outk == SEG_REDUCE) {
+      // One read of the values: local tile pass, scan of the tile
+      // aggregates, then the segments that cross tile boundaries.
+      uint32_t* span = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
+      T* lead = (T*)c.scratch(sizeof(T) * ntiles);
+      if (!span || !lead) return MSV_ERR_OUT_OF_MEMORY;
+      RunsArgs la = ra;
+      la.span = span;
+      la.lead = lead;
+      if constexpr (!KEYS) {
+        hipLaunchKernelGGL((seg_tile_kernel<T, OP, SEG_LOCAL, SEG_REDUCE, HK>), dim3(ntiles),
+                           dim3(kSegBlock), 0, c.stream(), in, n, offsets, nseg, out, init, aligned,
+                           (uint64_t*)nullptr, 0u, ntiles, af, av, la);
+      }
+      MSV_HIP(hipGetLastError());
+      hipLaunchKernelGGL((seg_pairs_exclusive_kernel<T, OP>), dim3(1), dim3(1024), 0, c.stream(),
+                         af, av, ntiles, pf, pv, (const uint32_t*)nullptr, (uint32_t*)nullptr);
+      hipLaunchKernelGGL((seg_span_fixup_kernel<T, OP>), dim3(div_up(ntiles, 256u)), dim3(256), 0,
+                         c.stream(), span, lead, pv, ntiles, out, init);
+      MSV_HIP(hipGetLastError());
+      return MSV_OK;
+    }
     RunsArgs agg = ra;
     agg.tile_c = acnt;
     // The aggregate pass computes the plain segmented aggregates (its output
