@@ -36,13 +36,7 @@ enum SegOut : int { SEG_INCL = 0, SEG_EXCL = 1, SEG_REDUCE = 2, SEG_RUNS = 3 };
 // occupancy. 512-thread blocks ran the seg reduce and reduce_by_key 3-7%
 // faster than 256-thread blocks with the same tiles.
 __host__ __device__ constexpr int seg_rows(int out) { return out == SEG_RUNS ? 2 : 4; }
-// SEG_LOCAL (float Reduce, one read of the values): the tile's segments are
-// combined from an empty carry; segments that start and end in the tile are
-// written, the tile's aggregate is stored, and the segment that enters the
-// tile from before and ends in it leaves (segment, partial up to its end)
-// for seg_span_fixup_kernel, which adds the carry from the scan of the tile
-// aggregates.
-enum SegMode : int { SEG_LOOKBACK = 0, SEG_AGGREGATE = 1, SEG_PREFIXED = 2, SEG_LOCAL = 3 };
+enum SegMode : int { SEG_LOOKBACK = 0, SEG_AGGREGATE = 1, SEG_PREFIXED = 2 };
 
 template <typename T, int OP>
 __device__ inline void seg_combine(uint32_t& f, T& v, uint32_t f2, T v2) {
@@ -75,8 +69,6 @@ struct RunsArgs {
   uint64_t* cstatus = nullptr;  // look-back form: run-count tile status
   uint32_t* tile_c = nullptr;   // fixed-shape form: tile run counts, then their exclusive prefixes
   const uint32_t* tile_first = nullptr;  // offsets form: per tile, the last segment starting before it
-  uint32_t* span = nullptr;     // SEG_LOCAL: per tile, the entering segment that ends in it (or ~0)
-  void* lead = nullptr;         // SEG_LOCAL: per tile, that segment's partial within the tile
 };
 
 // V consecutive elements from p + i (16-byte aligned when full).
@@ -103,7 +95,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
   constexpr bool KEYS = !std::is_same<HK, void>::value;
   constexpr bool RUNS = OUT == SEG_RUNS;
   static_assert(!RUNS || KEYS, "runs need keys");
-  static_assert(!(KEYS && OUT == SEG_REDUCE), "per-segment reduce takes offsets");
+  static_assert(OUT != SEG_REDUCE, "per-segment reduce: seg_reduce_kernel");
   using KT = typename std::conditional<KEYS, HK, uint32_t>::type;
   constexpr int V = 16 / sizeof(T);
   constexpr int NW = kSegBlock / kWave;
@@ -117,7 +109,6 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
   __shared__ T s_cv;
   __shared__ uint32_t s_cc;
   __shared__ uint32_t s_first;
-  __shared__ T s_vals[OUT == SEG_REDUCE ? TILE : 1];
   // SEG_RUNS: each thread parks its own keys in s_keys. Each wave's part of
   // a row (a chunk of 64 vectors) later compacts its run keys in place in
   // its part of s_keys and its run values in the same part of s_vals, in run
@@ -300,14 +291,6 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
       if constexpr (RUNS) ra.tile_c[tile] = ac;
     }
     return;
-  } else if constexpr (MODE == SEG_LOCAL) {
-    static_assert(OUT == SEG_REDUCE, "the local form serves Reduce");
-    if (tid == 0) {
-      tile_f[tile] = af;
-      tile_v[tile] = av;
-      s_cf = 0;
-      s_cv = id;
-    }
   } else if constexpr (MODE == SEG_LOOKBACK) {
     if (wid == 0) {
       TileStatus<T> st{status, gen};
@@ -385,10 +368,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
       run = head ? x[r][k] : C::apply(run, x[r][k]);
       if constexpr (OUT != SEG_EXCL) y[k] = run;
     }
-    if constexpr (OUT == SEG_REDUCE) {
-#pragma unroll
-      for (int k = 0; k < V; ++k) s_vals[li + k] = y[k];
-    } else if constexpr (RUNS) {
+    if constexpr (RUNS) {
       // Runs started before this vector, then one per head; consecutive lanes
       // write consecutive runs.
       // This wave's chunk of the row: its heads take runs [wb, wb + wh) and
@@ -417,7 +397,8 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
       const uint32_t wh = readlane(wc[r], kWave - 1);
       const uint32_t elast = readlane(ends[r], kWave - 1) >> (V - 1);
       const uint32_t last = cend >= n ? (cstart < n ? 1u : 0u) : elast;
-      if (lane == 0) s_chunk[r][wid] = make_uint4(s_cc + pc[r], wh, h0, wh - h0 + last);    } else {
+      if (lane == 0) s_chunk[r][wid] = make_uint4(s_cc + pc[r], wh, h0, wh - h0 + last);
+    } else {
       if (vec_ok) {
         struct alignas(16) Q { T v[V]; };
         Q q;
@@ -442,52 +423,6 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
       for (uint32_t i = lane; i < ch.y; i += kWave) ok[i] = s_keys[cbase + i];
       T* ov = out + (ch.x + ch.z - 1u);
       for (uint32_t i = lane; i < ch.w; i += kWave) ov[i] = s_rvals[cbase + i];
-    }
-  }
-
-  if constexpr (OUT == SEG_REDUCE) {
-    __syncthreads();
-    // One result per segment whose last element lies in this tile.
-    for (uint32_t c = 0;; c += kSegBlock) {
-      const uint32_t s = s0 + c + tid;
-      bool more = false;
-      if (s < nseg) {
-        const uint32_t st = offsets[s];
-        if (st < tile_end) {
-          more = tid == kSegBlock - 1;  // (sorted starts)
-          const uint32_t en = offsets[s + 1];
-          if (en == st) {
-            if (st >= base) out[s] = init;
-          } else if (en - 1 >= base && en - 1 < tile_end && (MODE != SEG_LOCAL || st >= base)) {
-            out[s] = C::apply(init, s_vals[en - 1 - base]);
-          }
-        }
-      }
-      if (!__syncthreads_or(more)) break;
-    }
-    if constexpr (MODE == SEG_LOCAL) {
-      // The segment holding element `base` when no segment starts there
-      // (s0: the last segment starting before the tile) and whether it ends
-      // in this tile.
-      if (tid == 0) {
-        uint32_t sp = ~0u;
-        T ld = id;
-        if (!(s_heads[0] & 1u)) {
-          const uint32_t en = offsets[s0 + 1];
-          if (en <= tile_end) {
-            sp = s0;
-            ld = s_vals[en - 1 - base];
-          }
-        }
-        ra.span[tile] = sp;
-        ((T*)ra.lead)[tile] = ld;
-      }
-    }
-    // Trailing empty segments (start == n) belong to the last tile.
-    if (tile == ntiles - 1) {
-      if (tid == 0) s_first = lower_bound_u32(offsets, 0, nseg + 1, n);
-      __syncthreads();
-      for (uint32_t s = s_first + tid; s < nseg; s += kSegBlock) out[s] = init;
     }
   }
 }
@@ -553,6 +488,202 @@ __global__ __launch_bounds__(1024) void seg_pairs_exclusive_kernel(
   }
 }
 
+// seg Reduce in one read of the values, for every element type, one tile
+// per block. The tile combines its elements from an empty carry (head-flag
+// segmented scan: vectors, waves by DPP, rows x waves), writes init (+) value
+// for every segment that starts and ends in it (init for empty ones), and
+// leaves four records: whether it holds a segment start (f), the combination
+// after its last start (v: all of it without one), the combination before its
+// first start (lead), and the segment entering it if that one ends in it
+// (span). seg_span_walk_kernel then completes the segments that cross tiles.
+// The offsets of the tile's first 512 segments, read once to mark the starts,
+// stay in registers for the result writes.
+template <typename T, int OP>
+__global__ __launch_bounds__(kSegBlock) void seg_reduce_kernel(
+    const T* __restrict__ in, uint32_t n, const uint32_t* __restrict__ offsets, uint32_t nseg,
+    T* __restrict__ out, T init, bool aligned, uint32_t ntiles,
+    const uint32_t* __restrict__ tile_first, uint32_t* __restrict__ rec_f, T* __restrict__ rec_v,
+    T* __restrict__ rec_lead, uint32_t* __restrict__ rec_span) {
+  using C = Combine<T, OP>;
+  constexpr int V = 16 / sizeof(T);
+  constexpr int NW = kSegBlock / kWave;
+  constexpr int R = 4;
+  constexpr uint32_t TILE = (uint32_t)kSegBlock * R * V;
+  constexpr int NP = R * NW;
+  static_assert(NP <= kWave, "one partial per lane");
+  __shared__ uint32_t s_heads[TILE / 32];
+  __shared__ uint32_t s_wf[R][NW];
+  __shared__ T s_wv[R][NW];
+  __shared__ T s_vals[TILE];
+  __shared__ uint32_t s_more;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
+  const T id = C::identity();
+  const uint32_t t = blockIdx.x;
+  const size_t base = (size_t)t * TILE;
+  const uint32_t tile_end = (uint32_t)min((size_t)n, base + TILE);
+  T x[R][V];
+  {
+    const bool full = aligned && base + TILE <= n;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      load_run<T, V>(in, base + ((uint32_t)r * kSegBlock + tid) * V, n, full, x[r]);
+  }
+  const uint32_t s0 = tile_first[t];
+  for (uint32_t i = tid; i < TILE / 32; i += kSegBlock) s_heads[i] = 0;
+  __syncthreads();
+  // Segment starts inside the tile; starts are sorted, so only a chunk whose
+  // last start is inside can be followed by more. The first chunk's start
+  // and end offsets are kept.
+  uint32_t st0 = n, en0 = n;
+  {
+    const uint32_t s = s0 + tid;
+    if (s < nseg) {
+      st0 = offsets[s];
+      en0 = offsets[s + 1];
+      if (st0 >= base && st0 < tile_end)
+        atomicOr(&s_heads[(st0 - base) >> 5], 1u << ((st0 - base) & 31u));
+    }
+  }
+  const bool more = __syncthreads_or(tid == kSegBlock - 1 && st0 < tile_end);
+  if (more) {
+    for (uint32_t c = kSegBlock;; c += kSegBlock) {
+      const uint32_t s = s0 + c + tid;
+      bool m = false;
+      if (s < nseg) {
+        const uint32_t st = offsets[s];
+        if (st < tile_end) {
+          m = tid == kSegBlock - 1;
+          if (st >= base) atomicOr(&s_heads[(st - base) >> 5], 1u << ((st - base) & 31u));
+        }
+      }
+      if (!__syncthreads_or(m)) break;
+    }
+  }
+  uint32_t hb[R];
+  uint32_t wf[R];
+  T wv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t li = ((uint32_t)r * kSegBlock + tid) * V;
+    hb[r] = (s_heads[li >> 5] >> (li & 31u)) & ((1u << V) - 1u);
+    uint32_t f = hb[r] & 1u;
+    T v = x[r][0];
+#pragma unroll
+    for (int k = 1; k < V; ++k) seg_combine<T, OP>(f, v, (hb[r] >> k) & 1u, x[r][k]);
+    wf[r] = f;
+    wv[r] = v;
+    wave_inclusive_seg_scan<T, OP>(wf[r], wv[r]);
+    if (lane == 63) {
+      s_wf[r][wid] = wf[r];
+      s_wv[r][wid] = wv[r];
+    }
+  }
+  __syncthreads();
+  // (row, wave) partials scanned by every wave, one per lane.
+  uint32_t qf = 0;
+  T qv = id;
+  if (lane < (uint32_t)NP) {
+    qf = s_wf[lane / NW][lane % NW];
+    qv = s_wv[lane / NW][lane % NW];
+  }
+  wave_inclusive_seg_scan<T, OP>(qf, qv);
+  const uint32_t af = readlane(qf, NP - 1);
+  const T av = readlane(qv, NP - 1);
+  const uint32_t xf = shift_up1<uint32_t>(qf, 0u);
+  const T xv = shift_up1<T>(qv, id);
+  const int w0 = __builtin_amdgcn_readfirstlane((int)wid);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    uint32_t cf = readlane(xf, r * NW + w0);
+    T cv = readlane(xv, r * NW + w0);
+    seg_combine<T, OP>(cf, cv, shift_up1<uint32_t>(wf[r], 0u), shift_up1<T>(wv[r], id));
+    const uint32_t li = ((uint32_t)r * kSegBlock + tid) * V;
+    T run = cv;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      run = ((hb[r] >> k) & 1u) ? x[r][k] : C::apply(run, x[r][k]);
+      s_vals[li + k] = run;
+    }
+  }
+  __syncthreads();
+  // One result per segment that starts in the tile and ends in it.
+  auto write = [&](uint32_t s, uint32_t st, uint32_t en) {
+    if (st >= base && st < tile_end) {
+      if (en == st) out[s] = init;
+      else if (en <= tile_end) out[s] = C::apply(init, s_vals[en - 1 - base]);
+    }
+  };
+  if (s0 + tid < nseg) write(s0 + tid, st0, en0);
+  if (more) {
+    for (uint32_t c = kSegBlock;; c += kSegBlock) {
+      const uint32_t s = s0 + c + tid;
+      bool m = false;
+      if (s < nseg) {
+        const uint32_t st = offsets[s];
+        if (st < tile_end) {
+          m = tid == kSegBlock - 1;
+          write(s, st, offsets[s + 1]);
+        }
+      }
+      if (!__syncthreads_or(m)) break;
+    }
+  }
+  if (tid == 0) {
+    rec_f[t] = af;
+    rec_v[t] = av;
+    uint32_t sp = ~0u;
+    T ld = av;
+    // No start at the tile's first element: segment s0 enters the tile and
+    // ends at its end offset (en0 of thread 0), in this tile or later.
+    if (!(s_heads[0] & 1u) && en0 <= tile_end) {
+      sp = s0;
+      ld = s_vals[en0 - 1 - base];
+    }
+    rec_span[t] = sp;
+    rec_lead[t] = ld;
+  }
+  if (t == ntiles - 1) {
+    // Trailing empty segments (start == n).
+    if (tid == 0) s_more = lower_bound_u32(offsets, 0, nseg + 1, n);
+    __syncthreads();
+    for (uint32_t s = s_more + tid; s < nseg; s += kSegBlock) out[s] = init;
+  }
+}
+
+// Completes the segments that enter a tile from before and end in it: one
+// thread per such tile walks back over the earlier tiles' records (16 per
+// step) to the tile where the segment starts, combining right to left:
+// init (+) (v[q] (+) (v[q+1] (+) ... (v[t-1] (+) lead[t]))).
+template <typename T, int OP>
+__global__ void seg_span_walk_kernel(const uint32_t* __restrict__ rec_f, const T* __restrict__ rec_v,
+                                     const T* __restrict__ rec_lead,
+                                     const uint32_t* __restrict__ rec_span, uint32_t ntiles,
+                                     T* __restrict__ out, T init) {
+  using C = Combine<T, OP>;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const uint32_t s = rec_span[t];
+  if (s == ~0u) return;
+  T acc = rec_lead[t];
+  int64_t q = (int64_t)t - 1;  // tile 0 starts segment 0, so the walk ends
+  for (bool done = false; !done && q >= 0; q -= 16) {
+    uint32_t f[16];
+    T v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      f[k] = q - k >= 0 ? rec_f[q - k] : 1u;
+      v[k] = q - k >= 0 ? rec_v[q - k] : C::identity();
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (done) break;
+      acc = C::apply(v[k], acc);
+      done = f[k] != 0;
+    }
+  }
+  out[s] = C::apply(init, acc);
+}
+
 // first[t] = the last segment starting before tile t (the non-empty segment
 // holding element t * tile - 1), first[0] = 0: one thread per segment writes
 // the tiles whose start lies in (its start, its end].
@@ -565,19 +696,6 @@ __global__ void seg_tile_first_kernel(const uint32_t* __restrict__ offsets, uint
   if (lo == hi) return;
   const uint32_t t1 = min(hi / tile, ntiles - 1);
   for (uint32_t t = lo / tile + 1; t <= t1; ++t) first[t] = s;
-}
-
-// The segments that cross into a tile and end in it (SEG_LOCAL): init (+)
-// (carry from the earlier tiles (+) partial within the tile).
-template <typename T, int OP>
-__global__ void seg_span_fixup_kernel(const uint32_t* __restrict__ span, const T* __restrict__ lead,
-                                      const T* __restrict__ carry, uint32_t ntiles,
-                                      T* __restrict__ out, T init) {
-  using C = Combine<T, OP>;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
-  const uint32_t s = span[t];
-  if (s != ~0u) out[s] = C::apply(init, C::apply(carry[t], lead[t]));
 }
 
 template <typename T>
@@ -598,6 +716,31 @@ __global__ void validate_offsets_kernel(const uint32_t* __restrict__ offsets, ui
 }
 
 // ---- host -------------------------------------------------------------------------
+// seg Reduce over CSR offsets: one read of the values (seg_reduce_kernel),
+// then the segments that cross tiles (seg_span_walk_kernel).
+template <typename T, int OP>
+static msv_status seg_reduce_run(Ctx& c, const T* in, uint32_t n, const uint32_t* offsets,
+                                 uint32_t nseg, T* out, T init) {
+  constexpr uint32_t TILE = (uint32_t)kSegBlock * 4 * (16 / sizeof(T));
+  const uint32_t ntiles = div_up(n, TILE);
+  uint32_t* first = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
+  uint32_t* rec_f = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
+  uint32_t* rec_span = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
+  T* rec_v = (T*)c.scratch(sizeof(T) * ntiles);
+  T* rec_lead = (T*)c.scratch(sizeof(T) * ntiles);
+  if (!first || !rec_f || !rec_span || !rec_v || !rec_lead) return MSV_ERR_OUT_OF_MEMORY;
+  hipLaunchKernelGGL(seg_tile_first_kernel, dim3(div_up(nseg, 256u)), dim3(256), 0, c.stream(),
+                     offsets, nseg, TILE, ntiles, first);
+  const bool aligned = ((uintptr_t)in & 15u) == 0;
+  hipLaunchKernelGGL((seg_reduce_kernel<T, OP>), dim3(ntiles), dim3(kSegBlock), 0, c.stream(), in, n,
+                     offsets, nseg, out, init, aligned, ntiles, first, rec_f, rec_v, rec_lead,
+                     rec_span);
+  hipLaunchKernelGGL((seg_span_walk_kernel<T, OP>), dim3(div_up(ntiles, 256u)), dim3(256), 0,
+                     c.stream(), rec_f, rec_v, rec_lead, rec_span, ntiles, out, init);
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
 // Segmented scans / reduce over CSR offsets (HK = void) or over runs of equal
 // keys (HK = the key type, ra.keys set).
 template <typename T, int OP, typename HK = void>
@@ -614,6 +757,9 @@ static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
     }
     return MSV_OK;
   }
+  if constexpr (!KEYS) {
+    if (outk == SEG_REDUCE) return seg_reduce_run<T, OP>(c, in, n, offsets, nseg, out, init);
+  }
   const uint32_t ntiles = div_up(n, TILE);
   // Per-element outputs are stored as vectors; the others only need the
   // inputs aligned.
@@ -626,8 +772,7 @@ static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
 #define MSV_SEG_BY_OUT(MODE, ST, TF, TV)                                  \
   if (outk == SEG_INCL) MSV_SEG_LAUNCH(MODE, SEG_INCL, ST, TF, TV);        \
   else if (outk == SEG_EXCL) MSV_SEG_LAUNCH(MODE, SEG_EXCL, ST, TF, TV);   \
-  else if constexpr (KEYS) MSV_SEG_LAUNCH(MODE, SEG_RUNS, ST, TF, TV);     \
-  else MSV_SEG_LAUNCH(MODE, SEG_REDUCE, ST, TF, TV);
+  else if constexpr (KEYS) MSV_SEG_LAUNCH(MODE, SEG_RUNS, ST, TF, TV);
   if constexpr (!KEYS) {
     if (nseg > 0 && ntiles > 1) {
       uint32_t* first = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
@@ -657,28 +802,6 @@ static msv_status seg_scan_impl(Ctx& c, int outk, const T* in, uint32_t n,
       if (!acnt || !pcnt) return MSV_ERR_OUT_OF_MEMORY;
     }
     if (!af || !av || !pf || !pv) return MSV_ERR_OUT_OF_MEMORY;
-    if (outk == SEG_REDUCE) {
-      // One read of the values: local tile pass, scan of the tile
-      // aggregates, then the segments that cross tile boundaries.
-      uint32_t* span = (uint32_t*)c.scratch(sizeof(uint32_t) * ntiles);
-      T* lead = (T*)c.scratch(sizeof(T) * ntiles);
-      if (!span || !lead) return MSV_ERR_OUT_OF_MEMORY;
-      RunsArgs la = ra;
-      la.span = span;
-      la.lead = lead;
-      if constexpr (!KEYS) {
-        hipLaunchKernelGGL((seg_tile_kernel<T, OP, SEG_LOCAL, SEG_REDUCE, HK>), dim3(ntiles),
-                           dim3(kSegBlock), 0, c.stream(), in, n, offsets, nseg, out, init, aligned,
-                           (uint64_t*)nullptr, 0u, ntiles, af, av, la);
-      }
-      MSV_HIP(hipGetLastError());
-      hipLaunchKernelGGL((seg_pairs_exclusive_kernel<T, OP>), dim3(1), dim3(1024), 0, c.stream(),
-                         af, av, ntiles, pf, pv, (const uint32_t*)nullptr, (uint32_t*)nullptr);
-      hipLaunchKernelGGL((seg_span_fixup_kernel<T, OP>), dim3(div_up(ntiles, 256u)), dim3(256), 0,
-                         c.stream(), span, lead, pv, ntiles, out, init);
-      MSV_HIP(hipGetLastError());
-      return MSV_OK;
-    }
     RunsArgs agg = ra;
     agg.tile_c = acnt;
     // The aggregate pass computes the plain segmented aggregates (its output
