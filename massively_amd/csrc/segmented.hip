@@ -139,9 +139,10 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(
         const uint32_t lb = wave_lower_bound(offsets, 0, nseg + 1, (uint32_t)base);
         if (lane == 0) s_first = lb > 0 ? lb - 1 : 0;
       }
-      __syncthreads();
-      s0 = s_first;
     }
+    // The bitmap is cleared before any thread marks a start in it.
+    __syncthreads();
+    if (!ra.tile_first) s0 = s_first;
     // Mark the segment starts that fall inside the tile.
     for (uint32_t c = 0;; c += kSegBlock) {
       const uint32_t s = s0 + c + tid;
@@ -651,22 +652,51 @@ __global__ __launch_bounds__(kSegBlock) void seg_reduce_kernel(
 }
 
 // Completes the segments that enter a tile from before and end in it: one
-// thread per such tile walks back over the earlier tiles' records (16 per
-// step) to the tile where the segment starts, combining right to left:
-// init (+) (v[q] (+) (v[q+1] (+) ... (v[t-1] (+) lead[t]))).
+// thread per such tile walks back over the earlier tiles' records, 16 per
+// step, to the tile where the segment starts. Within a step the partials are
+// folded right to left; the steps' results (lead[t] and v[t-1] first) are
+// combined pairwise by a binary cascade (level k holds the combination of 2^k
+// consecutive step results; a new result, to the left of all held ones,
+// merges upward with equal-sized blocks). So a segment's result is a fixed
+// tree over its partials whose depth grows with 16 + log2 of the number of
+// tiles it spans.
 template <typename T, int OP>
 __global__ void seg_span_walk_kernel(const uint32_t* __restrict__ rec_f, const T* __restrict__ rec_v,
                                      const T* __restrict__ rec_lead,
                                      const uint32_t* __restrict__ rec_span, uint32_t ntiles,
                                      T* __restrict__ out, T init) {
   using C = Combine<T, OP>;
+  constexpr int kLevels = 17;  // 32-bit lengths: at most 2^20 tiles, 2^16 steps
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   const uint32_t s = rec_span[t];
   if (s == ~0u) return;
-  T acc = rec_lead[t];
-  int64_t q = (int64_t)t - 1;  // tile 0 starts segment 0, so the walk ends
-  for (bool done = false; !done && q >= 0; q -= 16) {
+  // Common case: the segment starts in the tile before (t > 0: tile 0
+  // starts with a segment).
+  const uint32_t f1 = rec_f[t - 1];
+  const T v1 = rec_v[t - 1];
+  const T r1 = C::apply(v1, rec_lead[t]);
+  if (f1) {
+    out[s] = C::apply(init, r1);
+    return;
+  }
+  T lvl[kLevels];
+  uint32_t have = 0;
+  // Branch-free in the level (no early exit), so every level stays a register.
+  auto push = [&](T x) {
+    bool placed = false;
+#pragma unroll
+    for (int k = 0; k < kLevels; ++k) {
+      const bool occupied = (have >> k) & 1u;
+      if (!placed && occupied) x = C::apply(x, lvl[k]);  // x lies left of level k's block
+      if (!placed && !occupied) lvl[k] = x;
+      if (!placed) have ^= 1u << k;
+      placed = placed || !occupied;
+    }
+  };
+  push(r1);
+  bool done = false;
+  for (int64_t q = (int64_t)t - 2; !done && q >= 0; q -= 16) {
     uint32_t f[16];
     T v[16];
 #pragma unroll
@@ -674,11 +704,23 @@ __global__ void seg_span_walk_kernel(const uint32_t* __restrict__ rec_f, const T
       f[k] = q - k >= 0 ? rec_f[q - k] : 1u;
       v[k] = q - k >= 0 ? rec_v[q - k] : C::identity();
     }
+    T acc = v[0];
+    done = f[0] != 0;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (done) break;
-      acc = C::apply(v[k], acc);
-      done = f[k] != 0;
+    for (int k = 1; k < 16; ++k) {
+      if (!done) acc = C::apply(v[k], acc);
+      done = done || f[k] != 0;
+    }
+    push(acc);
+  }
+  // The held blocks, left (lowest level) to right.
+  bool any = false;
+  T acc = C::identity();
+#pragma unroll
+  for (int k = 0; k < kLevels; ++k) {
+    if (have & (1u << k)) {
+      acc = any ? C::apply(acc, lvl[k]) : lvl[k];
+      any = true;
     }
   }
   out[s] = C::apply(init, acc);

@@ -102,6 +102,33 @@ def test_int_seg_scans_and_reduce(exec_, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_float_seg_reduce_long_segments(exec_, dtype):
+    """Segments spanning hundreds of tiles (the cross-tile partials are
+    combined pairwise): within (2 log2 L + 40) eps sum|x| of the FP64 sum, the
+    same bytes on a second run, and integer sums exact."""
+    eps = 2.0**-24 if dtype == np.float32 else 2.0**-53
+    L = [3, (1 << 22) + 5, 100, 0, 0, (1 << 20) - 1, 8192, 8191, 1, 1 << 21]
+    off = offsets_of(L)
+    n = int(off[-1])
+    x = (RNG.standard_normal(n) + 0.25).astype(dtype)
+    sg = seg.Segmentation.from_offsets(exec_, exec_.to_device(off))
+    dx = exec_.to_device(x)
+    r1 = exec_.to_host(seg.ForEachSegment(seg.Reduce(m.op.Add, 1.5)).run(exec_, seg.SegmentIterator(dx, sg)))
+    r2 = exec_.to_host(seg.ForEachSegment(seg.Reduce(m.op.Add, 1.5)).run(exec_, seg.SegmentIterator(dx, sg)))
+    assert r1.tobytes() == r2.tobytes()
+    x64 = x.astype(np.float64)
+    for s, ln in enumerate(L):
+        a, b = off[s], off[s + 1]
+        exact = 1.5 + float(np.sum(x64[a:b]))
+        bound = (2 * math.ceil(math.log2(max(ln, 2))) + 40) * eps * (1.5 + float(np.sum(np.abs(x64[a:b]))))
+        assert abs(float(r1[s]) - exact) <= bound, (s, ln)
+    xi = RNG.integers(0, 2**32, n, dtype=np.uint32)
+    ri = exec_.to_host(seg.ForEachSegment(seg.Reduce(m.op.Add, 0)).run(
+        exec_, seg.SegmentIterator(exec_.to_device(xi), sg)))
+    assert np.array_equal(ri, oracle.seg_reduce(xi, off, np.uint32(0), "add"))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_float_seg_reduce_and_scan(exec_, dtype):
     eps = 2.0**-24 if dtype == np.float32 else 2.0**-53
     for lengths in cases():
