@@ -433,6 +433,36 @@ def w_columns(c):
         groups = 1 if nk == 2 else 2
         b = n * (groups * (8 * 24 + 4.0 * nk) + 3 * 12.0)
         out[f"keys{nk}_vals3"] = summarize(f"radix_sort_by_key_{nk}xu32_keys_3xu32_values_2^26", ms, b, n)
+    # Zipped rows moved by one launch that reads the indices / stencil once,
+    # against one call per column: gather of 4 u32 columns through a random
+    # permutation of 2^26, and copy_where of 4 u32 columns by a 50% stencil.
+    zc = tuple(cols) + (vals[0],)
+    perm = c.ex.alloc(np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, n - 1, 47, perm)
+    st = c.ex.alloc(np.uint32, n)
+    c.mr.uniform_u32(c.ex, 0, 1, 48, st)
+    gb = n * (4.0 + 4 * (4.0 + 4.0))  # indices once, per column a 4 B read and a 4 B write
+    ms = timed(c.ex, lambda: m.vector.gather(c.ex, zc, perm), c.steps, c.warmup)
+    out["gather_4cols_one_pass"] = summarize("gather_4xu32_columns_2^26_one_pass", ms, gb, n)
+    ms = timed(c.ex, lambda: [m.vector.gather(c.ex, col, perm) for col in zc], c.steps, c.warmup)
+    out["gather_4cols_per_column"] = summarize("gather_4xu32_columns_2^26_per_column", ms, gb, n)
+    # The same with coherent indices (every 4th row, ascending): the index
+    # reads are then a quarter of each column's traffic.
+    from massively_amd import lazy
+    from massively_amd.op import UnaryOp
+
+    sidx = m.vector.map(c.ex, lazy.stride(0, 4).take(n // 4), UnaryOp("x"))
+    sb = (n // 4) * (4.0 + 4 * (4.0 + 4.0))
+    ms = timed(c.ex, lambda: m.vector.gather(c.ex, zc, sidx), c.steps, c.warmup)
+    out["gather_4cols_strided_one_pass"] = summarize("gather_4xu32_columns_stride4_2^24_one_pass", ms, sb, n // 4)
+    ms = timed(c.ex, lambda: [m.vector.gather(c.ex, col, sidx) for col in zc], c.steps, c.warmup)
+    out["gather_4cols_strided_per_column"] = summarize("gather_4xu32_columns_stride4_2^24_per_column", ms, sb,
+                                                       n // 4)
+    cb = n * (4.0 + 4 * (4.0 + 0.5 * 4.0))  # stencil once, per column a read and the kept half written
+    ms = timed(c.ex, lambda: m.vector.copy_where(c.ex, zc, st), c.steps, c.warmup)
+    out["copy_where_4cols_one_pass"] = summarize("copy_where_4xu32_columns_2^26_one_pass", ms, cb, n)
+    ms = timed(c.ex, lambda: [m.vector.copy_where(c.ex, col, st) for col in zc], c.steps, c.warmup)
+    out["copy_where_4cols_per_column"] = summarize("copy_where_4xu32_columns_2^26_per_column", ms, cb, n)
     return out
 
 

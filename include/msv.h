@@ -277,6 +277,25 @@ msv_status msv_radix_sort_by_key(msv_executor* ex, msv_slice keys, msv_slice val
 msv_status msv_radix_sort_by_key_columns(msv_executor* ex, const msv_slice* keys, uint32_t nkeys,
                                          const msv_slice* values, uint32_t nvalues,
                                          msv_vec** out_values);
+/* Zipped rows of 1..12 columns of one length (4- and 8-byte dtypes mixed),
+ * moved by one launch that reads the shared input once: the reference's
+ * indexed copy over the leaves of a zipped read (core/indexed.rs:16-196) for
+ * gather and scatter, and its selection copy over zipped leaves
+ * (core/selection.rs:151-411) for copy_where / remove_where. Semantics per
+ * column are those of msv_gather / msv_scatter / msv_copy_where(_pred); every
+ * output of msv_copy_where_columns carries the same device-resident length.
+ * `out` receives ncols new vectors; `outputs` are ncols slices of one length
+ * (scatter outputs of 64 MiB or more per column are written column by column
+ * in the binned form). */
+msv_status msv_gather_columns(msv_executor* ex, const msv_slice* values, uint32_t ncols,
+                              msv_slice indices, msv_vec** out);
+msv_status msv_scatter_columns(msv_executor* ex, const msv_slice* values, uint32_t ncols,
+                               msv_slice indices, const msv_slice* outputs);
+/* pred == NULL: the stencil is u32 flags (kept where != 0); else the built-in
+ * predicate applied to the stencil column. invert != 0: remove_where. */
+msv_status msv_copy_where_columns(msv_executor* ex, const msv_slice* input, uint32_t ncols,
+                                  msv_slice stencil, const msv_predicate* pred, int32_t invert,
+                                  msv_vec** out);
 /* vector::sort_by_key (api/algorithm/by_key.rs:193-250): stable sort by the
  * built-in comparison; float keys compare with IEEE `<` (-0 == +0). */
 msv_status msv_sort_by_key(msv_executor* ex, msv_slice keys, msv_slice values, msv_cmp cmp,

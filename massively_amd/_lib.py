@@ -210,6 +210,10 @@ PROTOTYPES = {
     "msv_radix_sort_by_key_columns": (_I, [_P, C.POINTER(MsvSlice), C.c_uint32,
                                            C.POINTER(MsvSlice), C.c_uint32, _PP]),
     "msv_radix_sort_by_key": (_I, [_P, _S, _S, _PP, _PP]),
+    "msv_gather_columns": (_I, [_P, C.POINTER(MsvSlice), C.c_uint32, _S, _PP]),
+    "msv_scatter_columns": (_I, [_P, C.POINTER(MsvSlice), C.c_uint32, _S, C.POINTER(MsvSlice)]),
+    "msv_copy_where_columns": (_I, [_P, C.POINTER(MsvSlice), C.c_uint32, _S,
+                                    C.POINTER(MsvPredicate), C.c_int32, _PP]),
     "msv_sort_by_key": (_I, [_P, _S, _S, _I, _PP, _PP]),
     "msv_sort": (_I, [_P, _S, _I, _PP]),
     "msv_sort_user": (_I, [_P, _S, _P, _PP]),

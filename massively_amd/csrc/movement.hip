@@ -395,10 +395,14 @@ static msv_status binned_scatter(Ctx& c, const T* values, const uint32_t* idx, u
 // partial-line writes stay in the caches below it).
 constexpr uint64_t kBinScatterMinBytes = 64ull << 20;
 
+bool scatter_binned(msv_dtype dt, uint32_t n, uint32_t olen) {
+  return (uint64_t)olen * dtype_size(dt) >= kBinScatterMinBytes && n >= (1u << 20);
+}
+
 msv_status scatter_run(Ctx& c, msv_dtype dt, const void* values, const uint32_t* idx,
                        uint32_t n, void* out, uint32_t olen) {
   if (n == 0) return MSV_OK;
-  if ((uint64_t)olen * dtype_size(dt) >= kBinScatterMinBytes && n >= (1u << 20)) {
+  if (scatter_binned(dt, n, olen)) {
     if (dtype_size(dt) == 4)
       return binned_scatter<uint32_t>(c, (const uint32_t*)values, idx, n, (uint32_t*)out, olen);
     return binned_scatter<uint64_t>(c, (const uint64_t*)values, idx, n, (uint64_t*)out, olen);

@@ -226,6 +226,8 @@ msv_status gather_run(Ctx& c, msv_dtype dt, const void* values, uint32_t vlen,
                       const uint32_t* idx, void* out, uint32_t n);
 msv_status scatter_run(Ctx& c, msv_dtype dt, const void* values, const uint32_t* idx,
                        uint32_t n, void* out, uint32_t olen);
+// Whether scatter_run takes the binned form for these sizes.
+bool scatter_binned(msv_dtype dt, uint32_t n, uint32_t olen);
 msv_status iota_run(Ctx& c, uint32_t* out, uint32_t n, uint32_t start);
 // Seeded synthetic input of the library generator: full-range bits for
 // integers, [0.5, 1) for floats.

@@ -43,6 +43,16 @@ def _col_len(exec: Executor, x) -> int:
     return s.vec.len() - s.offset if s.length == L.WHOLE else s.length
 
 
+def _col_slices(exec: Executor, cols):
+    """A C array of the columns' slices (lazy columns written out first)."""
+    sl = [_in(exec, c)._c() for c in cols]
+    return (L.MsvSlice * len(sl))(*sl)
+
+
+def _columns_out(exec: Executor, outs) -> tuple:
+    return tuple(_out(exec, C.c_void_p(h)) for h in outs)
+
+
 def _same_lengths(exec: Executor, cols) -> int:
     lens = [_col_len(exec, c) for c in cols]
     for n in lens[1:]:
@@ -140,11 +150,15 @@ def reduce(exec: Executor, input, init, op: ReductionOp):
 
 def copy_where(exec: Executor, input, stencil) -> DeviceVec:
     """massively/src/api/algorithm/selection.rs:146-165 (device-resident length).
-    A zipped input (tuple of columns) gives the tuple of compacted columns."""
+    A zipped input (tuple of columns) gives the tuple of compacted columns, from
+    one launch that reads the stencil once (msv_copy_where_columns)."""
     cols = _cols(input)
     if cols is not None:
         _same_lengths(exec, cols)
-        return tuple(copy_where(exec, c, stencil) for c in cols)
+        outs = (C.c_void_p * len(cols))()
+        check(_lib().msv_copy_where_columns(exec.handle, _col_slices(exec, cols), len(cols),
+                                            _in(exec, stencil)._c(), None, 0, outs))
+        return _columns_out(exec, outs)
     h = C.c_void_p()
     check(_lib().msv_copy_where(exec.handle, _in(exec, input)._c(), _in(exec, stencil)._c(),
                                 C.byref(h)))
@@ -156,7 +170,10 @@ def remove_where(exec: Executor, input, stencil) -> DeviceVec:
     cols = _cols(input)
     if cols is not None:
         _same_lengths(exec, cols)
-        return tuple(remove_where(exec, c, stencil) for c in cols)
+        outs = (C.c_void_p * len(cols))()
+        check(_lib().msv_copy_where_columns(exec.handle, _col_slices(exec, cols), len(cols),
+                                            _in(exec, stencil)._c(), None, 1, outs))
+        return _columns_out(exec, outs)
     h = C.c_void_p()
     check(_lib().msv_remove_where(exec.handle, _in(exec, input)._c(), _in(exec, stencil)._c(),
                                   C.byref(h)))
@@ -175,7 +192,12 @@ def copy_where_if(exec: Executor, input, stencil, pred: PredicateOp, invert: boo
         if isinstance(pred, UserPredicate):
             raise ValueError("a UserPredicate applies to a single input column")
         _same_lengths(exec, cols)
-        return tuple(copy_where_if(exec, c, stencil, pred, invert) for c in cols)
+        st = _in(exec, stencil)
+        p = pred.c_struct(st.dtype)
+        outs = (C.c_void_p * len(cols))()
+        check(_lib().msv_copy_where_columns(exec.handle, _col_slices(exec, cols), len(cols), st._c(),
+                                            C.byref(p), 1 if invert else 0, outs))
+        return _columns_out(exec, outs)
     if isinstance(input, lazy.Lazy) and stencil is input:
         if invert:
             raise ValueError("remove_where_if of a lazy input: use the negated predicate")
@@ -220,12 +242,15 @@ def partition(exec: Executor, input, pred: PredicateOp):
 
 def gather(exec: Executor, values, indices) -> DeviceVec:
     """massively/src/api/algorithm/indexed.rs:157-208: out[i] = values[indices[i]].
-    Zipped values (tuple of columns) give the tuple of gathered columns."""
+    Zipped values (tuple of columns) give the tuple of gathered columns, from one
+    launch that reads the indices once (msv_gather_columns)."""
     cols = _cols(values)
     if cols is not None:
         _same_lengths(exec, cols)
-        idx = _in(exec, indices)
-        return tuple(gather(exec, c, idx) for c in cols)
+        outs = (C.c_void_p * len(cols))()
+        check(_lib().msv_gather_columns(exec.handle, _col_slices(exec, cols), len(cols),
+                                        _in(exec, indices)._c(), outs))
+        return _columns_out(exec, outs)
     h = C.c_void_p()
     check(_lib().msv_gather(exec.handle, _in(exec, values)._c(), _in(exec, indices)._c(),
                             C.byref(h)))
@@ -234,7 +259,8 @@ def gather(exec: Executor, values, indices) -> DeviceVec:
 
 def scatter(exec: Executor, values, indices, output) -> None:
     """massively/src/api/algorithm/scatter.rs:248-265: output[indices[i]] = values[i].
-    Zipped values scatter column by column into the zipped output."""
+    Zipped values scatter into the zipped output in one launch that reads the
+    indices once (msv_scatter_columns)."""
     cols = _cols(values)
     if cols is not None:
         outs = _cols(output)
@@ -242,9 +268,9 @@ def scatter(exec: Executor, values, indices, output) -> None:
             raise ValueError("zipped values need a zipped output of as many columns")
         _same_lengths(exec, cols)
         _same_lengths(exec, outs)
-        idx = _in(exec, indices)
-        for c, o in zip(cols, outs):
-            scatter(exec, c, idx, o)
+        osl = [as_slice(o)._c() for o in outs]
+        check(_lib().msv_scatter_columns(exec.handle, _col_slices(exec, cols), len(cols),
+                                         _in(exec, indices)._c(), (L.MsvSlice * len(osl))(*osl)))
         return
     check(_lib().msv_scatter(exec.handle, _in(exec, values)._c(), _in(exec, indices)._c(),
                              as_slice(output)._c()))

@@ -1,11 +1,13 @@
 """GPU parity of zipped columns (the reference's zip2 .. zip12 rows) in the
 algorithms beyond radix_sort_by_key, and of lazy slicing chains.
 
-Zipped values are processed column by column through the same kernels
+Zipped gather, scatter and copy_where / remove_where move all columns in one
+launch that reads the indices or the stencil once (msv_*_columns); the
+other algorithms process the columns through the single-column kernels
 (sort_by_key and merge_by_key order the keys once, with row indices, and
-gather every column by that order), so each column must equal the
-single-column result and numpy's; the columns of one call share lengths
-(LengthMismatch otherwise)."""
+gather every column by that order). Each column must equal the single-column
+result and numpy's; the columns of one call share lengths (LengthMismatch
+otherwise)."""
 import numpy as np
 import pytest
 
@@ -85,6 +87,51 @@ def test_zipped_copy_where(exec_):
         key = cols[0]
         for got, c in zip(_host(exec_, m.vector.copy_where_if(exec_, d, d[0], m.op.Lt(1 << 31))), cols):
             assert got.tobytes() == c[key < (1 << 31)].tobytes(), n
+
+
+def test_twelve_columns_one_pass(exec_):
+    """12 columns of mixed widths (u32, i32, f32, u64, i64, f64), read through
+    views that start one element in (unaligned rows), against numpy and the
+    single-column calls: gather with out-of-range indices (0), scatter with
+    out-of-range indices (dropped) into small and into 2^24-element outputs
+    (the latter column by column in the binned form), copy_where /
+    remove_where and a predicate on a separate stencil column."""
+    dts = [np.uint32, np.int32, np.float32, np.uint64, np.int64, np.float64] * 2
+    for n in (5, 8193, 70001):
+        cols = [RNG.integers(0, 2**31, n + 1).astype(dt) if np.dtype(dt).kind != "f"
+                else RNG.standard_normal(n + 1).astype(dt) for dt in dts]
+        base = [exec_.to_device(c) for c in cols]
+        views = tuple(b.slice(1, n + 1) for b in base)
+        host = [c[1:] for c in cols]
+        idx = RNG.integers(0, n + 7, n // 3 + 3).astype(np.uint32)
+        got = _host(exec_, m.vector.gather(exec_, views, exec_.to_device(idx)))
+        ok = idx < n
+        for g, c in zip(got, host):
+            want = np.zeros(idx.size, dtype=c.dtype)
+            want[ok] = c[idx[ok]]
+            assert g.tobytes() == want.tobytes(), n
+        single = exec_.to_host(m.vector.gather(exec_, views[4], exec_.to_device(idx)))
+        assert single.tobytes() == got[4].tobytes()
+        st = (RNG.random(n) < 0.3).astype(np.uint32)
+        ds = exec_.to_device(st)
+        for got, c in zip(_host(exec_, m.vector.copy_where(exec_, views, ds)), host):
+            assert got.tobytes() == c[st != 0].tobytes(), n
+        for got, c in zip(_host(exec_, m.vector.remove_where(exec_, views, ds)), host):
+            assert got.tobytes() == c[st == 0].tobytes(), n
+        key = host[5]
+        res = m.vector.copy_where_if(exec_, views, views[5], m.op.Gt(2**30))
+        for got, c in zip(_host(exec_, res), host):
+            assert got.tobytes() == c[key > 2**30].tobytes(), n
+        for olen in (n, 1 << 24):
+            if olen == 1 << 24 and n < 8193:
+                continue
+            perm = RNG.permutation(max(olen, n))[:n].astype(np.uint32)
+            perm[::97] = olen + 3  # dropped
+            outs = tuple(exec_.alloc(c.dtype, olen) for c in host[:3])
+            m.vector.scatter(exec_, views[:3], exec_.to_device(perm), outs)
+            keep = perm < olen  # destinations are distinct: check the written ones
+            for o, c in zip(_host(exec_, outs), host[:3]):
+                assert o[perm[keep]].tobytes() == c[keep].tobytes(), (n, olen)
 
 
 def test_zipped_sort_and_merge_by_key(exec_):
