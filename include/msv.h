@@ -179,10 +179,25 @@ msv_status msv_graph_destroy(msv_graph* g);
 uint32_t msv_segsort_config_count(void);
 msv_status msv_segsort_config_info(uint32_t k, uint32_t* block, uint32_t* items_per_thread,
                                    uint32_t* digit_bits);
-/* The candidate the autotuner chose for this shape (key/value bytes, launch
- * form, total elements, segments), or -1 if it has not run for it. */
+/* The candidate the autotuner chose on the current device for this shape
+ * (key/value bytes, launch form, total elements, segments; short segments
+ * sorted inside a wave, or packed into windows for the _windows form), or -1
+ * if it has not run for it. */
 int32_t msv_segsort_tuned_config(uint32_t key_bytes, uint32_t value_bytes, int64_t launch,
                                  uint32_t n, uint32_t nseg);
+int32_t msv_segsort_tuned_config_windows(uint32_t key_bytes, uint32_t value_bytes, int64_t launch,
+                                         uint32_t n, uint32_t nseg);
+/* The autotuner's table (no counterpart in the reference, which specialises
+ * kernels in its JIT compiler): one entry per (device signature "<gfx arch
+ * name>/<compute units>", family, key), saved to and merged from a text file
+ * so that a later process starts tuned; entries made on another kind of GPU
+ * are kept but never used. With the environment variable MSV_TUNING_CACHE
+ * naming a file, the library reads it at first use and rewrites it after
+ * every new entry. */
+msv_status msv_tuning_save(const char* path);
+msv_status msv_tuning_load(const char* path);
+msv_status msv_tuning_clear(void);
+uint32_t msv_tuning_count(void);
 
 /* ---- device vectors (runtime.rs:79-177, allocation.rs:852-861) -------- */
 /* Executor::alloc: uninitialised vector of `len` elements. */

@@ -17,7 +17,8 @@ from .error import (
     UnboundColumn,
     Unsupported,
 )
-from .runtime import DeviceSlice, DeviceVec, Executor, segsort_configs, segsort_tuned_config
+from .runtime import (DeviceSlice, DeviceVec, Executor, segsort_configs, segsort_tuned_config,
+                      tuning_clear, tuning_count, tuning_load, tuning_save)
 
 __all__ = [
     "Executor",
@@ -33,6 +34,10 @@ __all__ = [
     "Unsupported",
     "segsort_configs",
     "segsort_tuned_config",
+    "tuning_clear",
+    "tuning_count",
+    "tuning_load",
+    "tuning_save",
     "lazy",
     "op",
     "seg",

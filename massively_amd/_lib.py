@@ -171,6 +171,11 @@ PROTOTYPES = {
     "msv_server_now_us": (C.c_double, [_P]),
     "msv_segsort_config_count": (_U32, []),
     "msv_segsort_tuned_config": (_I, [_U32, _U32, C.c_int64, _U32, _U32]),
+    "msv_segsort_tuned_config_windows": (_I, [_U32, _U32, C.c_int64, _U32, _U32]),
+    "msv_tuning_save": (_I, [C.c_char_p]),
+    "msv_tuning_load": (_I, [C.c_char_p]),
+    "msv_tuning_clear": (_I, []),
+    "msv_tuning_count": (_U32, []),
     "msv_segsort_config_info": (_I, [_U32, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32)]),
     "msv_alloc": (_I, [_P, _I, _U32, _PP]),
     "msv_to_device": (_I, [_P, _I, _P, _U32, _PP]),

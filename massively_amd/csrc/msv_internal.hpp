@@ -83,6 +83,8 @@ struct msv_executor {
   int device;
   int num_cus;
   hipStream_t stream;
+  // Device signature for tuning entries: architecture name and CU count.
+  char signature[96];
   // Second stream for work one call runs beside its main work (created on
   // first use, default priority), and the events that fork it from and join
   // it back to `stream`.
@@ -253,6 +255,14 @@ msv_status unique_runs_run(Ctx& c, msv_dtype kdt, const void* keys, size_t vsize
 // Once per device, with `device` current: resolves the device address of the
 // static scratch array of the no-allocation entry points and clears it.
 msv_status noalloc_setup(int device, hipStream_t stream);
+
+// Autotuned choices, per device signature, family (a short word such as
+// "segsort") and key: -1 when absent. With MSV_TUNING_CACHE set, the file it
+// names is read at first use and rewritten after every new entry.
+int tune_get(const msv_executor* ex, const char* family, uint32_t key);
+void tune_put(const msv_executor* ex, const char* family, uint32_t key, int choice);
+// The same lookup for the current device (outside any executor).
+int tune_get_current(const char* family, uint32_t key);
 // Enqueues a batch of validated-on-entry jobs (msv_run_jobs).
 msv_status run_jobs(Ctx& c, const msv_job* jobs, uint32_t count);
 // Stable sort of every segment of offsets[0..nseg] on its own (mode 1 Less,

@@ -356,6 +356,9 @@ msv_status msv_executor_create(int32_t device, msv_executor** out) {
   ex->id = g_next_executor_id.fetch_add(1);
   ex->device = device;
   ex->num_cus = prop.multiProcessorCount;
+  snprintf(ex->signature, sizeof ex->signature, "%s/%d", prop.gcnArchName, prop.multiProcessorCount);
+  for (char* q = ex->signature; *q; ++q)
+    if (*q == ' ') *q = '_';
   ex->refs = 1;
   ex->destroyed = false;
   ex->d_counter = nullptr;

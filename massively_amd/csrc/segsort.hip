@@ -22,7 +22,6 @@
 // candidates, chosen per (key width, element width, mean segment length
 // class) by timing them on first use.
 #include <algorithm>
-#include <mutex>
 
 #include "msv_internal.hpp"
 #include "segsort_huge.hpp"
@@ -425,15 +424,14 @@ static msv_status segsort_config_go(Ctx& c, int k, const SegSortArgs& a, uint32_
   return MSV_ERR_INVALID_ARGUMENT;
 }
 
-// Process-wide autotuner table: best candidate per (key words, element words,
-// launch mode, log2 of the mean segment length).
-static std::mutex g_tune_mu;
-static std::map<uint32_t, int> g_tune;
-
-static uint32_t tune_key(int kw, int ew, int64_t launch, uint32_t n, uint32_t nseg) {
+// Autotuner key (tuning table family "segsort"): key words, element words,
+// launch mode, log2 of the mean segment length, and whether short segments
+// are sorted inside a wave.
+static uint32_t tune_key(int kw, int ew, int64_t launch, uint32_t n, uint32_t nseg, int64_t wave) {
   const uint32_t mean = nseg ? n / nseg : n;
   const uint32_t cls = mean ? 31u - (uint32_t)__builtin_clz(mean) : 0u;
-  return (uint32_t)kw | ((uint32_t)ew << 4) | ((uint32_t)launch << 8) | (cls << 12);
+  return (uint32_t)kw | ((uint32_t)ew << 4) | ((uint32_t)launch << 8) | (cls << 12) |
+         ((wave ? 1u : 0u) << 20);
 }
 
 template <int KW, int EW>
@@ -442,12 +440,11 @@ static msv_status segsort_dispatch(Ctx& c, const SegSortArgs& a, uint32_t n, uin
   msv_executor* ex = c.ex();
   if (ex->opt_segsort_config >= 0)
     return segsort_config_go<KW, EW>(c, (int)ex->opt_segsort_config, a, n, nseg, offsets);
-  const uint32_t key = tune_key(KW, EW, ex->opt_seg_launch, n, nseg);
-  {
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    auto it = g_tune.find(key);
-    if (it != g_tune.end()) return segsort_config_go<KW, EW>(c, it->second, a, n, nseg, offsets);
-  }
+  const uint32_t key = tune_key(KW, EW, ex->opt_seg_launch, n, nseg, ex->opt_segsort_wave);
+  const int tuned = tune_get(ex, "segsort", key);
+  if (tuned >= 0 && tuned < (int)kSegSortNumConfigs &&
+      segsort_fits(EW, kSegSortConfigs[tuned].block * kSegSortConfigs[tuned].ipt))
+    return segsort_config_go<KW, EW>(c, tuned, a, n, nseg, offsets);
   // Timing needs host synchronisation, so a graph capture uses the default.
   if (!ex->opt_autotune || n < (1u << 16) || ex->capturing)
     return segsort_config_go<KW, EW>(c, kSegSortDefault, a, n, nseg, offsets);
@@ -480,10 +477,7 @@ static msv_status segsort_dispatch(Ctx& c, const SegSortArgs& a, uint32_t n, uin
   }
   MSV_HIP(hipEventDestroy(e0));
   MSV_HIP(hipEventDestroy(e1));
-  {
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[key] = best;
-  }
+  tune_put(ex, "segsort", key, best);
   return segsort_config_go<KW, EW>(c, best, a, n, nseg, offsets);
 }
 
@@ -522,9 +516,13 @@ uint32_t msv_segsort_config_count(void) { return msv::kSegSortNumConfigs; }
 int32_t msv_segsort_tuned_config(uint32_t key_bytes, uint32_t value_bytes, int64_t launch,
                                  uint32_t n, uint32_t nseg) {
   const int kw = (int)key_bytes / 4, ew = kw + (int)value_bytes / 4;
-  std::lock_guard<std::mutex> g(msv::g_tune_mu);
-  auto it = msv::g_tune.find(msv::tune_key(kw, ew, launch, n, nseg));
-  return it == msv::g_tune.end() ? -1 : it->second;
+  return msv::tune_get_current("segsort", msv::tune_key(kw, ew, launch, n, nseg, 1));
+}
+
+int32_t msv_segsort_tuned_config_windows(uint32_t key_bytes, uint32_t value_bytes, int64_t launch,
+                                         uint32_t n, uint32_t nseg) {
+  const int kw = (int)key_bytes / 4, ew = kw + (int)value_bytes / 4;
+  return msv::tune_get_current("segsort", msv::tune_key(kw, ew, launch, n, nseg, 0));
 }
 
 msv_status msv_segsort_config_info(uint32_t k, uint32_t* block, uint32_t* items_per_thread,

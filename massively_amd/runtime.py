@@ -287,6 +287,27 @@ class GraphCapture:
         return False
 
 
-def segsort_tuned_config(key_bytes: int, value_bytes: int, launch: int, n: int, nseg: int) -> int:
-    """Index of the seg::Sort candidate the autotuner chose for this shape (-1: not tuned)."""
-    return int(L.lib().msv_segsort_tuned_config(key_bytes, value_bytes, launch, n, nseg))
+def segsort_tuned_config(key_bytes: int, value_bytes: int, launch: int, n: int, nseg: int,
+                         windows: bool = False) -> int:
+    """Index of the seg::Sort candidate the autotuner chose on the current device
+    for this shape (-1: not tuned); ``windows`` for the window-packing form."""
+    f = L.lib().msv_segsort_tuned_config_windows if windows else L.lib().msv_segsort_tuned_config
+    return int(f(key_bytes, value_bytes, launch, n, nseg))
+
+
+def tuning_save(path: str) -> None:
+    """Writes the autotuner's table (every device's entries) to ``path``."""
+    check(L.lib().msv_tuning_save(str(path).encode()))
+
+
+def tuning_load(path: str) -> None:
+    """Merges the entries of a file written by :func:`tuning_save`."""
+    check(L.lib().msv_tuning_load(str(path).encode()))
+
+
+def tuning_clear() -> None:
+    check(L.lib().msv_tuning_clear())
+
+
+def tuning_count() -> int:
+    return int(L.lib().msv_tuning_count())

@@ -9,7 +9,7 @@ import pytest
 
 import massively_amd as m
 from massively_amd import _lib as L
-from massively_amd.error import from_status
+from massively_amd.error import InvalidArgument, from_status
 
 ROOT = Path(__file__).resolve().parent.parent
 HEADER = (ROOT / "include" / "msv.h").read_text()
@@ -97,3 +97,28 @@ def test_host_mirror_surface():
         assert hasattr(m.Executor, a), a
     cfgs = m.segsort_configs()
     assert len(cfgs) >= 2 and all(b % 64 == 0 and i > 0 and d in (4, 8) for b, i, d in cfgs)
+
+
+def test_tuning_file_round_trip(tmp_path):
+    """The autotuner's table saves to and merges from a text file (entries of
+    any device signature), rejects a malformed line, and clears."""
+    m.tuning_clear()
+    f = tmp_path / "tune.txt"
+    f.write_text("# comment\ngfx950:sramecc+:xnack-/256 segsort 1048849 3\n"
+                 "gfx942:sramecc+:xnack-/304 segsort 1048849 1\n")
+    m.tuning_load(f)
+    assert m.tuning_count() == 2
+    g = tmp_path / "tune2.txt"
+    m.tuning_save(g)
+    lines = sorted(x for x in g.read_text().splitlines() if not x.startswith("#"))
+    assert lines == ["gfx942:sramecc+:xnack-/304 segsort 1048849 1",
+                     "gfx950:sramecc+:xnack-/256 segsort 1048849 3"]
+    m.tuning_clear()
+    assert m.tuning_count() == 0
+    bad = tmp_path / "bad.txt"
+    bad.write_text("not an entry\n")
+    with pytest.raises(InvalidArgument):
+        m.tuning_load(bad)
+    with pytest.raises(InvalidArgument):
+        m.tuning_load(tmp_path / "missing.txt")
+    m.tuning_clear()

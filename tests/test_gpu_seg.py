@@ -320,3 +320,31 @@ def test_seg_sort_config_2p26_autotuned(exec_):
     perm = oracle.seg_sort_perm(keys, off, "less")
     assert exec_.to_host(out.values[0]).tobytes() == keys[perm].tobytes()
     assert exec_.to_host(out.values[1]).tobytes() == vals[perm].tobytes()
+
+
+def test_tuning_cache_round_trip(exec_, tmp_path):
+    """The seg::Sort autotuner's choice is saved, cleared and loaded back (the
+    lookup then returns it without timing again), and sorts stay exact."""
+    m.tuning_clear()
+    L = RNG.integers(0, 200, 1 << 12)
+    off = offsets_of(L)
+    n = int(off[-1])
+    keys = RNG.integers(0, 2**32, n, dtype=np.uint32)
+    vals = RNG.standard_normal(n).astype(np.float32)
+    sg = seg.Segmentation.from_offsets(exec_, exec_.to_device(off))
+    it = seg.SegmentIterator((exec_.to_device(keys), exec_.to_device(vals)), sg)
+    launch = m.Executor.SEG_LAUNCH_PERSISTENT
+    out = seg.ForEachSegment(seg.Sort(m.op.Less)).run(exec_, it)
+    k = m.segsort_tuned_config(4, 4, launch, n, len(L))
+    assert 0 <= k < len(m.segsort_configs())
+    path = tmp_path / "tune.txt"
+    m.tuning_save(path)
+    m.tuning_clear()
+    assert m.segsort_tuned_config(4, 4, launch, n, len(L)) == -1
+    m.tuning_load(path)
+    assert m.segsort_tuned_config(4, 4, launch, n, len(L)) == k
+    again = seg.ForEachSegment(seg.Sort(m.op.Less)).run(exec_, it)
+    perm = oracle.seg_sort_perm(keys, off, "less")
+    for o in (out, again):
+        assert exec_.to_host(o.values[0]).tobytes() == keys[perm].tobytes()
+        assert exec_.to_host(o.values[1]).tobytes() == vals[perm].tobytes()
