@@ -15,12 +15,12 @@
 //     msv_run_jobs call, the copy_where counts copied to pinned memory, an
 //     event) and returns to step 1 without waiting.
 // While batches run, new arrivals are admitted onto the other streams, so
-// their work overlaps on the device. A large request owns the device by
-// itself, so at most half of the streams run large requests at once; small
-// requests behind a large head that cannot be admitted (that cap, or the
-// budget) are taken from up to kServeBypass positions behind it, so they do
-// not wait for the large ones (bounded bypass: the head keeps its place and
-// is admitted as soon as a large slot and the budget allow).
+// their work overlaps on the device. A large request fills the device by
+// itself, so while small requests wait at most half of the streams run large
+// requests; small requests behind a large head that cannot be admitted (that
+// cap, or the budget) are taken from up to kServeBypass positions behind it,
+// so they do not wait for the large ones (bounded bypass: the head keeps its
+// place and is admitted as soon as a large slot and the budget allow).
 // MSV_SERVE_SERIAL instead runs one request at a time through the device-wide
 // kernels, and MSV_SERVE_PER_JOB runs one request per launch through the
 // device-wide kernels on every stream (concurrency without batching): the two
@@ -281,8 +281,14 @@ static void scheduler(msv_server* s) {
         uint32_t large_busy = 0;
         for (const Slot& o : s->slots) large_busy += (o.busy && !o.reqs.empty() && !is_small(o.reqs[0]->r)) ? 1u : 0u;
         const uint32_t large_cap = std::max<uint32_t>(1u, (uint32_t)s->slots.size() / 2);
+        // The cap on large requests only holds while small ones wait within
+        // the bypass window; with none waiting, large requests take any free
+        // stream.
+        bool small_waiting = false;
+        for (size_t i = 0; i < s->pending.size() && i < kServeBypass && !small_waiting; ++i)
+          small_waiting = is_small(s->pending[i]->r);
         Request* head = s->pending.front();
-        if (!is_small(head->r) && large_busy < large_cap &&
+        if (!is_small(head->r) && (large_busy < large_cap || !small_waiting) &&
             s->used + head->bytes <= s->cfg.hbm_budget) {
           s->used += head->bytes;
           s->pending.pop_front();
