@@ -1,5 +1,5 @@
-// LSD radix sort (8-bit digits, reduce-then-scan passes) for gfx950: vector::radix_sort_by_key, sort_by_key
-// and sort with the built-in comparators.
+// LSD radix sort (8-bit digits) for gfx950: vector::radix_sort_by_key,
+// sort_by_key and sort with the built-in comparators.
 //
 // Reference: massively/src/core/radix.rs:369-546 (4-bit digits, 4 launches
 // per pass, every pass reading keys through a permutation, a 1-thread bucket
@@ -8,26 +8,27 @@
 // IEEE total order for floats. For the comparator forms (`Less`/`Greater`)
 // -0.0 and +0.0 compare equal, so they are ordered as equal keys (stably).
 //
-// Pipeline (keys K of 4 or 8 bytes, optional 4/8-byte values):
-//  1. pass 0's upsweep reads the caller's keys and also records, per tile,
-//     the OR and the AND of the order words;
-//  2. plan (one block): a digit position on which OR and AND agree over all
-//     keys holds one value and is skipped; the ping-pong source/destination
-//     of every remaining pass is written into a device-side table, so no
-//     host round trip is needed;
-//  3. per remaining digit, reduce-then-scan: an upsweep counts the digits of
-//     every tile (512 threads x 16 or 8 keys) into a digit-major
-//     count matrix, one exclusive scan of that matrix (the library's lookback
-//     scan) gives every (digit, tile) run its global position, and the
-//     downsweep ranks each tile's keys stably inside each wave by matching
-//     8-bit digits with ballots (wave-level multi-split), reorders the
-//     elements through LDS into digit-sorted runs and writes the runs out.
-//     Keys and values travel together as one interleaved element between
-//     passes (one reorder and one stream of >= 128-byte runs per pass); the
-//     first pass reads the caller's arrays and the last writes them. No tile
-//     waits on another: a per-digit decoupled lookback measured ~2 ms per
-//     2^28-key pass on MI355X, because the per-XCD L2s send every status poll
-//     to memory while ~1000 tiles are in flight.
+// Keys and values travel together as one interleaved element between passes
+// (one reorder and one stream of runs per pass); the first pass reads the
+// caller's arrays and the last writes them. Digit positions on which every key
+// has the same digit are skipped (no host round trip: a device-side table
+// gives every remaining pass its source and destination). Every pass ranks a
+// tile's keys stably inside each wave by matching 8-bit digits with ballots
+// (wave-level multi-split) and reorders the tile through LDS into digit runs.
+// Two schedules:
+//  * onesweep (default for 4-byte keys): one read of the keys builds the
+//    histograms of every digit position (per-wave LDS histograms), a one-block
+//    plan scans them into digit starts and the pass table, then each pass
+//    claims tiles in order, publishes a tile's digit counts before ranking it,
+//    and finds each digit's global position by a decoupled look-back over the
+//    earlier tiles (4 status words per step);
+//  * reduce-then-scan (default for 8-byte keys, whose Zipf-distributed keys
+//    sort 13% slower under onesweep here): pass 0's upsweep also records per
+//    tile the OR and AND of the order words (the skip test), and per digit an
+//    upsweep counts the digits of every tile into a digit-major matrix, the
+//    library's scan turns it into run starts, and a downsweep ranks and writes;
+//    no tile waits on another.
+// MSV_OPT_RADIX_ONESWEEP selects either.
 #include "msv_internal.hpp"
 #include "radix_kernels.hpp"
 
