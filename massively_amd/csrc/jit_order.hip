@@ -228,9 +228,9 @@ static msv_status cmp_set(Ctx& c, const msv_user_op* op, int kind, const void* A
     MSV_HIP(hipGetLastError());
     return MSV_OK;
   }
-  uint32_t ntiles = (uint32_t)((total + kSetTile - 1) / kSetTile);
+  uint32_t ntiles = (uint32_t)((total + kSetSpan - 1) / kSetSpan);
   uint32_t *sa, *sb;
-  MSV_TRY(cmp_partition(c, op->f_set_part, A, na, B, nb, ntiles, kSetTile, &sa, &sb));
+  MSV_TRY(cmp_partition(c, op->f_set_part, A, na, B, nb, ntiles, kSetSpan, &sa, &sb));
   uint64_t *counter, *status;
   uint32_t gen;
   MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));

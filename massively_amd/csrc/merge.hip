@@ -50,9 +50,9 @@ static msv_status set_impl(Ctx& c, int kind, const void* A, uint32_t na, const v
     MSV_HIP(hipGetLastError());
     return MSV_OK;
   }
-  const uint32_t ntiles = (uint32_t)((total + kSetTile - 1) / kSetTile);
+  const uint32_t ntiles = (uint32_t)((total + kSetSpan - 1) / kSetSpan);
   uint32_t *sa, *sb;
-  MSV_TRY((partition<K, MODE, true>(c, (const K*)A, na, (const K*)B, nb, ntiles, kSetTile, &sa,
+  MSV_TRY((partition<K, MODE, true>(c, (const K*)A, na, (const K*)B, nb, ntiles, kSetSpan, &sa,
                                     &sb)));
   uint64_t *counter, *status;
   uint32_t gen;

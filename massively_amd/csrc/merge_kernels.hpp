@@ -20,13 +20,19 @@
 // multiset rules core/set.rs:18-46, oracle vector.rs:595-664): the oracle walks
 // both inputs once, pairing equal keys (the k-th occurrence on the left with
 // the k-th on the right). Splitting that walk is valid at any point that does
-// not separate a matched pair, so tile and thread splits are taken from the
-// merge path and then moved, inside a run of equal keys, to the nearest
-// pair-aligned point (both sides consumed equally, or one side's run
-// exhausted). Each thread then runs the oracle's loop on its own range; the
-// emitted counts are compacted with a block scan plus a decoupled lookback
-// over tiles, and the block of the last tile stores the output's
-// device-resident length.
+// not separate a matched pair, so tile splits are taken from the merge path
+// and then moved, inside a run of equal keys, to the nearest pair-aligned
+// point (both sides consumed equally, or one side's run exhausted). Inside a
+// tile that starts and ends pair-aligned, the walk's decision for an element
+// depends only on counts within the tile: a left copy of rank r among the
+// tile's left copies of its key is paired (kept by the intersection, dropped
+// by the difference) iff r is below the number of right copies in the tile,
+// and a right copy of rank r is unpaired (kept by the union) iff r reaches
+// the number of left copies. Threads therefore walk plain merge-path ranges of
+// the tile (ties: left first) and flag each element from the neighbouring
+// keys the walk already holds; the emitted counts are compacted with a block
+// scan plus a decoupled lookback over tiles, and the block of the last tile
+// stores the output's device-resident length.
 //
 // Comparison sort (sort / sort_by_key with a user comparator; reference:
 // core/ordering/mod.rs:325-748, stable, ties keep input order): a block sorts
@@ -51,6 +57,9 @@ constexpr uint32_t kMergeTile = kMergeBlock * kMergeIpt;
 // tiles (fewer tiles in the look-back chain).
 constexpr int kSetIpt = 32;
 constexpr uint32_t kSetTile = kSetBlock * kSetIpt;
+// Tile diagonals are kSetSpan apart: moving a split to a pair-aligned point
+// lowers its diagonal by at most one, so a tile holds at most kSetTile inputs.
+constexpr uint32_t kSetSpan = kSetTile - 1;
 // Block sort of the comparison sort: 512 threads x 8 elements. Two of its
 // tiles make one merge tile, so every merge-pass tile lies inside one pair
 // of runs.
@@ -511,6 +520,101 @@ __device__ uint32_t set_tile_count(const K* __restrict__ A, const K* __restrict_
   return wave_reduce<uint32_t, OP_ADD>(cnt);
 }
 
+// Rank of v among the left copies of its run below `hi`, and the number of
+// copies from `lo` on: searches for the rare case of a key repeated inside
+// one side that also occurs on the other (out of line: the walk stays small).
+template <typename K, typename Cmp, typename W>
+__device__ __attribute__((noinline)) uint32_t set_run_lower(const W& X, uint32_t hi, K v) {
+  return lower_bound_c<K, Cmp>(X, 0, hi, v);
+}
+template <typename K, typename Cmp, typename W>
+__device__ __attribute__((noinline)) uint32_t set_run_upper(const W& X, uint32_t lo, uint32_t hi,
+                                                           K v) {
+  return upper_bound_c<K, Cmp>(X, lo, hi, v);
+}
+
+// Merge walk of outputs [d, d + kSetIpt) of the tile's merge (ties: left
+// first); step q stores the element it visits in ok[q] and sets bit q of
+// `emit` if the operation outputs it. A left element a has right copies in
+// the tile iff the right head equals it (every smaller right key was taken
+// before it); its rank is 0 unless it repeats the previous left key. A right
+// element b has left copies iff the last left key taken equals it. Only a key
+// repeated on one side and present on the other needs searches, and their
+// result carries along the run.
+template <typename K, typename Cmp, int KIND>
+__device__ inline void set_flag_walk(const PadWindow<K>& LA, uint32_t la, const PadWindow<K>& LB,
+                                     uint32_t lb, uint32_t d, uint32_t L, K* ok, uint64_t& emit) {
+  uint32_t ia = merge_path<K, Cmp>(LA, la, LB, lb, d);
+  uint32_t jb = d - ia;
+  K a = LA[ia < la ? ia : 0u], b = LB[jb < lb ? jb : 0u];
+  bool hpa = ia > 0, hpb = jb > 0;
+  K pa = LA[hpa ? ia - 1 : 0u], pb = LB[hpb ? jb - 1 : 0u];
+  // (rank, other side's count) of the run of pa / pb, valid when va / vb.
+  uint32_t ra = 0, ca = 0, rb = 0, cb = 0;
+  bool va = false, vb = false;
+  emit = 0;
+#pragma unroll
+  for (int q = 0; q < kSetIpt; ++q) {
+    const bool act = d + q < L;
+    const bool ha = ia < la, hb = jb < lb;
+    const bool take_a = ha && (!hb || !Cmp::less(b, a));
+    bool keep;
+    if (take_a) {
+      const bool eqb = hb && !Cmp::less(a, b);    // the tile's right side holds a copy of a
+      const bool dup = hpa && !Cmp::less(pa, a);  // a repeats the previous left key
+      uint32_t r = 0, cnt = eqb ? 1u : 0u;         // 1 stands for "at least one"
+      if (eqb && dup) {
+        if (va) {
+          r = ra + 1;
+          cnt = ca;
+        } else {
+          r = ia - set_run_lower<K, Cmp>(LA, ia, a);
+          cnt = set_run_upper<K, Cmp>(LB, jb, lb, a) - jb;
+        }
+      }
+      va = !eqb || dup;  // exact unless a first copy met a right copy
+      ra = r;
+      ca = cnt;
+      keep = KIND == SET_UNION ? true : (KIND == SET_INTERSECTION ? r < cnt : r >= cnt);
+      ok[q] = a;
+      if (act) {
+        pa = a;
+        hpa = true;
+        ++ia;
+        a = LA[ia < la ? ia : 0u];
+      }
+    } else {
+      keep = false;
+      if constexpr (KIND == SET_UNION) {
+        const bool eqa = hpa && !Cmp::less(pa, b);  // the tile's left side holds a copy of b
+        const bool dup = hpb && !Cmp::less(pb, b);
+        uint32_t r = 0, cnt = eqa ? 1u : 0u;
+        if (eqa && dup) {
+          if (vb) {
+            r = rb + 1;
+            cnt = cb;
+          } else {
+            r = jb - set_run_lower<K, Cmp>(LB, jb, b);
+            cnt = ia - set_run_lower<K, Cmp>(LA, ia, b);
+          }
+        }
+        vb = !eqa || dup;
+        rb = r;
+        cb = cnt;
+        keep = r >= cnt;
+      }
+      ok[q] = b;
+      if (act) {
+        pb = b;
+        hpb = true;
+        ++jb;
+        b = LB[jb < lb ? jb : 0u];
+      }
+    }
+    emit |= (uint64_t)((act && keep) ? 1u : 0u) << q;
+  }
+}
+
 template <typename K, typename Cmp, int KIND>
 __global__ __launch_bounds__(kSetBlock) void set_tile_kernel(
     const K* __restrict__ A, uint32_t na, const K* __restrict__ B, uint32_t nb,
@@ -518,9 +622,8 @@ __global__ __launch_bounds__(kSetBlock) void set_tile_kernel(
     uint32_t* __restrict__ out_len, uint64_t* counter, uint64_t* status, uint32_t gen,
     uint32_t ntiles) {
   constexpr int NW = kSetBlock / kWave;
-  constexpr uint32_t CAP = LdsPadded<kSetTile + 2>::value;
+  constexpr uint32_t CAP = LdsPadded<kSetTile>::value;
   __shared__ K s_keys[CAP];
-  __shared__ uint32_t s_split[kSetBlock + 1][2];
   __shared__ uint32_t s_wave[NW];
   __shared__ uint32_t s_prefix;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid / 64u;
@@ -529,57 +632,18 @@ __global__ __launch_bounds__(kSetBlock) void set_tile_kernel(
   (void)nb;
   const uint32_t b = blockIdx.x;
   const uint32_t i0 = sa[b], i1 = sa[b + 1], j0 = sb[b], j1 = sb[b + 1];
-  const uint32_t la = i1 - i0, lb = j1 - j0, L = la + lb;
-  // A set tile holds up to kSetTile + 2 inputs (pair-aligned ends).
-  load_windows<K, kSetIpt, kSetBlock>(A + i0, la, B + j0, min(L, (uint32_t)kSetTile), s_keys, tid);
-  for (uint32_t k = kSetTile + tid; k < L; k += kSetBlock)
-    s_keys[lds_pad(k)] = k < la ? A[i0 + k] : B[j0 + k - la];
+  // la + lb <= kSetTile by the kSetSpan spacing; the clamps only keep every
+  // LDS access inside the tile.
+  const uint32_t la = min(i1 - i0, kSetTile);
+  const uint32_t lb = min(j1 - j0, kSetTile - la);
+  const uint32_t L = la + lb;
+  load_windows<K, kSetIpt, kSetBlock>(A + i0, la, B + j0, L, s_keys, tid);
   __syncthreads();
   const PadWindow<K> LA{s_keys, 0};
   const PadWindow<K> LB{s_keys, la};
-  {
-    uint32_t ia, jb;
-    set_split<K, Cmp>(LA, la, LB, lb, min(tid * (uint32_t)kSetIpt, L), ia, jb);
-    s_split[tid][0] = ia;
-    s_split[tid][1] = jb;
-    if (tid == 0) {
-      s_split[kSetBlock][0] = la;
-      s_split[kSetBlock][1] = lb;
-    }
-  }
-  __syncthreads();
-  uint32_t ia = s_split[tid][0], jb = s_split[tid][1];
-  const uint32_t ia_e = s_split[tid + 1][0], jb_e = s_split[tid + 1][1];
-  // The oracle's loop (vector.rs:595-664) on this thread's range.
-  // A thread range holds at most kSetIpt + 2 steps (both of its ends can
-  // move by one when they are pair-aligned). Step q always writes ok[q] (the
-  // element it visits) and records in `emit` whether it is an output, so the
-  // register array is indexed statically (a running output index would put
-  // it in scratch memory).
-  constexpr int STEPS = kSetIpt + 2;
-  static_assert(STEPS <= 64, "emit mask width");
-  K ok[STEPS];
-  uint64_t emit = 0;
-#pragma unroll
-  for (int q = 0; q < STEPS; ++q) {
-    const bool ha = ia < ia_e, hb = jb < jb_e;
-    const K a = LA[ha ? ia : 0u];
-    const K b = LB[hb ? jb : 0u];
-    int step;  // 0: left only, 1: right only, 2: matched pair
-    if (ha && hb) {
-      step = Cmp::less(a, b) ? 0 : (Cmp::less(b, a) ? 1 : 2);
-    } else {
-      step = ha ? 0 : 1;
-    }
-    const bool act = ha || hb;
-    const bool out = act && (step == 0 ? KIND != SET_INTERSECTION
-                             : step == 1 ? KIND == SET_UNION
-                                         : KIND != SET_DIFFERENCE);
-    ok[q] = step == 1 ? b : a;
-    emit |= (uint64_t)(out ? 1u : 0u) << q;
-    ia += (act && step != 1) ? 1u : 0u;
-    jb += (act && step != 0) ? 1u : 0u;
-  }
+  K ok[kSetIpt];
+  uint64_t emit;
+  set_flag_walk<K, Cmp, KIND>(LA, la, LB, lb, min(tid * (uint32_t)kSetIpt, L), L, ok, emit);
   const uint32_t cnt = (uint32_t)__builtin_popcountll(emit);
   // Block scan of emitted counts, then lookback over tiles.
   const uint32_t incl = wave_inclusive_scan<uint32_t, OP_ADD>(cnt);
@@ -611,7 +675,7 @@ __global__ __launch_bounds__(kSetBlock) void set_tile_kernel(
   __syncthreads();  // everyone is done reading s_keys as input
   uint32_t my = woff + incl - cnt;
 #pragma unroll
-  for (int q = 0; q < STEPS; ++q) {
+  for (int q = 0; q < kSetIpt; ++q) {
     if ((emit >> q) & 1u) {
       s_keys[lds_pad(my)] = ok[q];
       ++my;
