@@ -327,22 +327,34 @@ static msv_status segsort_wave_go(Ctx& c, const SegSortArgs& base, uint32_t n, u
       }
     }
     // Enough waves to fill the device; each walks its share of the units.
+    // The multi-register classes run on the main stream; the one-slot classes
+    // follow the block and long paths on the forked stream, which otherwise
+    // ends well before the main one.
     const uint32_t ncu = (uint32_t)c.num_cus();
     const uint32_t sgrid = std::max(1u, std::min(div_up(nseg, kWaveSortWaves), ncu * 8u));
-    hipLaunchKernelGGL((wave_sort_large_kernel<KW, VW, 16>), dim3(sgrid), dim3(kWaveSortBlock), 0,
-                       c.stream(), w);
+    if constexpr (KW == 1) {
+      // 4-byte keys: a pair of waves per segment (5 waves per SIMD); with
+      // 8-byte keys its LDS would allow fewer waves than the one-wave form.
+      const uint32_t pgrid = std::max(1u, std::min(nseg, ncu * 10u));
+      hipLaunchKernelGGL((wave_sort_pair_kernel<KW, VW>), dim3(pgrid), dim3(kPairBlock), 0,
+                         c.stream(), w);
+    } else {
+      hipLaunchKernelGGL((wave_sort_large_kernel<KW, VW, 16>), dim3(sgrid), dim3(kWaveSortBlock),
+                         0, c.stream(), w);
+    }
     hipLaunchKernelGGL((wave_sort_large_kernel<KW, VW, 8>), dim3(sgrid), dim3(kWaveSortBlock), 0,
                        c.stream(), w);
     hipLaunchKernelGGL((wave_sort_large_kernel<KW, VW, 4>), dim3(sgrid), dim3(kWaveSortBlock), 0,
                        c.stream(), w);
     hipLaunchKernelGGL((wave_sort_large_kernel<KW, VW, 2>), dim3(sgrid), dim3(kWaveSortBlock), 0,
                        c.stream(), w);
+    hipStream_t small = longer ? aux : c.stream();
     hipLaunchKernelGGL((wave_sort_small_kernel<KW, VW, 6, 6>), dim3(sgrid), dim3(kWaveSortBlock), 0,
-                       c.stream(), w);
+                       small, w);
     hipLaunchKernelGGL((wave_sort_small_kernel<KW, VW, 5, 5>), dim3(sgrid), dim3(kWaveSortBlock), 0,
-                       c.stream(), w);
+                       small, w);
     hipLaunchKernelGGL((wave_sort_small_kernel<KW, VW, 0, 4>), dim3(sgrid), dim3(kWaveSortBlock), 0,
-                       c.stream(), w);
+                       small, w);
     const hipError_t le = hipGetLastError();
     if (longer) MSV_TRY(c.join());
     MSV_HIP(le);

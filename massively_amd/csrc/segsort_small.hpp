@@ -17,9 +17,9 @@
 // Work: a classify kernel counts the segments of every class (a histogram
 // per block, one global add per class), a fill kernel lists them per class
 // as (start, length), the block path's segments (1024 < L <= long limit) and
-// the long ones by id; the one-slot classes run in one launch and the
-// multi-register classes and classes 6 and 5 one launch per class, classes
-// 0..4 in one, longest class first. Wave w takes work
+// the long ones by id; the multi-register classes and classes 6 and 5 run
+// one launch per class, classes 0..4 in one, longest class first (with
+// 4-byte keys, class 10 by pairs of waves). Wave w takes work
 // units w, w + W, w + 2W, ... (a unit is 64 >> c segments of a one-slot
 // class or one segment of a larger class) and loads the next unit's
 // descriptors before it sorts the current one.
@@ -591,6 +591,72 @@ void wave_sort_large_kernel(WaveSortArgs a) {
       d0 = d1u;
       d1 = d2;
     }
+  }
+}
+
+// Class 10 (513..1024 elements) by a pair of waves per segment, 8 slots per
+// lane each (instead of one wave with 16 slots per lane, which holds 136-168
+// VGPRs and runs 3 waves per SIMD): wave h holds slots h * 512 + r * 64 +
+// lane. Both halves are sorted in opposite directions in registers (the
+// stages up to size 512), the size-1024 merge's first stage (distance 512)
+// pairs slot e with slot e ^ 512 of the other wave through LDS, and its
+// remaining stages stay in registers. One block is one pair, so its barriers
+// never depend on the other waves of a larger block.
+constexpr int kPairBlock = 2 * kWave;
+
+template <int KW, int VW>
+__global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void wave_sort_pair_kernel(WaveSortArgs a) {
+  constexpr int R = 8;
+  constexpr uint32_t C = 10;
+  __shared__ WaveStage<KW, VW, 1024> st;
+  __shared__ SortSlot<KW> xch[1024];
+  __shared__ uint32_t s_first;
+  if (threadIdx.x == 0) {
+    uint32_t first = 0;
+    for (uint32_t c = 0; c < C; ++c) first += a.counts[c];
+    s_first = first;
+  }
+  __syncthreads();
+  const uint32_t first = s_first, n = a.counts[C];
+  const uint32_t lane = __lane_id();
+  const uint32_t h = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+  for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {  // uniform over the block
+    const uint2 dd = a.list[first + k];
+    const size_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)dd.x);
+    const uint32_t L = (uint32_t)__builtin_amdgcn_readfirstlane((int)dd.y);
+    SortSlot<KW> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t e = h * 512u + (uint32_t)r * 64 + lane;
+      v[r] = slot_pad<KW>();
+      if (e < L) {
+        const KeyWord<KW> raw = load_raw_key<KW>(a.keys, g0 + e);
+        st.k[e] = raw;
+        if constexpr (VW > 0) st.v[e] = load_val<VW>(a.vals, g0 + e);
+        v[r] = slot_make<KW>(order_word<KeyWord<KW>>(raw, a.om), e);
+      }
+    }
+    uint32_t e0 = h * 512u + lane;
+    asm volatile("" : "+v"(e0));  // as in bitonic_wave
+    bitonic_sort<KW, 2, 512, R>(v, e0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) xch[h * 512u + (uint32_t)r * 64 + lane] = v[r];
+    __syncthreads();  // also publishes the staged input of both halves
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      cmp_keep<KW>(v[r], xch[(h ^ 1u) * 512u + (uint32_t)r * 64 + lane], h == 0);
+    bitonic_merge<KW, 1024, 256, R>(v, e0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t e = h * 512u + (uint32_t)r * 64 + lane;
+      if (e < L) {
+        const uint32_t p = slot_pos<KW>(v[r]);
+        ((KeyWord<KW>*)a.out_k)[g0 + e] = st.k[p];
+        if constexpr (VW > 0) ((typename ValWord<VW>::T*)a.out_v)[g0 + e] = st.v[p];
+      }
+    }
+    __syncthreads();  // before the next segment overwrites st and xch
   }
 }
 
