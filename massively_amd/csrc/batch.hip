@@ -46,6 +46,20 @@ struct BatchItem {
   void* s1;
 };
 
+// A batch table on the device: this header, then the jobs grouped by family
+// (family f at [start[f], start[f + 1]); 0 = scan / reduce / copy_where,
+// 1..6 = sorts by element width), then the generation entries.
+constexpr int kBatchFamilies = 7;
+struct BatchHeader {
+  uint32_t start[8];
+  uint32_t queue[8];  // per-family job counters, uploaded as zero
+  uint32_t ngen;      // generation entries
+  uint32_t gen_chunks;
+  uint32_t pad[6];
+};
+static_assert(sizeof(BatchHeader) % 16 == 0, "table alignment");
+constexpr uint32_t kBatchGenChunk = 256 * 8;  // elements per generation chunk
+
 // ---- scan ------------------------------------------------------------------
 template <typename T, int OP>
 __device__ void block_scan_job(char* smem, const BatchItem& it) {
@@ -226,15 +240,18 @@ __device__ inline uint32_t next_job(uint32_t* queue, uint32_t* s_item) {
   return j;
 }
 
+// Family f's jobs are items[hdr->start[f], hdr->start[f + 1]); its queue is
+// hdr->queue[f] (uploaded as zero with the table), so a launch's grid does
+// not depend on the batch and a captured launch can be replayed for any.
 __global__ __launch_bounds__(kBatchBlock) void batch_stream_kernel(const BatchItem* __restrict__ items,
-                                                                   uint32_t count,
-                                                                   uint32_t* queue) {
+                                                                   BatchHeader* hdr) {
   __shared__ __align__(16) char smem[sizeof(uint64_t) * (kBatchBlock / kWave + kBatchMaxPartials)];
   __shared__ uint32_t s_item;
+  const uint32_t first = hdr->start[0], count = hdr->start[1] - first;
   for (;;) {
-    const uint32_t j = next_job(queue, &s_item);
+    const uint32_t j = next_job(&hdr->queue[0], &s_item);
     if (j >= count) break;
-    const BatchItem it = items[j];
+    const BatchItem it = items[first + j];
     if (it.kind == MSV_JOB_COPY_WHERE) {
       MSV_BATCH_DTYPE(block_select_job, smem, it)
     } else {
@@ -243,17 +260,25 @@ __global__ __launch_bounds__(kBatchBlock) void batch_stream_kernel(const BatchIt
   }
 }
 
+// Family of the sort jobs with key width KW and element width EW (words).
+template <int KW, int EW>
+constexpr int sort_family() {
+  return KW == 1 ? EW : 2 + EW;  // (1,1) (1,2) (1,3) (2,2) (2,3) (2,4) -> 1..6
+}
+
 template <int KW, int EW>
 __global__ __launch_bounds__(kBatchBlock) void batch_sort_kernel(const BatchItem* __restrict__ items,
-                                                                 uint32_t count, uint32_t* queue) {
+                                                                 BatchHeader* hdr) {
   using T = TileSmem<EW, kBatchBlock, kBatchSortIpt, 8>;
   using R = RangeSmem<EW, kBatchBlock, kBatchSortIpt>;
+  constexpr int F = sort_family<KW, EW>();
   __shared__ __align__(16) char smem[sizeof(T) > sizeof(R) ? sizeof(T) : sizeof(R)];
   __shared__ uint32_t s_item;
+  const uint32_t first = hdr->start[F], count = hdr->start[F + 1] - first;
   for (;;) {
-    const uint32_t j = next_job(queue, &s_item);
+    const uint32_t j = next_job(&hdr->queue[F], &s_item);
     if (j >= count) break;
-    const BatchItem it = items[j];
+    const BatchItem it = items[first + j];
     if (it.n == 0) continue;
     OrderMap om;
     om.kind = (it.dtype == MSV_F32 || it.dtype == MSV_F64) ? KEY_FLOAT
@@ -269,6 +294,51 @@ __global__ __launch_bounds__(kBatchBlock) void batch_sort_kernel(const BatchItem
                                                      (Elem<EW>*)it.s0, (Elem<EW>*)it.s1, it.n, om);
     }
   }
+}
+
+// Input generation of a batch table: chunk c of all entries' elements, block
+// by block (entry found by binary search over the entries' first chunks).
+__global__ __launch_bounds__(256) void batch_gen_kernel(const BatchGen* __restrict__ gens,
+                                                        const BatchHeader* __restrict__ hdr) {
+  const uint32_t ngen = hdr->ngen, chunks = hdr->gen_chunks;
+  for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+    uint32_t lo = 0, hi = ngen;  // last entry with chunk0 <= c
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (gens[mid].chunk0 <= c) lo = mid;
+      else hi = mid;
+    }
+    const BatchGen g = gens[lo];
+    const uint32_t base = (c - g.chunk0) * kBatchGenChunk;
+    for (uint32_t i = base + threadIdx.x; i < min(base + kBatchGenChunk, g.n); i += 256) {
+      if (g.in) {
+        const uint32_t v = pcg_hash32(i ^ g.key);
+        switch (g.dtype) {
+          case MSV_U32:
+          case MSV_I32: ((uint32_t*)g.in)[i] = v; break;
+          case MSV_F32: ((uint32_t*)g.in)[i] = 0x3F000000u | (v % 0x00800000u); break;
+          case MSV_U64:
+          case MSV_I64:
+          case MSV_F64: {
+            const uint64_t w = (uint64_t)v | ((uint64_t)pcg_hash32(i ^ g.key ^ 0x9e3779b9u) << 32);
+            ((uint64_t*)g.in)[i] =
+                g.dtype == MSV_F64 ? 0x3FE0000000000000ull | (w % 0x0010000000000000ull) : w;
+            break;
+          }
+          default: break;
+        }
+      }
+      if (g.iota) g.iota[i] = i;
+    }
+  }
+}
+
+// Count gather of a batch table: counts[count_index] = *count_src.
+__global__ __launch_bounds__(256) void batch_counts_kernel(const BatchGen* __restrict__ gens,
+                                                           const BatchHeader* __restrict__ hdr,
+                                                           uint32_t* __restrict__ counts) {
+  for (uint32_t g = threadIdx.x; g < hdr->ngen; g += 256)
+    if (gens[g].count_src) counts[gens[g].count_index] = *gens[g].count_src;
 }
 
 // ---- host ------------------------------------------------------------------
@@ -325,22 +395,6 @@ static BatchItem to_item(const msv_job& j) {
   return it;
 }
 
-// One persistent launch over `items` (already on the device at d_items).
-template <typename Kern>
-static msv_status launch_family(Ctx& c, Kern kern, const BatchItem* d_items, uint32_t count,
-                                uint32_t* queue, int* per_cu_cache) {
-  if (count == 0) return MSV_OK;
-  if (*per_cu_cache == 0) {
-    int b = 0;
-    MSV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, kBatchBlock, 0));
-    *per_cu_cache = std::max(1, b);
-  }
-  const uint32_t grid = std::min<uint32_t>(count, (uint32_t)(*per_cu_cache * c.num_cus()));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBatchBlock), 0, c.stream(), d_items, count, queue);
-  MSV_HIP(hipGetLastError());
-  return MSV_OK;
-}
-
 // Element-width class of a sort job: 0..5 = (KW,EW) in (1,1) (1,2) (1,3) (2,2) (2,3) (2,4).
 static int sort_class(const msv_job& j) {
   const int kw = (int)dtype_size((msv_dtype)j.dtype) / 4;
@@ -366,65 +420,157 @@ static msv_status large_job(Ctx& c, const msv_job& j) {
   }
 }
 
-msv_status run_jobs(Ctx& c, const msv_job* jobs, uint32_t count) {
-  for (uint32_t i = 0; i < count; ++i) MSV_TRY(check_job(jobs[i], i));
-  // Families: 0 = scan/reduce/copy_where, 1 + class = sort by element width.
-  std::vector<uint32_t> fam[7];
-  std::vector<uint32_t> large;
-  for (uint32_t i = 0; i < count; ++i) {
-    const msv_job& j = jobs[i];
-    if (!job_is_small(j)) {
-      large.push_back(i);
-      continue;
-    }
-    fam[j.kind == MSV_JOB_SORT_BY_KEY ? 1 + sort_class(j) : 0].push_back(i);
+bool batch_job_small(const msv_job& j) { return job_is_small(j); }
+
+size_t batch_table_bytes(uint32_t capacity) {
+  return sizeof(BatchHeader) + (size_t)capacity * (sizeof(BatchItem) + sizeof(BatchGen));
+}
+
+msv_status batch_table_fill(Ctx& c, const msv_job* jobs, uint32_t count, const BatchGen* gens,
+                            uint32_t ngen, uint32_t capacity, char* host, uint32_t* mask) {
+  if (count > capacity || ngen > capacity) {
+    set_error(MSV_ERR_LENGTH_TOO_LARGE, count, capacity, "batch larger than its table");
+    return MSV_ERR_LENGTH_TOO_LARGE;
   }
-  std::vector<BatchItem> items;
-  uint32_t start[8] = {0};
-  size_t sort_scratch = 0;
-  for (int f = 0; f < 7; ++f) {
+  std::vector<uint32_t> fam[kBatchFamilies];
+  for (uint32_t i = 0; i < count; ++i) {
+    MSV_TRY(check_job(jobs[i], i));
+    if (!job_is_small(jobs[i])) {
+      set_error(MSV_ERR_INVALID_ARGUMENT, i, 0, "job %u is too large for a batch table", i);
+      return MSV_ERR_INVALID_ARGUMENT;
+    }
+    fam[jobs[i].kind == MSV_JOB_SORT_BY_KEY ? 1 + sort_class(jobs[i]) : 0].push_back(i);
+  }
+  BatchHeader* hdr = (BatchHeader*)host;
+  BatchItem* items = (BatchItem*)(host + sizeof(BatchHeader));
+  BatchGen* tg = (BatchGen*)(host + sizeof(BatchHeader) + (size_t)capacity * sizeof(BatchItem));
+  *hdr = BatchHeader{};
+  *mask = 0;
+  uint32_t k = 0;
+  for (int f = 0; f < kBatchFamilies; ++f) {
     // Longest first: the queue hands them out before the short ones.
     std::stable_sort(fam[f].begin(), fam[f].end(),
                      [&](uint32_t a, uint32_t b) { return jobs[a].n > jobs[b].n; });
-    start[f] = (uint32_t)items.size();
+    hdr->start[f] = k;
+    if (!fam[f].empty()) *mask |= 1u << f;
     for (uint32_t i : fam[f]) {
-      items.push_back(to_item(jobs[i]));
-      const msv_job& j = jobs[i];
-      if (j.kind == MSV_JOB_SORT_BY_KEY && j.n > kBatchSortTile) {
-        const size_t ew = dtype_size((msv_dtype)j.dtype) + j.value_bytes;
-        sort_scratch += 2 * (((size_t)j.n * ew + 255) & ~(size_t)255);
+      BatchItem it = to_item(jobs[i]);
+      if (it.kind == MSV_JOB_SORT_BY_KEY && it.n > kBatchSortTile) {
+        const size_t bytes =
+            ((size_t)it.n * (dtype_size((msv_dtype)it.dtype) + it.vbytes) + 255) & ~(size_t)255;
+        char* sc = (char*)c.scratch(2 * bytes);
+        if (!sc) return MSV_ERR_OUT_OF_MEMORY;
+        it.s0 = sc;
+        it.s1 = sc + bytes;
       }
+      items[k++] = it;
     }
   }
-  start[7] = (uint32_t)items.size();
-  if (!items.empty()) {
-    BatchItem* d_items = (BatchItem*)c.scratch(sizeof(BatchItem) * items.size());
-    uint32_t* queues = (uint32_t*)c.scratch(sizeof(uint32_t) * 8);
-    char* scratch = sort_scratch ? (char*)c.scratch(sort_scratch) : nullptr;
-    if (!d_items || !queues || (sort_scratch && !scratch)) return MSV_ERR_OUT_OF_MEMORY;
-    size_t off = 0;
-    for (BatchItem& it : items) {
-      if (it.kind != MSV_JOB_SORT_BY_KEY || it.n <= kBatchSortTile) continue;
-      const size_t bytes =
-          ((size_t)it.n * (dtype_size((msv_dtype)it.dtype) + it.vbytes) + 255) & ~(size_t)255;
-      it.s0 = scratch + off;
-      it.s1 = scratch + off + bytes;
-      off += 2 * bytes;
+  hdr->start[kBatchFamilies] = k;
+  uint32_t chunks = 0;
+  for (uint32_t g = 0; g < ngen; ++g) {
+    tg[g] = gens[g];
+    tg[g].chunk0 = chunks;
+    chunks += (gens[g].n + kBatchGenChunk - 1) / kBatchGenChunk;
+  }
+  hdr->ngen = ngen;
+  hdr->gen_chunks = chunks;
+  return MSV_OK;
+}
+
+// Resident blocks per CU of the batched kernels (cached per executor).
+static int family_per_cu(msv_executor* ex, int f) {
+  int& slot = ex->batch_per_cu[f];
+  if (slot == 0) {
+    int b = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (f) {
+      case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, batch_stream_kernel, kBatchBlock, 0); break;
+      case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, batch_sort_kernel<1, 1>, kBatchBlock, 0); break;
+      case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, batch_sort_kernel<1, 2>, kBatchBlock, 0); break;
+      case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, batch_sort_kernel<1, 3>, kBatchBlock, 0); break;
+      case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, batch_sort_kernel<2, 2>, kBatchBlock, 0); break;
+      case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, batch_sort_kernel<2, 3>, kBatchBlock, 0); break;
+      default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, batch_sort_kernel<2, 4>, kBatchBlock, 0); break;
     }
-    const void* src = c.staging(items.data(), sizeof(BatchItem) * items.size());
+    if (e != hipSuccess) (void)hipGetLastError();
+    slot = std::max(1, b);
+  }
+  return slot;
+}
+
+void batch_occupancy_warm(msv_executor* ex) {
+  for (int f = 0; f < kBatchFamilies; ++f) (void)family_per_cu(ex, f);
+}
+
+// The family launches of a table at `dev`; grid_of(f) gives each grid.
+template <typename GridOf>
+static msv_status launch_families(msv_executor* ex, hipStream_t stream, char* dev, uint32_t mask,
+                                  GridOf grid_of) {
+  const BatchItem* items = (const BatchItem*)(dev + sizeof(BatchHeader));
+  BatchHeader* hdr = (BatchHeader*)dev;
+  for (int f = 0; f < kBatchFamilies; ++f) {
+    if (!(mask & (1u << f))) continue;
+    const dim3 grid(grid_of(f)), block(kBatchBlock);
+    switch (f) {
+      case 0: hipLaunchKernelGGL(batch_stream_kernel, grid, block, 0, stream, items, hdr); break;
+      case 1: hipLaunchKernelGGL((batch_sort_kernel<1, 1>), grid, block, 0, stream, items, hdr); break;
+      case 2: hipLaunchKernelGGL((batch_sort_kernel<1, 2>), grid, block, 0, stream, items, hdr); break;
+      case 3: hipLaunchKernelGGL((batch_sort_kernel<1, 3>), grid, block, 0, stream, items, hdr); break;
+      case 4: hipLaunchKernelGGL((batch_sort_kernel<2, 2>), grid, block, 0, stream, items, hdr); break;
+      case 5: hipLaunchKernelGGL((batch_sort_kernel<2, 3>), grid, block, 0, stream, items, hdr); break;
+      default: hipLaunchKernelGGL((batch_sort_kernel<2, 4>), grid, block, 0, stream, items, hdr); break;
+    }
+    MSV_HIP(hipGetLastError());
+  }
+  (void)ex;
+  return MSV_OK;
+}
+
+msv_status batch_table_enqueue(msv_executor* ex, hipStream_t stream, const char* host, char* dev,
+                               uint32_t capacity, uint32_t mask, uint32_t* d_counts,
+                               uint32_t* h_counts) {
+  int per_cu[kBatchFamilies];
+  for (int f = 0; f < kBatchFamilies; ++f) per_cu[f] = family_per_cu(ex, f);
+  MSV_HIP(hipMemcpyAsync(dev, host, batch_table_bytes(capacity), hipMemcpyHostToDevice, stream));
+  const BatchHeader* hdr = (const BatchHeader*)dev;
+  const BatchGen* gens = (const BatchGen*)(dev + sizeof(BatchHeader) + (size_t)capacity * sizeof(BatchItem));
+  hipLaunchKernelGGL(batch_gen_kernel, dim3((uint32_t)ex->num_cus * 4u), dim3(256), 0, stream, gens,
+                     hdr);
+  MSV_HIP(hipGetLastError());
+  MSV_TRY(launch_families(ex, stream, dev, mask,
+                          [&](int f) { return (uint32_t)(per_cu[f] * ex->num_cus); }));
+  hipLaunchKernelGGL(batch_counts_kernel, dim3(1), dim3(256), 0, stream, gens, hdr, d_counts);
+  MSV_HIP(hipGetLastError());
+  MSV_HIP(hipMemcpyAsync(h_counts, d_counts, sizeof(uint32_t) * capacity, hipMemcpyDeviceToHost,
+                         stream));
+  return MSV_OK;
+}
+
+msv_status run_jobs(Ctx& c, const msv_job* jobs, uint32_t count) {
+  for (uint32_t i = 0; i < count; ++i) MSV_TRY(check_job(jobs[i], i));
+  std::vector<msv_job> small;
+  std::vector<uint32_t> large;
+  for (uint32_t i = 0; i < count; ++i) {
+    if (job_is_small(jobs[i])) small.push_back(jobs[i]);
+    else large.push_back(i);
+  }
+  if (!small.empty()) {
+    const uint32_t cap = (uint32_t)small.size();
+    std::vector<char> host(batch_table_bytes(cap));
+    uint32_t mask = 0;
+    MSV_TRY(batch_table_fill(c, small.data(), cap, nullptr, 0, cap, host.data(), &mask));
+    char* dev = (char*)c.scratch(host.size());
+    if (!dev) return MSV_ERR_OUT_OF_MEMORY;
+    const void* src = c.staging(host.data(), host.size());
     if (!src) return MSV_ERR_OUT_OF_MEMORY;
-    MSV_HIP(hipMemcpyAsync(d_items, src, sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice,
-                           c.stream()));
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, c.stream(), queues, 8u, 0u);
-    int* per_cu = c.ex()->batch_per_cu;
-    MSV_TRY(launch_family(c, batch_stream_kernel, d_items + start[0], start[1] - start[0],
-                          queues + 0, &per_cu[0]));
-    MSV_TRY(launch_family(c, batch_sort_kernel<1, 1>, d_items + start[1], start[2] - start[1], queues + 1, &per_cu[1]));
-    MSV_TRY(launch_family(c, batch_sort_kernel<1, 2>, d_items + start[2], start[3] - start[2], queues + 2, &per_cu[2]));
-    MSV_TRY(launch_family(c, batch_sort_kernel<1, 3>, d_items + start[3], start[4] - start[3], queues + 3, &per_cu[3]));
-    MSV_TRY(launch_family(c, batch_sort_kernel<2, 2>, d_items + start[4], start[5] - start[4], queues + 4, &per_cu[4]));
-    MSV_TRY(launch_family(c, batch_sort_kernel<2, 3>, d_items + start[5], start[6] - start[5], queues + 5, &per_cu[5]));
-    MSV_TRY(launch_family(c, batch_sort_kernel<2, 4>, d_items + start[6], start[7] - start[6], queues + 6, &per_cu[6]));
+    MSV_HIP(hipMemcpyAsync(dev, src, host.size(), hipMemcpyHostToDevice, c.stream()));
+    const BatchHeader* hh = (const BatchHeader*)host.data();
+    msv_executor* ex = c.ex();
+    MSV_TRY(launch_families(ex, c.stream(), dev, mask, [&](int f) {
+      const uint32_t n = hh->start[f + 1] - hh->start[f];
+      return std::min<uint32_t>(n, (uint32_t)(family_per_cu(ex, f) * ex->num_cus));
+    }));
   }
   for (uint32_t i : large) MSV_TRY(large_job(c, jobs[i]));
   return MSV_OK;

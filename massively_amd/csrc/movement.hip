@@ -309,15 +309,7 @@ __global__ __launch_bounds__(kMoveBlock) void bin_scatter_kernel(const uint32_t*
 }
 
 // ---- seeded streams (massively/src/api/util/random.rs:12-22,331-396) ------
-__host__ __device__ inline uint32_t pcg_hash32(uint32_t input) {
-  const uint32_t state = input * 747796405u + 2891336453u;
-  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
-  return (word >> 22u) ^ word;
-}
-
-static uint32_t seed_key(uint64_t seed) {
-  return pcg_hash32((uint32_t)seed ^ pcg_hash32((uint32_t)(seed >> 32)));
-}
+// pcg_hash32 and seed_key: msv_internal.hpp.
 
 __global__ void uniform_u32_kernel(uint32_t* __restrict__ out, uint32_t n, uint32_t start,
                                    uint32_t key, uint32_t lo, uint32_t span) {

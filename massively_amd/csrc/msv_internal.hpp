@@ -234,6 +234,18 @@ msv_status iota_run(Ctx& c, uint32_t* out, uint32_t n, uint32_t start);
 // Seeded synthetic input of the library generator: full-range bits for
 // integers, [0.5, 1) for floats.
 msv_status uniform_run(Ctx& c, msv_dtype dt, void* out, uint32_t n, uint64_t seed);
+
+// The PCG hash of the seeded streams (massively/src/api/util/random.rs:12-22)
+// and the key a 64-bit seed selects (element i of a stream is
+// pcg_hash32(i ^ key), mapped to the dtype's range).
+__host__ __device__ inline uint32_t pcg_hash32(uint32_t input) {
+  const uint32_t state = input * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+inline uint32_t seed_key(uint64_t seed) {
+  return pcg_hash32((uint32_t)seed ^ pcg_hash32((uint32_t)(seed >> 32)));
+}
 // Every job through the device-wide kernels, one after another (the
 // one-launch-per-job form the batched path is compared with).
 msv_status run_jobs_plain(Ctx& c, const msv_job* jobs, uint32_t count);
@@ -265,6 +277,42 @@ void tune_put(const msv_executor* ex, const char* family, uint32_t key, int choi
 int tune_get_current(const char* family, uint32_t key);
 // Enqueues a batch of validated-on-entry jobs (msv_run_jobs).
 msv_status run_jobs(Ctx& c, const msv_job* jobs, uint32_t count);
+
+// Batches whose launches are captured once and replayed (the serving loop).
+// A batch table holds up to `capacity` small jobs and as many generation
+// entries; its launches read everything from the table, so their grids and
+// arguments do not depend on the batch.
+//
+// A generation entry: before the jobs run, the seeded stream of `seed_key`
+// (the bits uniform_run writes) is written to `in` (unless null) and the
+// positions 0..n-1 to `iota` (unless null); after them, the word at
+// `count_src` (unless null) is copied to the batch's counts at `count_index`.
+struct BatchGen {
+  void* in;
+  uint32_t* iota;
+  const uint32_t* count_src;
+  uint32_t n, dtype, key, count_index;
+  uint32_t chunk0;  // generation chunks before this entry (filled by batch_table_fill)
+  uint32_t pad;
+};
+size_t batch_table_bytes(uint32_t capacity);
+// Whether a job runs inside the batched kernels (else device-wide).
+bool batch_job_small(const msv_job& j);
+// Validates `jobs` (all small, at most `capacity`) and writes the table for
+// them and `gens` into `host` (batch_table_bytes(capacity) bytes). Sort jobs
+// above the tile size get element scratch from `c`. *mask: the job families
+// present (bit f).
+msv_status batch_table_fill(Ctx& c, const msv_job* jobs, uint32_t count, const BatchGen* gens,
+                            uint32_t ngen, uint32_t capacity, char* host, uint32_t* mask);
+// Enqueues on `stream`: the upload of the host table to `dev`, the input
+// generation, one persistent launch per family in `mask` (resident-size
+// grids), the count gather into d_counts and its download to h_counts
+// (capacity words each). Capturable: no memset, no allocation.
+// Resolves the batched kernels' occupancy (before a capture of the enqueue).
+void batch_occupancy_warm(msv_executor* ex);
+msv_status batch_table_enqueue(msv_executor* ex, hipStream_t stream, const char* host, char* dev,
+                               uint32_t capacity, uint32_t mask, uint32_t* d_counts,
+                               uint32_t* h_counts);
 // Stable sort of every segment of offsets[0..nseg] on its own (mode 1 Less,
 // 2 Greater); vsize 0, 4 or 8.
 msv_status seg_sort_run(Ctx& c, msv_dtype kdt, int mode, const void* keys, const void* vals,

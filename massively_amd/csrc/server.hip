@@ -29,6 +29,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <memory>
 #include <thread>
 #include <unordered_map>
@@ -67,6 +68,13 @@ struct Slot {
   uint32_t* h_counts = nullptr;  // pinned, max_batch entries
   bool busy = false;
   std::vector<Request*> reqs;
+  // Batched mode: the batch table (pinned host copy and device copy), the
+  // device counts, and one captured graph of the batch launches per set of
+  // job families present.
+  char* h_table = nullptr;
+  char* d_table = nullptr;
+  uint32_t* d_counts = nullptr;
+  std::map<uint32_t, hipGraphExec_t> graphs;
 };
 
 }  // namespace msv
@@ -126,12 +134,50 @@ static void* salloc(msv_executor* ex, uint64_t bytes) {
   return ex->allocator.alloc(bytes ? bytes : 16);
 }
 
+// Enqueues a batch of small requests on slot k by replaying the slot's
+// captured graph for the families present (capturing it on first use): the
+// seeded inputs, the sort positions and the copy_where counts are produced by
+// the graph's own kernels from the batch table.
+static msv_status launch_batch_graph(msv_server* s, int k, Ctx& c, std::vector<msv_job>& jobs,
+                                     std::vector<BatchGen>& gens) {
+  Slot& sl = s->slots[k];
+  msv_executor* ex = sl.ex;
+  const uint32_t cap = s->cfg.max_batch;
+  uint32_t mask = 0;
+  MSV_TRY(batch_table_fill(c, jobs.data(), (uint32_t)jobs.size(), gens.data(),
+                           (uint32_t)gens.size(), cap, sl.h_table, &mask));
+  auto it = sl.graphs.find(mask);
+  if (it == sl.graphs.end()) {
+    batch_occupancy_warm(ex);
+    MSV_HIP(hipStreamBeginCapture(c.stream(), hipStreamCaptureModeThreadLocal));
+    const msv_status st =
+        batch_table_enqueue(ex, c.stream(), sl.h_table, sl.d_table, cap, mask, sl.d_counts,
+                            sl.h_counts);
+    hipGraph_t graph = nullptr;
+    const hipError_t ee = hipStreamEndCapture(c.stream(), &graph);
+    MSV_TRY(st);
+    MSV_HIP(ee);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MSV_HIP(ie);
+    it = sl.graphs.emplace(mask, exec).first;
+  }
+  MSV_HIP(hipGraphLaunch(it->second, c.stream()));
+  return MSV_OK;
+}
+
 // Enqueues one batch on slot k (scheduler thread, lock not held).
 static msv_status launch_batch(msv_server* s, int k) {
   Slot& sl = s->slots[k];
   msv_executor* ex = sl.ex;
   MSV_HIP(hipSetDevice(ex->device));
   Ctx c(ex);
+  // Batched mode replays a graph for batches of small requests (a large
+  // request always has a batch of its own).
+  bool graph = s->cfg.mode == MSV_SERVE_BATCHED && sl.h_table != nullptr;
+  for (Request* q : sl.reqs) graph = graph && is_small(q->r);
+  std::vector<BatchGen> gens;
   std::vector<msv_job> jobs;
   jobs.reserve(sl.reqs.size());
   uint32_t ncounts = 0;
@@ -142,9 +188,15 @@ static msv_status launch_batch(msv_server* s, int k) {
     q->slot = k;
     q->d_in = salloc(ex, (uint64_t)r.n * es);
     if (!q->d_in) return MSV_ERR_OUT_OF_MEMORY;
+    BatchGen g{};
+    g.n = r.n;
+    g.dtype = r.dtype;
     if (!q->host.empty()) {
       MSV_HIP(hipMemcpyAsync(q->d_in, q->host.data(), (size_t)r.n * es, hipMemcpyHostToDevice,
                              c.stream()));
+    } else if (graph) {
+      g.in = q->d_in;
+      g.key = seed_key(r.seed);
     } else {
       MSV_TRY(uniform_run(c, dt, q->d_in, r.n, r.seed));
     }
@@ -167,6 +219,8 @@ static msv_status launch_batch(msv_server* s, int k) {
         q->d_out2 = salloc(ex, 256);
         if (!q->d_out || !q->d_out2) return MSV_ERR_OUT_OF_MEMORY;
         q->count_index = ncounts++;
+        g.count_src = (const uint32_t*)q->d_out2;
+        g.count_index = q->count_index;
         break;
       case MSV_JOB_SORT_BY_KEY:
         q->d_out2 = salloc(ex, (uint64_t)r.n * es);  // sorted keys
@@ -175,7 +229,8 @@ static msv_status launch_batch(msv_server* s, int k) {
           q->d_in2 = salloc(ex, (uint64_t)r.n * 4);
           q->d_out = salloc(ex, (uint64_t)r.n * 4);
           if (!q->d_in2 || !q->d_out) return MSV_ERR_OUT_OF_MEMORY;
-          MSV_TRY(iota_run(c, (uint32_t*)q->d_in2, r.n, 0));
+          if (graph) g.iota = (uint32_t*)q->d_in2;
+          else MSV_TRY(iota_run(c, (uint32_t*)q->d_in2, r.n, 0));
           j.value_bytes = 4;
         }
         break;
@@ -188,6 +243,12 @@ static msv_status launch_batch(msv_server* s, int k) {
     j.out = q->d_out;
     j.out2 = q->d_out2;
     jobs.push_back(j);
+    if (g.in || g.iota || g.count_src) gens.push_back(g);
+  }
+  if (graph) {
+    MSV_TRY(launch_batch_graph(s, k, c, jobs, gens));
+    MSV_HIP(hipEventRecord(sl.ev, c.stream()));
+    return MSV_OK;
   }
   if (s->cfg.mode != MSV_SERVE_BATCHED) {
     MSV_TRY(run_jobs_plain(c, jobs.data(), (uint32_t)jobs.size()));
@@ -384,8 +445,20 @@ msv_status msv_server_create(const msv_server_config* cfg, msv_server** out) {
     if (st == MSV_OK && hipHostMalloc((void**)&sl.h_counts, sizeof(uint32_t) * s->cfg.max_batch) !=
                             hipSuccess)
       st = MSV_ERR_OUT_OF_MEMORY;
+    if (st == MSV_OK && s->cfg.mode == MSV_SERVE_BATCHED) {
+      const size_t tb = batch_table_bytes(s->cfg.max_batch);
+      if (hipHostMalloc((void**)&sl.h_table, tb) != hipSuccess ||
+          hipMalloc((void**)&sl.d_table, tb) != hipSuccess ||
+          hipMalloc((void**)&sl.d_counts, sizeof(uint32_t) * s->cfg.max_batch) != hipSuccess) {
+        (void)hipGetLastError();
+        st = MSV_ERR_OUT_OF_MEMORY;
+      }
+    }
     if (st != MSV_OK) {
       for (Slot& t : s->slots) {
+        if (t.h_table) hipHostFree(t.h_table);
+        if (t.d_table) hipFree(t.d_table);
+        if (t.d_counts) hipFree(t.d_counts);
         if (t.h_counts) hipHostFree(t.h_counts);
         if (t.ev) hipEventDestroy(t.ev);
         if (t.ex) msv_executor_destroy(t.ex);
@@ -414,6 +487,10 @@ msv_status msv_server_destroy(msv_server* s) {
   s->worker.join();
   for (auto& kv : s->all) drop_outputs(s, kv.second.get());
   for (Slot& sl : s->slots) {
+    for (auto& kv : sl.graphs) (void)hipGraphExecDestroy(kv.second);
+    if (sl.h_table) hipHostFree(sl.h_table);
+    if (sl.d_table) hipFree(sl.d_table);
+    if (sl.d_counts) hipFree(sl.d_counts);
     hipHostFree(sl.h_counts);
     hipEventDestroy(sl.ev);
     msv_executor_destroy(sl.ex);
