@@ -175,11 +175,11 @@ def w_sort(c):
     vals = c.ex.alloc(np.uint32, n)
     c.mr.iota_u32(c.ex, vals)
     out["u64_uniform_2^27"] = _sort_case(c, "sort_by_key_u64_uniform_2^27", keys, vals, n, 8, 4, 8)
-    # The onesweep schedule on the same input (the default for 8-byte keys
-    # is reduce-then-scan), for comparison.
-    c.ex.set_option(m.Executor.OPT_RADIX_ONESWEEP, 1)
-    out["u64_uniform_2^27_onesweep"] = _sort_case(c, "sort_by_key_u64_uniform_2^27_onesweep", keys, vals,
-                                                  n, 8, 4, 8)
+    # The reduce-then-scan schedule on the same input (the default is
+    # onesweep), for comparison.
+    c.ex.set_option(m.Executor.OPT_RADIX_ONESWEEP, 0)
+    out["u64_uniform_2^27_reduce_then_scan"] = _sort_case(
+        c, "sort_by_key_u64_uniform_2^27_reduce_then_scan", keys, vals, n, 8, 4, 8)
     c.ex.set_option(m.Executor.OPT_RADIX_ONESWEEP, -1)
     ref = c.ex.to_host(m.vector.radix_sort_by_key(c.ex, keys, vals))
     out["rocprim_u64_uniform_2^27"] = rocprim_sort_case(c, "sort_pairs_u64_u32_2^27", keys, vals, n, 8,
@@ -195,6 +195,10 @@ def w_sort(c):
     zipf = (r * np.uint64(0x9E3779B97F4A7C15)) ^ np.uint64(0xD1B54A32D192ED03)
     zk = c.ex.to_device(zipf)
     out["u64_zipf_2^27"] = _sort_case(c, "sort_by_key_u64_zipf1.1_2^27", zk, vals, n, 8, 4, 8)
+    c.ex.set_option(m.Executor.OPT_RADIX_ONESWEEP, 0)
+    out["u64_zipf_2^27_reduce_then_scan"] = _sort_case(
+        c, "sort_by_key_u64_zipf1.1_2^27_reduce_then_scan", zk, vals, n, 8, 4, 8)
+    c.ex.set_option(m.Executor.OPT_RADIX_ONESWEEP, -1)
     del zk, zipf, r
     top = (np.uint64(0xA5) << np.uint64(56)) | (hk & np.uint64((1 << 56) - 1))
     tk = c.ex.to_device(top)

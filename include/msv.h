@@ -141,7 +141,7 @@ msv_status msv_executor_trim(msv_executor* ex);
  *  MSV_OPT_RADIX_ONESWEEP: the radix sorts' schedule: 1 = one histogram pass
  *    and one look-back pass per digit (onesweep); 0 = a counting upsweep, a
  *    scan of the counts and a downsweep per digit (reduce-then-scan); -1
- *    (default) = onesweep for 4-byte keys, reduce-then-scan for 8-byte keys.
+ *    (default) = onesweep.
  *  MSV_OPT_SEGSORT_WAVE: 1 (default) seg::Sort sorts segments of up to 1024
  *    elements inside one wave each (bitonic network in registers), longer
  *    ones up to the tile size one block each with the tile shapes above;

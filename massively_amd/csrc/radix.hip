@@ -16,14 +16,17 @@
 // tile's keys stably inside each wave by matching 8-bit digits with ballots
 // (wave-level multi-split) and reorders the tile through LDS into digit runs.
 // Two schedules:
-//  * onesweep (default for 4-byte keys): one read of the keys builds the
+//  * onesweep (the default): one read of the keys builds the
 //    histograms of every digit position (per-wave LDS histograms), a one-block
 //    plan scans them into digit starts and the pass table, then each pass
 //    claims tiles in order, publishes a tile's digit counts before ranking it,
 //    and finds each digit's global position by a decoupled look-back over the
-//    earlier tiles (4 status words per step);
-//  * reduce-then-scan (default for 8-byte keys, whose Zipf-distributed keys
-//    sort 13% slower under onesweep here): pass 0's upsweep also records per
+//    earlier tiles (4 status words per step); elements of 12 or 16 bytes go
+//    through LDS as keys and then values, which leaves room for 6 waves per
+//    SIMD (2^27 u64 keys + u32 values: uniform 10.50 ms, Zipf 1.1 10.70 ms,
+//    constant top byte 9.22 ms);
+//  * reduce-then-scan (11.86 / 10.84 / 10.43 ms there, 8.22 ms against 7.80
+//    for 2^28 u32 + u32; kept behind the option): pass 0's upsweep also records per
 //    tile the OR and AND of the order words (the skip test), and per digit an
 //    upsweep counts the digits of every tile into a digit-major matrix, the
 //    library's scan turns it into run starts, and a downsweep ranks and writes;
@@ -59,11 +62,9 @@ static msv_status radix_sort_impl(Ctx& c, const void* keys, const void* vals, ui
   void* aos1 = c.scratch(elem_bytes);
   RadixPass* table = (RadixPass*)c.scratch(sizeof(RadixPass) * P);
   if (!aos0 || !aos1 || !table) return MSV_ERR_OUT_OF_MEMORY;
-  // Schedule: onesweep for 4-byte keys, reduce-then-scan for 8-byte keys
-  // (faster for each on MI355X at 2^28 u32 / 2^27 u64 keys), unless the
-  // executor option forces one.
+  // Schedule: onesweep unless the executor option forces reduce-then-scan.
   const int64_t sched = c.ex()->opt_radix_onesweep;
-  if (sched == 1 || (sched < 0 && sizeof(K) == 4)) {
+  if (sched != 0) {
     // Onesweep: histograms of every digit position in one read of the keys,
     // then one look-back pass per digit position that needs one.
     uint32_t* hist = (uint32_t*)c.scratch(sizeof(uint32_t) * (kRadixBins * P + P));

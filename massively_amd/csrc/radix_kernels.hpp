@@ -233,6 +233,38 @@ __device__ inline void store_elem(const RadixPass& rp, size_t i, const Elem<EW>&
   for (int v = 0; v < EW - KW; ++v) ((uint32_t*)rp.dst_v)[i * (EW - KW) + v] = e.w[KW + v];
 }
 
+// The key or the value part of element i of the pass's destination.
+template <typename K, int EW>
+__device__ inline void store_key_part(const RadixPass& rp, size_t i, K k) {
+  if (rp.out_aos) {
+    uint32_t* w = ((Elem<EW>*)rp.dst_k)[i].w;
+    if constexpr (sizeof(K) == 4) {
+      w[0] = bit_cast<uint32_t>(k);
+    } else {
+      const uint64_t u = bit_cast<uint64_t>(k);
+      w[0] = (uint32_t)u;
+      w[1] = (uint32_t)(u >> 32);
+    }
+    return;
+  }
+  ((K*)rp.dst_k)[i] = k;
+}
+template <typename K, int EW, typename V>
+__device__ inline void store_val_part(const RadixPass& rp, size_t i, V v) {
+  constexpr int KW = sizeof(K) / 4;
+  if (rp.out_aos) {
+    uint32_t* w = ((Elem<EW>*)rp.dst_k)[i].w;
+    if constexpr (sizeof(V) == 4) {
+      w[KW] = v;
+    } else {
+      w[KW] = (uint32_t)v;
+      w[KW + 1] = (uint32_t)(v >> 32);
+    }
+    return;
+  }
+  ((V*)rp.dst_v)[i] = v;
+}
+
 // 3. Per-pass reduce-then-scan over tiles of BLOCK * IPT keys.
 //  3a. upsweep: digit counts of every tile, digit-major
 //      (counts[d * ntiles + tile]), so that one exclusive scan of `counts`
@@ -414,9 +446,12 @@ __global__ __launch_bounds__(BLOCK) void radix_downsweep_kernel(
 //    hist[p * 256 + d] = number of keys whose digit p is d. Each thread reads
 //    16 keys per step with 16-byte loads (all in flight before the first is
 //    counted), each wave counts into its own LDS histogram, then the block adds
-//    its counts to the global histogram. A wave whose lanes all hold the same digit (constant bytes are
-//    common: small keys, a constant top byte) adds its count once instead of
-//    64 times to one bank.
+//    its counts to the global histogram. On a full step the wave ORs every
+//    key's difference from its first key once (a wave reduction), and a digit
+//    position that is constant over the step's 1024 keys (constant bytes are
+//    common: small keys, a constant top byte) is counted by one add; the other
+//    positions take one LDS add per key and no per-key test (per key and
+//    position, a uniformity test had cost more instructions than the add).
 template <typename K, int MODE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void radix_hist_kernel(const K* __restrict__ keys, uint32_t n,
                                                           uint32_t* __restrict__ hist) {
@@ -447,9 +482,39 @@ __global__ __launch_bounds__(BLOCK) void radix_hist_kernel(const K* __restrict__
           if constexpr (sizeof(K) == 4) key = bit_cast<K>(qq[k]);
           else key = bit_cast<K>(((uint64_t)qq[2 * k + 1] << 32) | qq[2 * k]);
           w[r * V + k] = order_bits<K, MODE>(key);
-          ok[r * V + k] = true;
         }
       }
+      // Bytes in which some key of the wave's step differs from its first key.
+      uint32_t lo0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)w[0]);
+      uint32_t hi0 = 0, xlo = 0, xhi = 0;
+      if constexpr (sizeof(U) == 8)
+        hi0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)w[0] >> 32));
+#pragma unroll
+      for (int j = 0; j < KPT; ++j) {
+        xlo |= (uint32_t)w[j] ^ lo0;
+        if constexpr (sizeof(U) == 8) xhi |= (uint32_t)((uint64_t)w[j] >> 32) ^ hi0;
+      }
+#pragma unroll
+      for (int k = 1; k < kWave; k <<= 1) {
+        xlo |= (uint32_t)__shfl_xor((int)xlo, k);
+        if constexpr (sizeof(U) == 8) xhi |= (uint32_t)__shfl_xor((int)xhi, k);
+      }
+      xlo = (uint32_t)__builtin_amdgcn_readfirstlane((int)xlo);
+      xhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)xhi);
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const uint32_t sh = 8u * (uint32_t)(p & 3);
+        const uint32_t diff = ((p < 4 ? xlo : xhi) >> sh) & 0xffu;
+        if (diff == 0) {
+          const uint32_t d0 = ((p < 4 ? lo0 : hi0) >> sh) & 0xffu;
+          if (__lane_id() == 0) atomicAdd(&s_h[p * kRadixBins + d0], (uint32_t)(kWave * KPT));
+        } else {
+#pragma unroll
+          for (int j = 0; j < KPT; ++j)
+            atomicAdd(&s_h[p * kRadixBins + ((uint32_t)(w[j] >> (8 * p)) & 0xffu)], 1u);
+        }
+      }
+      continue;
     } else {
 #pragma unroll
       for (int r = 0; r < KPT / V; ++r)
@@ -549,6 +614,11 @@ __global__ __launch_bounds__(kRadixBins) void radix_onesweep_plan_kernel(
 //    inclusive prefix and places the digit's run at
 //    global start + prefix. A tile publishes its aggregate before it waits on
 //    anything, so every walk ends.
+//    Elements of 12 or 16 bytes are reordered through LDS in two phases, the
+//    keys and then the values (the same LDS), so LDS holds one of them per
+//    tile instead of the whole element: 6 waves per SIMD instead of 4 for
+//    u64 keys with u32 values (11.33 -> 10.52 ms at 2^27). Narrower elements
+//    gain no waves that way (registers bound them) and keep one phase.
 template <typename K, int MODE, int EW, int BLOCK, int IPT = kRadixIpt>
 __global__ __launch_bounds__(BLOCK) void radix_onesweep_kernel(
     const RadixPass* __restrict__ table, int pass, const uint32_t* __restrict__ gbase,
@@ -556,7 +626,11 @@ __global__ __launch_bounds__(BLOCK) void radix_onesweep_kernel(
     uint32_t n) {
   constexpr int NW = BLOCK / kWave;
   constexpr uint32_t TILE = (uint32_t)BLOCK * IPT;
-  __shared__ Elem<EW> s_el[TILE];
+  constexpr int KW = (int)sizeof(K) / 4, VW = EW - KW;
+  using V = typename std::conditional<VW == 2, uint64_t, uint32_t>::type;
+  constexpr bool kSplit = EW >= 3;
+  constexpr int MW = kSplit ? (KW > VW ? KW : VW) : EW;  // LDS words per element
+  __shared__ __align__(16) uint32_t s_part[TILE * MW];
   __shared__ uint32_t s_hist[NW][kRadixBins];     // per-wave counts, then wave offsets
   __shared__ uint32_t s_tile_start[kRadixBins];   // in-tile start of each digit
   __shared__ uint32_t s_goff[kRadixBins];         // global position - in-tile position
@@ -676,19 +750,63 @@ __global__ __launch_bounds__(BLOCK) void radix_onesweep_kernel(
   }
   __syncthreads();
 
-  // Reorder elements into digit-sorted runs in LDS, then write the runs out.
+  // Reorder into digit-sorted runs in LDS, then write the runs out (split:
+  // the keys first, each output position's digit kept for the values, then
+  // the values).
 #pragma unroll
   for (int j = 0; j < IPT; ++j)
-    if (dig[j] < 0x100u) s_el[s_tile_start[dig[j]] + s_hist[wid][dig[j]] + rank[j]] = el[j];
-  __syncthreads();
+    if (dig[j] < 0x100u) rank[j] += s_tile_start[dig[j]] + s_hist[wid][dig[j]];
   const uint32_t tile_n = (uint32_t)min((size_t)TILE, (size_t)n - (size_t)tile * TILE);
+  if constexpr (!kSplit) {
+    Elem<EW>* s_el = reinterpret_cast<Elem<EW>*>(s_part);
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+      if (dig[j] < 0x100u) s_el[rank[j]] = el[j];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      const uint32_t i = tid + (uint32_t)q * BLOCK;
+      if (i < tile_n) {
+        const Elem<EW> e = s_el[i];
+        const uint32_t d = (uint32_t)(order_bits<K, MODE>(elem_key<K, EW>(e)) >> shift) & 0xffu;
+        store_elem<K, EW>(rp, (size_t)s_goff[d] + i, e);
+      }
+    }
+    return;
+  }
+  K* s_key = reinterpret_cast<K*>(s_part);
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (dig[j] < 0x100u) s_key[rank[j]] = elem_key<K, EW>(el[j]);
+  __syncthreads();
+  uint32_t dq[IPT];
 #pragma unroll
   for (int q = 0; q < IPT; ++q) {
     const uint32_t i = tid + (uint32_t)q * BLOCK;
+    dq[q] = 0;
     if (i < tile_n) {
-      const Elem<EW> e = s_el[i];
-      const uint32_t d = (uint32_t)(order_bits<K, MODE>(elem_key<K, EW>(e)) >> shift) & 0xffu;
-      store_elem<K, EW>(rp, (size_t)s_goff[d] + i, e);
+      const K k = s_key[i];
+      dq[q] = (uint32_t)(order_bits<K, MODE>(k) >> shift) & 0xffu;
+      store_key_part<K, EW>(rp, (size_t)s_goff[dq[q]] + i, k);
+    }
+  }
+  if constexpr (VW > 0) {
+    __syncthreads();  // every key read out before the values overwrite them
+    V* s_val = reinterpret_cast<V*>(s_part);
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if (dig[j] < 0x100u) {
+        V v;
+        if constexpr (VW == 1) v = el[j].w[KW];
+        else v = ((uint64_t)el[j].w[KW + 1] << 32) | el[j].w[KW];
+        s_val[rank[j]] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      const uint32_t i = tid + (uint32_t)q * BLOCK;
+      if (i < tile_n) store_val_part<K, EW, V>(rp, (size_t)s_goff[dq[q]] + i, s_val[i]);
     }
   }
 }
