@@ -23,8 +23,9 @@
 //    and finds each digit's global position by a decoupled look-back over the
 //    earlier tiles (4 status words per step); elements of 12 or 16 bytes go
 //    through LDS as keys and then values, which leaves room for 6 waves per
-//    SIMD (2^27 u64 keys + u32 values: uniform 10.50 ms, Zipf 1.1 10.70 ms,
-//    constant top byte 9.22 ms);
+//    SIMD, and stay in separate key and value arrays between passes (2^27 u64
+//    keys + u32 values: uniform 10.39 ms, Zipf 1.1 10.58 ms, constant top
+//    byte 9.13 ms);
 //  * reduce-then-scan (11.86 / 10.84 / 10.43 ms there, 8.22 ms against 7.80
 //    for 2^28 u32 + u32; kept behind the option): pass 0's upsweep also records per
 //    tile the OR and AND of the order words (the skip test), and per digit an
@@ -58,8 +59,11 @@ static msv_status radix_sort_impl(Ctx& c, const void* keys, const void* vals, ui
   if (n == 0) return MSV_OK;
   const uint32_t ntiles = div_up(n, TILE);
   const size_t elem_bytes = (size_t)n * EW * 4;
-  void* aos0 = c.scratch(elem_bytes);
-  void* aos1 = c.scratch(elem_bytes);
+  // Two-phase onesweep passes keep keys and values apart between passes: the
+  // values start at a 256-byte boundary after the keys.
+  const size_t vofs = radix_split(EW) ? (((size_t)n * sizeof(K) + 255) & ~(size_t)255) : 0;
+  void* aos0 = c.scratch(elem_bytes + 256);
+  void* aos1 = c.scratch(elem_bytes + 256);
   RadixPass* table = (RadixPass*)c.scratch(sizeof(RadixPass) * P);
   if (!aos0 || !aos1 || !table) return MSV_ERR_OUT_OF_MEMORY;
   // Schedule: onesweep unless the executor option forces reduce-then-scan.
@@ -77,7 +81,8 @@ static msv_status radix_sort_impl(Ctx& c, const void* keys, const void* vals, ui
     hipLaunchKernelGGL((radix_hist_kernel<K, MODE, 256>), dim3(hist_blocks), dim3(256), 0,
                        c.stream(), (const K*)keys, n, hist);
     hipLaunchKernelGGL((radix_onesweep_plan_kernel<P>), dim3(1), dim3(kRadixBins), 0, c.stream(),
-                       hist, n, table, gbase, tile_counters, keys, vals, out_k, out_v, aos0, aos1);
+                       hist, n, table, gbase, tile_counters, keys, vals, out_k, out_v, aos0, aos1,
+                       vofs);
     MSV_HIP(hipGetLastError());
     for (int p = 0; p < P; ++p) {
       uint64_t *counter, *status;

@@ -554,11 +554,13 @@ __global__ __launch_bounds__(BLOCK) void radix_hist_kernel(const K* __restrict__
 //    pass), which positions need a pass (a position whose histogram puts all
 //    n keys in one bin is skipped), the pass table (as the reduce-then-scan
 //    plan) and zeroed tile counters.
+//    vofs = 0: the intermediate buffers hold interleaved elements; else each
+//    holds the keys and, vofs bytes in, the values (the two-phase passes).
 template <int P>
 __global__ __launch_bounds__(kRadixBins) void radix_onesweep_plan_kernel(
     const uint32_t* __restrict__ hist, uint32_t n, RadixPass* __restrict__ table,
     uint32_t* __restrict__ gbase, uint32_t* __restrict__ tile_counters, const void* in_k,
-    const void* in_v, void* out_k, void* out_v, void* aos0, void* aos1) {
+    const void* in_v, void* out_k, void* out_v, void* aos0, void* aos1, size_t vofs) {
   __shared__ uint32_t s_trivial[P];
   __shared__ uint32_t s_wsum[kRadixBins / kWave];
   const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
@@ -593,18 +595,24 @@ __global__ __launch_bounds__(kRadixBins) void radix_onesweep_plan_kernel(
       rp.src_k = rp.src_v = nullptr;
       rp.dst_k = rp.dst_v = nullptr;
       if (rp.active) {
-        rp.in_aos = j > 0;
-        rp.out_aos = j < active - 1;
-        rp.src_k = j == 0 ? in_k : (((j - 1) & 1) ? aos1 : aos0);
-        rp.src_v = j == 0 ? in_v : nullptr;
-        rp.dst_k = j == active - 1 ? out_k : ((j & 1) ? aos1 : aos0);
-        rp.dst_v = j == active - 1 ? out_v : nullptr;
+        char* src = (char*)(((j - 1) & 1) ? aos1 : aos0);
+        char* dst = (char*)((j & 1) ? aos1 : aos0);
+        rp.in_aos = j > 0 && vofs == 0;
+        rp.out_aos = j < active - 1 && vofs == 0;
+        rp.src_k = j == 0 ? in_k : src;
+        rp.src_v = j == 0 ? in_v : (vofs ? src + vofs : nullptr);
+        rp.dst_k = j == active - 1 ? out_k : dst;
+        rp.dst_v = j == active - 1 ? out_v : (vofs ? dst + vofs : nullptr);
         ++j;
       }
       table[p] = rp;
     }
   }
 }
+
+// Whether a onesweep pass over EW-word elements reorders keys and values in
+// two phases (and keeps them in separate arrays between passes).
+constexpr bool radix_split(int EW) { return EW >= 3; }
 
 // 7. Onesweep pass: a block claims the next tile id from the pass's counter
 //    (so every predecessor of a tile is already running), ranks its keys as
@@ -628,7 +636,7 @@ __global__ __launch_bounds__(BLOCK) void radix_onesweep_kernel(
   constexpr uint32_t TILE = (uint32_t)BLOCK * IPT;
   constexpr int KW = (int)sizeof(K) / 4, VW = EW - KW;
   using V = typename std::conditional<VW == 2, uint64_t, uint32_t>::type;
-  constexpr bool kSplit = EW >= 3;
+  constexpr bool kSplit = radix_split(EW);
   constexpr int MW = kSplit ? (KW > VW ? KW : VW) : EW;  // LDS words per element
   __shared__ __align__(16) uint32_t s_part[TILE * MW];
   __shared__ uint32_t s_hist[NW][kRadixBins];     // per-wave counts, then wave offsets
