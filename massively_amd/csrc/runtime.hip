@@ -40,12 +40,16 @@ size_t Allocator::round(size_t bytes) {
 void* Allocator::alloc(size_t bytes) {
   const size_t r = round(bytes);
   {
+    // Best fit among the cached blocks of at most twice the rounded size: with
+    // requests of varied sizes an exact match is rare, and every miss would
+    // leave one more block in the cache (until hipMalloc fails and the whole
+    // cache is freed block by block, which stalls for seconds).
     std::lock_guard<std::mutex> lk(mu_);
-    auto it = free_.find(r);
-    if (it != free_.end()) {
+    auto it = free_.lower_bound(r);
+    if (it != free_.end() && it->first <= 2 * r) {
       void* p = it->second;
+      cached_ -= it->first;
       free_.erase(it);
-      cached_ -= r;
       return p;
     }
   }
