@@ -154,6 +154,34 @@ __device__ inline bool equivalent(K a, K b) {
   return !Cmp::less(a, b) && !Cmp::less(b, a);
 }
 
+// The same bounds found by galloping outward from a position known to hold v
+// (the run of keys equivalent to v around a split): O(log run length) steps
+// instead of O(log n), so a split inside a short run costs a few loads.
+// First index in [0, i) from which every element up to i is not before v.
+template <typename K, typename Cmp, typename AA>
+__device__ inline uint32_t gallop_lower(const AA& A, uint32_t i, K v) {
+  uint32_t lo = i, step = 1;  // A[lo, i) are not before v
+  while (lo > 0) {
+    const uint32_t probe = lo > step ? lo - step : 0u;
+    if (Cmp::less(A[probe], v)) return lower_bound_c<K, Cmp>(A, probe + 1, lo, v);
+    lo = probe;
+    step *= 2;
+  }
+  return 0;
+}
+// First index in [i, n) whose element is after v.
+template <typename K, typename Cmp, typename AA>
+__device__ inline uint32_t gallop_upper(const AA& A, uint32_t i, uint32_t n, K v) {
+  uint32_t hi = i, step = 1;  // A[i, hi) are not after v
+  while (hi < n) {
+    const uint32_t probe = n - hi > step ? hi + step : n;  // exclusive end of the next window
+    if (Cmp::less(v, A[probe - 1])) return upper_bound_c<K, Cmp>(A, hi, probe - 1, v);
+    hi = probe;
+    step *= 2;
+  }
+  return n;
+}
+
 // Merge-path split at diagonal d, moved to the nearest pair-aligned point of
 // the oracle's set walk when it falls inside a run of equal keys.
 template <typename K, typename Cmp, typename AA, typename BA>
@@ -180,10 +208,10 @@ __device__ inline void set_split(const AA& A, uint32_t na, const BA& B, uint32_t
     }
     if (equivalent<K, Cmp>(prev, next)) {
       const K v = next;
-      const uint32_t lba = lower_bound_c<K, Cmp>(A, 0, i, v);
-      const uint32_t lbb = lower_bound_c<K, Cmp>(B, 0, j, v);
-      const uint32_t uba = upper_bound_c<K, Cmp>(A, i, na, v);
-      const uint32_t ubb = upper_bound_c<K, Cmp>(B, j, nb, v);
+      const uint32_t lba = gallop_lower<K, Cmp>(A, i, v);
+      const uint32_t lbb = gallop_lower<K, Cmp>(B, j, v);
+      const uint32_t uba = gallop_upper<K, Cmp>(A, i, na, v);
+      const uint32_t ubb = gallop_upper<K, Cmp>(B, j, nb, v);
       const uint32_t ma = uba - lba, mb = ubb - lbb;
       const uint32_t mp = ma < mb ? ma : mb;
       const uint32_t x = (i - lba) + (j - lbb);
