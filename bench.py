@@ -602,8 +602,9 @@ def run_served(np, mode, reqs, arrivals_us=None):
 
     srv = serve.Server(streams=4, max_batch=256, mode=mode, hbm_budget=64 << 30)
     try:
-        # Warm-up: one pass over a few requests (allocator and kernel caches).
-        for t in [srv.submit(r) for r in reqs[:16]]:
+        # Warm-up: one pass over the requests, all at once (allocator and
+        # kernel caches; the batched grids' autotuning of the batch shapes).
+        for t in [srv.submit(r) for r in reqs]:
             srv.wait(t)
             srv.release(t)
         t0 = srv.now_us()

@@ -133,8 +133,10 @@ msv_status msv_executor_trim(msv_executor* ex);
  *    launch over a work queue (MSV_SEG_LAUNCH_PERSISTENT, default) or one
  *    launch per size class (MSV_SEG_LAUNCH_PER_CLASS).
  *  MSV_OPT_AUTOTUNE: 1 (default) times the tile-shape candidates on the first
- *    seg::Sort of each (key width, element width, mean-segment-length class)
- *    and keeps the fastest for the process; 0 uses the built-in default.
+ *    seg::Sort of each (key width, element width, mean-segment-length class),
+ *    and the grid candidates of each batched job-kernel family on the first
+ *    batch of each (family, job-count class, mean-length class), and keeps
+ *    the fastest in the tuning table; 0 uses the built-in defaults.
  *  MSV_OPT_SEGSORT_CONFIG: -1 (default) = as above, else force candidate k
  *    (0 <= k < msv_segsort_config_count()); MSV_ERR_UNSUPPORTED if candidate
  *    k does not fit the element width.
@@ -187,6 +189,14 @@ int32_t msv_segsort_tuned_config(uint32_t key_bytes, uint32_t value_bytes, int64
                                  uint32_t n, uint32_t nseg);
 int32_t msv_segsort_tuned_config_windows(uint32_t key_bytes, uint32_t value_bytes, int64_t launch,
                                          uint32_t n, uint32_t nseg);
+/* The grid candidate the autotuner chose on the current device for a batched
+ * job-kernel family (0 = scan / reduce / copy_where, 1..6 = sort_by_key by
+ * (key, value) words: (1,0) (1,1) (1,2) (2,0) (2,1) (2,2)) in a batch of
+ * `jobs` jobs of mean length `mean_len`, or -1 if it has not run for that
+ * class. Candidate c launches the family's resident blocks per CU divided by
+ * 2^c (at least one per CU). msv_run_jobs and the batched server tune a class
+ * on its first batch of at least 16 jobs of that family. */
+int32_t msv_batch_tuned_grid(uint32_t family, uint32_t jobs, uint32_t mean_len);
 /* The autotuner's table (no counterpart in the reference, which specialises
  * kernels in its JIT compiler): one entry per (device signature "<gfx arch
  * name>/<compute units>", family, key), saved to and merged from a text file

@@ -17,8 +17,8 @@ from .error import (
     UnboundColumn,
     Unsupported,
 )
-from .runtime import (DeviceSlice, DeviceVec, Executor, segsort_configs, segsort_tuned_config,
-                      tuning_clear, tuning_count, tuning_load, tuning_save)
+from .runtime import (DeviceSlice, DeviceVec, Executor, batch_tuned_grid, segsort_configs,
+                      segsort_tuned_config, tuning_clear, tuning_count, tuning_load, tuning_save)
 
 __all__ = [
     "Executor",
@@ -32,6 +32,7 @@ __all__ = [
     "UnboundColumn",
     "ForeignExecutor",
     "Unsupported",
+    "batch_tuned_grid",
     "segsort_configs",
     "segsort_tuned_config",
     "tuning_clear",

@@ -171,6 +171,7 @@ PROTOTYPES = {
     "msv_server_now_us": (C.c_double, [_P]),
     "msv_segsort_config_count": (_U32, []),
     "msv_segsort_tuned_config": (_I, [_U32, _U32, C.c_int64, _U32, _U32]),
+    "msv_batch_tuned_grid": (_I, [_U32, _U32, _U32]),
     "msv_segsort_tuned_config_windows": (_I, [_U32, _U32, C.c_int64, _U32, _U32]),
     "msv_tuning_save": (_I, [C.c_char_p]),
     "msv_tuning_load": (_I, [C.c_char_p]),

@@ -17,6 +17,7 @@
 // fixed block order above (deterministic; a different association than the
 // device-wide kernels).
 #include <algorithm>
+#include <vector>
 
 #include "msv_internal.hpp"
 #include "predicate.hpp"
@@ -480,8 +481,9 @@ msv_status batch_table_fill(Ctx& c, const msv_job* jobs, uint32_t count, const B
 
 // Resident blocks per CU of the batched kernels (cached per executor).
 static int family_per_cu(msv_executor* ex, int f) {
-  int& slot = ex->batch_per_cu[f];
-  if (slot == 0) {
+  std::atomic<int>& slot = ex->batch_per_cu[f];
+  int pc = slot.load(std::memory_order_relaxed);
+  if (pc == 0) {
     int b = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (f) {
@@ -494,52 +496,220 @@ static int family_per_cu(msv_executor* ex, int f) {
       default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, batch_sort_kernel<2, 4>, kBatchBlock, 0); break;
     }
     if (e != hipSuccess) (void)hipGetLastError();
-    slot = std::max(1, b);
+    pc = std::max(1, b);
+    slot.store(pc, std::memory_order_relaxed);
   }
-  return slot;
+  return pc;
 }
 
 void batch_occupancy_warm(msv_executor* ex) {
   for (int f = 0; f < kBatchFamilies; ++f) (void)family_per_cu(ex, f);
 }
 
-// The family launches of a table at `dev`; grid_of(f) gives each grid.
-template <typename GridOf>
-static msv_status launch_families(msv_executor* ex, hipStream_t stream, char* dev, uint32_t mask,
-                                  GridOf grid_of) {
+// ---- autotuned grids -------------------------------------------------------
+// A family's grid is its resident blocks per CU shifted right by the chosen
+// candidate c (0 = every block the CUs hold at once, 1 = half, 2 = a quarter,
+// 3 = an eighth; at least one per CU), times the CUs. Fewer blocks mean fewer
+// idle blocks to launch and retire and less contention on the job queue and
+// the caches; more blocks hide the latency of short jobs. The choice is made
+// per (family, job-count class, mean-length class) of a batch, by timing, and
+// kept in the tuning table under "batch".
+constexpr int kBatchGridCands = 4;
+
+static uint32_t log2_floor(uint32_t x) { return x ? 31u - (uint32_t)__builtin_clz(x) : 0u; }
+static uint32_t log2_ceil(uint32_t x) { return x <= 1 ? 0u : log2_floor(x - 1) + 1; }
+
+uint32_t batch_tune_key(int f, uint32_t jobs, uint32_t mean_len) {
+  return (uint32_t)f | (std::min(31u, log2_ceil(mean_len)) << 3) |
+         (std::min(15u, log2_floor(jobs)) << 8);
+}
+
+// Jobs and mean length of family f in a host table.
+static void family_shape(const char* host, int f, uint32_t* jobs, uint32_t* mean_len) {
+  const BatchHeader* hdr = (const BatchHeader*)host;
+  const BatchItem* items = (const BatchItem*)(host + sizeof(BatchHeader));
+  const uint32_t a = hdr->start[f], b = hdr->start[f + 1];
+  uint64_t total = 0;
+  for (uint32_t i = a; i < b; ++i) total += items[i].n;
+  *jobs = b - a;
+  *mean_len = b > a ? (uint32_t)std::min<uint64_t>(total / (b - a), 0xffffffffu) : 0u;
+}
+
+static int family_choice(msv_executor* ex, const char* host, int f) {
+  uint32_t jobs, mean;
+  family_shape(host, f, &jobs, &mean);
+  const int c = tune_get(ex, "batch", batch_tune_key(f, jobs, mean));
+  return c >= 0 && c < kBatchGridCands ? c : 0;
+}
+
+static uint32_t family_grid(msv_executor* ex, int f, int choice) {
+  return (uint32_t)std::max(1, family_per_cu(ex, f) >> choice) * (uint32_t)ex->num_cus;
+}
+
+uint32_t batch_shape_code(msv_executor* ex, const char* host, uint32_t mask) {
+  uint32_t code = mask;
+  for (int f = 0; f < kBatchFamilies; ++f)
+    if (mask & (1u << f)) code |= (uint32_t)family_choice(ex, host, f) << (8 + 2 * f);
+  return code;
+}
+
+// Byte ranges a job reads and writes (empty ranges for n == 0).
+struct ByteRange {
+  uintptr_t a, b;
+};
+static void job_ranges(const BatchItem& it, std::vector<ByteRange>* rd, std::vector<ByteRange>* wr) {
+  if (it.n == 0) return;
+  const uintptr_t es = dtype_size((msv_dtype)it.dtype), n = it.n;
+  auto add = [](std::vector<ByteRange>* v, const void* p, uintptr_t bytes) {
+    if (p && bytes) v->push_back({(uintptr_t)p, (uintptr_t)p + bytes});
+  };
+  add(rd, it.in, n * es);
+  switch (it.kind) {
+    case MSV_JOB_REDUCE: add(wr, it.out, es); break;
+    case MSV_JOB_COPY_WHERE:
+      add(wr, it.out, n * es);
+      add(wr, it.out2, 4);
+      break;
+    case MSV_JOB_SORT_BY_KEY:
+      add(rd, it.in2, n * it.vbytes);
+      add(wr, it.out2, n * es);
+      add(wr, it.out, n * it.vbytes);
+      break;
+    default: add(wr, it.out, n * es); break;
+  }
+}
+
+// Whether no job of family f writes bytes that a job of the family reads, so
+// that running the family again gives the same outputs.
+static bool family_rerunnable(const char* host, int f) {
+  const BatchHeader* hdr = (const BatchHeader*)host;
+  const BatchItem* items = (const BatchItem*)(host + sizeof(BatchHeader));
+  std::vector<ByteRange> rd, wr;
+  for (uint32_t i = hdr->start[f]; i < hdr->start[f + 1]; ++i) job_ranges(items[i], &rd, &wr);
+  std::sort(rd.begin(), rd.end(), [](const ByteRange& x, const ByteRange& y) { return x.a < y.a; });
+  std::vector<uintptr_t> end_max(rd.size());
+  for (size_t i = 0; i < rd.size(); ++i) end_max[i] = std::max(rd[i].b, i ? end_max[i - 1] : 0);
+  for (const ByteRange& w : wr) {
+    // Reads starting before w.b: the one reaching furthest must end at or before w.a.
+    const size_t k = std::lower_bound(rd.begin(), rd.end(), w.b,
+                                      [](const ByteRange& r, uintptr_t v) { return r.a < v; }) -
+                     rd.begin();
+    if (k && end_max[k - 1] > w.a) return false;
+  }
+  return true;
+}
+
+static msv_status launch_family(hipStream_t stream, char* dev, int f, uint32_t grid);
+
+msv_status batch_autotune(msv_executor* ex, hipStream_t stream, const char* host, char* dev,
+                          uint32_t capacity, uint32_t mask) {
+  if (!ex->opt_autotune || ex->capturing) return MSV_OK;
+  const BatchHeader* hdr = (const BatchHeader*)host;
+  bool uploaded = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  msv_status st = MSV_OK;
+  for (int f = 0; f < kBatchFamilies && st == MSV_OK; ++f) {
+    if (!(mask & (1u << f))) continue;
+    uint32_t jobs, mean;
+    family_shape(host, f, &jobs, &mean);
+    const uint32_t key = batch_tune_key(f, jobs, mean);
+    if (jobs < 16 || tune_get(ex, "batch", key) >= 0 || !family_rerunnable(host, f)) continue;
+    if (!uploaded) {
+      uploaded = true;
+      if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+          hipMemcpyAsync(dev, host, batch_table_bytes(capacity), hipMemcpyHostToDevice, stream) !=
+              hipSuccess) {
+        st = MSV_ERR_LAUNCH;
+        break;
+      }
+      const BatchGen* gens =
+          (const BatchGen*)(dev + sizeof(BatchHeader) + (size_t)capacity * sizeof(BatchItem));
+      hipLaunchKernelGGL(batch_gen_kernel, dim3((uint32_t)ex->num_cus * 4u), dim3(256), 0, stream,
+                         gens, (const BatchHeader*)dev);
+    }
+    int best = 0;
+    float best_ms = 1e30f;
+    uint32_t last_grid = 0;
+    for (int cand = 0; cand < kBatchGridCands && st == MSV_OK; ++cand) {
+      const uint32_t grid = family_grid(ex, f, cand);
+      if (grid == last_grid) continue;
+      last_grid = grid;
+      float ms_min = 1e30f;
+      // One untimed run, then the faster of two timed runs; each starts from
+      // a queue counter of zero (the header is uploaded again).
+      for (int rep = 0; rep < 3 && st == MSV_OK; ++rep) {
+        if (hipMemcpyAsync(dev, host, sizeof(BatchHeader), hipMemcpyHostToDevice, stream) !=
+                hipSuccess ||
+            (rep && hipEventRecord(e0, stream) != hipSuccess)) {
+          st = MSV_ERR_LAUNCH;
+          break;
+        }
+        st = launch_family(stream, dev, f, grid);
+        if (st != MSV_OK || !rep) continue;
+        float ms = 0;
+        if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          st = MSV_ERR_LAUNCH;
+          break;
+        }
+        ms_min = std::min(ms_min, ms);
+      }
+      if (ms_min < best_ms) {
+        best_ms = ms_min;
+        best = cand;
+      }
+    }
+    if (st == MSV_OK) tune_put(ex, "batch", key, best);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st != MSV_OK) {
+    (void)hipGetLastError();
+    set_error(st, 0, 0, "batch autotuning failed");
+  }
+  (void)hdr;
+  return st;
+}
+
+// One persistent launch of family f of the table at `dev`.
+static msv_status launch_family(hipStream_t stream, char* dev, int f, uint32_t grid) {
   const BatchItem* items = (const BatchItem*)(dev + sizeof(BatchHeader));
   BatchHeader* hdr = (BatchHeader*)dev;
-  for (int f = 0; f < kBatchFamilies; ++f) {
-    if (!(mask & (1u << f))) continue;
-    const dim3 grid(grid_of(f)), block(kBatchBlock);
-    switch (f) {
-      case 0: hipLaunchKernelGGL(batch_stream_kernel, grid, block, 0, stream, items, hdr); break;
-      case 1: hipLaunchKernelGGL((batch_sort_kernel<1, 1>), grid, block, 0, stream, items, hdr); break;
-      case 2: hipLaunchKernelGGL((batch_sort_kernel<1, 2>), grid, block, 0, stream, items, hdr); break;
-      case 3: hipLaunchKernelGGL((batch_sort_kernel<1, 3>), grid, block, 0, stream, items, hdr); break;
-      case 4: hipLaunchKernelGGL((batch_sort_kernel<2, 2>), grid, block, 0, stream, items, hdr); break;
-      case 5: hipLaunchKernelGGL((batch_sort_kernel<2, 3>), grid, block, 0, stream, items, hdr); break;
-      default: hipLaunchKernelGGL((batch_sort_kernel<2, 4>), grid, block, 0, stream, items, hdr); break;
-    }
-    MSV_HIP(hipGetLastError());
+  const dim3 g(grid), block(kBatchBlock);
+  switch (f) {
+    case 0: hipLaunchKernelGGL(batch_stream_kernel, g, block, 0, stream, items, hdr); break;
+    case 1: hipLaunchKernelGGL((batch_sort_kernel<1, 1>), g, block, 0, stream, items, hdr); break;
+    case 2: hipLaunchKernelGGL((batch_sort_kernel<1, 2>), g, block, 0, stream, items, hdr); break;
+    case 3: hipLaunchKernelGGL((batch_sort_kernel<1, 3>), g, block, 0, stream, items, hdr); break;
+    case 4: hipLaunchKernelGGL((batch_sort_kernel<2, 2>), g, block, 0, stream, items, hdr); break;
+    case 5: hipLaunchKernelGGL((batch_sort_kernel<2, 3>), g, block, 0, stream, items, hdr); break;
+    default: hipLaunchKernelGGL((batch_sort_kernel<2, 4>), g, block, 0, stream, items, hdr); break;
   }
-  (void)ex;
+  MSV_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
+// The family launches of a table at `dev`; grid_of(f) gives each grid.
+template <typename GridOf>
+static msv_status launch_families(hipStream_t stream, char* dev, uint32_t mask, GridOf grid_of) {
+  for (int f = 0; f < kBatchFamilies; ++f)
+    if (mask & (1u << f)) MSV_TRY(launch_family(stream, dev, f, grid_of(f)));
   return MSV_OK;
 }
 
 msv_status batch_table_enqueue(msv_executor* ex, hipStream_t stream, const char* host, char* dev,
                                uint32_t capacity, uint32_t mask, uint32_t* d_counts,
                                uint32_t* h_counts) {
-  int per_cu[kBatchFamilies];
-  for (int f = 0; f < kBatchFamilies; ++f) per_cu[f] = family_per_cu(ex, f);
+  uint32_t grid[kBatchFamilies];
+  for (int f = 0; f < kBatchFamilies; ++f)
+    grid[f] = (mask & (1u << f)) ? family_grid(ex, f, family_choice(ex, host, f)) : 0u;
   MSV_HIP(hipMemcpyAsync(dev, host, batch_table_bytes(capacity), hipMemcpyHostToDevice, stream));
   const BatchHeader* hdr = (const BatchHeader*)dev;
   const BatchGen* gens = (const BatchGen*)(dev + sizeof(BatchHeader) + (size_t)capacity * sizeof(BatchItem));
   hipLaunchKernelGGL(batch_gen_kernel, dim3((uint32_t)ex->num_cus * 4u), dim3(256), 0, stream, gens,
                      hdr);
   MSV_HIP(hipGetLastError());
-  MSV_TRY(launch_families(ex, stream, dev, mask,
-                          [&](int f) { return (uint32_t)(per_cu[f] * ex->num_cus); }));
+  MSV_TRY(launch_families(stream, dev, mask, [&](int f) { return grid[f]; }));
   hipLaunchKernelGGL(batch_counts_kernel, dim3(1), dim3(256), 0, stream, gens, hdr, d_counts);
   MSV_HIP(hipGetLastError());
   MSV_HIP(hipMemcpyAsync(h_counts, d_counts, sizeof(uint32_t) * capacity, hipMemcpyDeviceToHost,
@@ -562,14 +732,17 @@ msv_status run_jobs(Ctx& c, const msv_job* jobs, uint32_t count) {
     MSV_TRY(batch_table_fill(c, small.data(), cap, nullptr, 0, cap, host.data(), &mask));
     char* dev = (char*)c.scratch(host.size());
     if (!dev) return MSV_ERR_OUT_OF_MEMORY;
+    msv_executor* ex = c.ex();
+    // Families whose shape class has no tuned grid yet are timed first (a
+    // one-time host synchronisation per class; skipped while capturing).
+    MSV_TRY(batch_autotune(ex, c.stream(), host.data(), dev, cap, mask));
     const void* src = c.staging(host.data(), host.size());
     if (!src) return MSV_ERR_OUT_OF_MEMORY;
     MSV_HIP(hipMemcpyAsync(dev, src, host.size(), hipMemcpyHostToDevice, c.stream()));
     const BatchHeader* hh = (const BatchHeader*)host.data();
-    msv_executor* ex = c.ex();
-    MSV_TRY(launch_families(ex, c.stream(), dev, mask, [&](int f) {
+    MSV_TRY(launch_families(c.stream(), dev, mask, [&](int f) {
       const uint32_t n = hh->start[f + 1] - hh->start[f];
-      return std::min<uint32_t>(n, (uint32_t)(family_per_cu(ex, f) * ex->num_cus));
+      return std::min<uint32_t>(n, family_grid(ex, f, family_choice(ex, host.data(), f)));
     }));
   }
   for (uint32_t i : large) MSV_TRY(large_job(c, jobs[i]));
@@ -587,6 +760,11 @@ msv_status run_jobs_plain(Ctx& c, const msv_job* jobs, uint32_t count) {
 using namespace msv;
 
 extern "C" {
+
+int32_t msv_batch_tuned_grid(uint32_t family, uint32_t jobs, uint32_t mean_len) {
+  if (family >= (uint32_t)kBatchFamilies) return -1;
+  return tune_get_current("batch", batch_tune_key((int)family, jobs, mean_len));
+}
 
 msv_status msv_run_jobs(msv_executor* ex, const msv_job* jobs, uint32_t count) {
   clear_error();

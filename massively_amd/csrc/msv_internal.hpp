@@ -108,9 +108,10 @@ struct msv_executor {
   int64_t opt_radix_onesweep = -1;
   int64_t opt_segsort_wave = 1;
   // Resident blocks per CU of each batched job-kernel family (0 = not yet
-  // queried). Per executor, so per device, and touched only by the thread
-  // that submits on this executor.
-  int batch_per_cu[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // queried). Per executor, so per device; atomic because server scheduler
+  // threads and user threads may launch batches on one executor at once
+  // (they store the same value).
+  std::atomic<int> batch_per_cu[8] = {};
   // HIP graph capture (msv_executor_begin_capture): while capturing, call
   // scratch and pinned staging are kept for the graph's lifetime.
   bool capturing = false;
@@ -305,11 +306,25 @@ bool batch_job_small(const msv_job& j);
 msv_status batch_table_fill(Ctx& c, const msv_job* jobs, uint32_t count, const BatchGen* gens,
                             uint32_t ngen, uint32_t capacity, char* host, uint32_t* mask);
 // Enqueues on `stream`: the upload of the host table to `dev`, the input
-// generation, one persistent launch per family in `mask` (resident-size
-// grids), the count gather into d_counts and its download to h_counts
+// generation, one persistent launch per family in `mask` (autotuned
+// grids, resident-size until tuned), the count gather into d_counts and its download to h_counts
 // (capacity words each). Capturable: no memset, no allocation.
 // Resolves the batched kernels' occupancy (before a capture of the enqueue).
 void batch_occupancy_warm(msv_executor* ex);
+// Tuning-table key of a family's grid for a batch of `jobs` jobs of mean
+// length `mean_len`.
+uint32_t batch_tune_key(int family, uint32_t jobs, uint32_t mean_len);
+// Times each grid candidate for the families in `mask` whose shape class in
+// the host table has no tuned grid (at least 16 jobs, no job writing what a
+// job of its family reads), running them on `stream` from `dev` (the table is
+// uploaded and its inputs generated first), and records the fastest.
+// Synchronises with the stream when it times anything; a no-op while
+// capturing or with autotuning off.
+msv_status batch_autotune(msv_executor* ex, hipStream_t stream, const char* host, char* dev,
+                          uint32_t capacity, uint32_t mask);
+// `mask` with the tuned grid choice of each family present (2 bits per
+// family from bit 8): batches with the same code launch the same grids.
+uint32_t batch_shape_code(msv_executor* ex, const char* host, uint32_t mask);
 msv_status batch_table_enqueue(msv_executor* ex, hipStream_t stream, const char* host, char* dev,
                                uint32_t capacity, uint32_t mask, uint32_t* d_counts,
                                uint32_t* h_counts);

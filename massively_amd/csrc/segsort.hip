@@ -22,11 +22,31 @@
 // candidates, chosen per (key width, element width, mean segment length
 // class) by timing them on first use.
 #include <algorithm>
+#include <atomic>
 
 #include "msv_internal.hpp"
 #include "segsort_huge.hpp"
 #include "segsort_small.hpp"
 #include "tile_sort.hpp"
+
+namespace {
+
+// Resident blocks per CU of one kernel instantiation. The caller keeps one
+// cache per instantiation; concurrent first calls (scheduler threads, user
+// threads) compute and store the same value, so a relaxed atomic is enough.
+template <typename Kern>
+int blocks_per_cu(std::atomic<int>& cache, Kern kern, int block) {
+  int pc = cache.load(std::memory_order_relaxed);
+  if (pc == 0) {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, block, 0) != hipSuccess) b = 1;
+    pc = std::max(1, b);
+    cache.store(pc, std::memory_order_relaxed);
+  }
+  return pc;
+}
+
+}  // namespace
 
 namespace msv {
 
@@ -226,13 +246,8 @@ static msv_status long_launch(Ctx& c, hipStream_t st, const LongPlan<KW>& lp) {
   hipLaunchKernelGGL((huge_starts_kernel<P>), dim3(std::min(lp.nlong_max, ncu)), dim3(kRadixBins),
                      0, st, h);
   MSV_HIP(hipGetLastError());
-  static int per_cu = 0;  // resident blocks per CU of this instantiation
-  if (per_cu == 0) {
-    int b = 0;
-    MSV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, huge_pass_kernel<KW, EW>,
-                                                         kHugeBlock, 0));
-    per_cu = std::max(1, b);
-  }
+  static std::atomic<int> per_cu_cache{0};  // resident blocks per CU of this instantiation
+  const int per_cu = blocks_per_cu(per_cu_cache, huge_pass_kernel<KW, EW>, kHugeBlock);
   const uint32_t pgrid = std::min<uint32_t>(lp.tiles_max, (uint32_t)per_cu * ncu);
   for (int p = 0; p < P; ++p) {
     hipLaunchKernelGGL((huge_pass_kernel<KW, EW>), dim3(pgrid), dim3(kHugeBlock), 0, st, h, p,
@@ -306,12 +321,8 @@ static msv_status segsort_wave_go(Ctx& c, const SegSortArgs& base, uint32_t n, u
       MSV_TRY((long_prepare<KW, EW>(c, base, n, long_min, longs, hctl, s0, s1, &lp)));
     }
     auto kern = segsort_kernel<KW, EW, BLOCK, IPT, DBITS>;
-    static int per_cu = 0;  // resident blocks per CU of this instantiation
-    if (longer && per_cu == 0) {
-      int bpc = 0;
-      MSV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, BLOCK, 0));
-      per_cu = std::max(1, bpc);
-    }
+    static std::atomic<int> per_cu_cache{0};  // resident blocks per CU of this instantiation
+    const int per_cu = longer ? blocks_per_cu(per_cu_cache, kern, BLOCK) : 1;
     hipStream_t aux = nullptr;
     if (longer) {
       aux = c.fork();
@@ -404,12 +415,8 @@ static msv_status segsort_go(Ctx& c, const SegSortArgs& base, uint32_t n, uint32
       a.klass = SEG_ITEMS_WINDOWS;
       hipLaunchKernelGGL(kern, dim3(nwin), dim3(BLOCK), 0, c.stream(), a);
     } else {
-      static int per_cu = 0;  // resident blocks per CU of this instantiation
-      if (per_cu == 0) {
-        int b = 0;
-        MSV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, BLOCK, 0));
-        per_cu = std::max(1, b);
-      }
+      static std::atomic<int> per_cu_cache{0};  // resident blocks per CU of this instantiation
+      const int per_cu = blocks_per_cu(per_cu_cache, kern, BLOCK);
       a.klass = SEG_ITEMS_ALL;
       const uint32_t grid = std::min<uint32_t>(2 * nwin, (uint32_t)(per_cu * c.num_cus()));
       hipLaunchKernelGGL(kern, dim3(std::max(1u, grid)), dim3(BLOCK), 0, c.stream(), a);

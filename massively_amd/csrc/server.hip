@@ -146,9 +146,13 @@ static msv_status launch_batch_graph(msv_server* s, int k, Ctx& c, std::vector<m
   uint32_t mask = 0;
   MSV_TRY(batch_table_fill(c, jobs.data(), (uint32_t)jobs.size(), gens.data(),
                            (uint32_t)gens.size(), cap, sl.h_table, &mask));
-  auto it = sl.graphs.find(mask);
+  // The first batch of a shape class times the grid candidates (one host
+  // synchronisation per class); graphs are kept per family set and grids.
+  batch_occupancy_warm(ex);
+  MSV_TRY(batch_autotune(ex, c.stream(), sl.h_table, sl.d_table, cap, mask));
+  const uint32_t code = batch_shape_code(ex, sl.h_table, mask);
+  auto it = sl.graphs.find(code);
   if (it == sl.graphs.end()) {
-    batch_occupancy_warm(ex);
     MSV_HIP(hipStreamBeginCapture(c.stream(), hipStreamCaptureModeThreadLocal));
     const msv_status st =
         batch_table_enqueue(ex, c.stream(), sl.h_table, sl.d_table, cap, mask, sl.d_counts,
@@ -161,7 +165,7 @@ static msv_status launch_batch_graph(msv_server* s, int k, Ctx& c, std::vector<m
     const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     MSV_HIP(ie);
-    it = sl.graphs.emplace(mask, exec).first;
+    it = sl.graphs.emplace(code, exec).first;
   }
   MSV_HIP(hipGraphLaunch(it->second, c.stream()));
   return MSV_OK;

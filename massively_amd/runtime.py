@@ -295,6 +295,15 @@ def segsort_tuned_config(key_bytes: int, value_bytes: int, launch: int, n: int, 
     return int(f(key_bytes, value_bytes, launch, n, nseg))
 
 
+def batch_tuned_grid(family: int, jobs: int, mean_len: int) -> int:
+    """Grid candidate the autotuner chose on the current device for batched
+    job-kernel family ``family`` (0 = scan / reduce / copy_where, 1..6 =
+    sort_by_key by key and value width) in a batch of ``jobs`` jobs of mean
+    length ``mean_len`` (-1: not tuned). Candidate c launches the family's
+    resident blocks per CU divided by 2**c."""
+    return int(L.lib().msv_batch_tuned_grid(family, jobs, mean_len))
+
+
 def tuning_save(path: str) -> None:
     """Writes the autotuner's table (every device's entries) to ``path``."""
     check(L.lib().msv_tuning_save(str(path).encode()))

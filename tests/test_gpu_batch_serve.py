@@ -287,3 +287,62 @@ def test_server_large_requests_alone_equals_mix():
                 assert abs(float(a) - s) <= oracle.f32_sum_bound(x.size, mag)
             else:
                 assert a == oracle.reduce(x, 0, "add")
+
+
+def test_run_jobs_grid_autotune(exec_, tmp_path):
+    """The batched kernels' grid is tuned on the first batch of a (family, job
+    count, mean length) class: the choice is recorded, survives a save / clear
+    / load of the tuning table, and the results are the bytes of a run with
+    autotuning off (each job runs inside one workgroup whatever the grid)."""
+    m.tuning_clear()
+    n_jobs, n = 300, 3000
+    xs = [_data(np.uint32, n) for _ in range(n_jobs)]
+    ks = [(_data(np.uint32, n) % 5000).astype(np.uint32) for _ in range(n_jobs)]
+    vs = np.arange(n, dtype=np.uint32)
+    dx = [exec_.to_device(x) for x in xs]
+    dk = [exec_.to_device(k) for k in ks]
+    dv = exec_.to_device(vs)
+
+    def batch():
+        jobs = [serve.Job("inclusive_scan", d, exec_.alloc(np.uint32, n), op=m.op.Add) for d in dx]
+        jobs += [serve.Job("sort_by_key", d, exec_.alloc(np.uint32, n), op=m.op.Less, inp2=dv,
+                           out2=exec_.alloc(np.uint32, n)) for d in dk]
+        return jobs
+
+    def results(jobs):
+        return [exec_.to_host(j.out).tobytes() + (exec_.to_host(j.out2).tobytes() if j.out2 is not None
+                                                   else b"") for j in jobs]
+
+    exec_.set_option(m.Executor.OPT_AUTOTUNE, 0)
+    try:
+        plain = batch()
+        serve.run_jobs(exec_, plain)
+        want = results(plain)
+        assert m.batch_tuned_grid(0, n_jobs, n) == -1
+    finally:
+        exec_.set_option(m.Executor.OPT_AUTOTUNE, 1)
+    tuned = batch()
+    serve.run_jobs(exec_, tuned)
+    assert results(tuned) == want
+    c0, c2 = m.batch_tuned_grid(0, n_jobs, n), m.batch_tuned_grid(2, n_jobs, n)
+    assert 0 <= c0 < 4 and 0 <= c2 < 4
+    path = tmp_path / "tune.txt"
+    m.tuning_save(path)
+    m.tuning_clear()
+    assert m.batch_tuned_grid(0, n_jobs, n) == -1
+    m.tuning_load(path)
+    assert (m.batch_tuned_grid(0, n_jobs, n), m.batch_tuned_grid(2, n_jobs, n)) == (c0, c2)
+    again = batch()
+    serve.run_jobs(exec_, again)
+    assert results(again) == want
+    for i in (0, 7, n_jobs - 1):
+        assert results([again[i]])[0] == oracle.inclusive_scan(xs[i], "add").tobytes()
+        perm = oracle.sort_perm(ks[i], "less")
+        assert results([again[n_jobs + i]])[0] == vs[perm].tobytes() + ks[i][perm].tobytes()
+    # In-place jobs are not rerunnable: their class stays untuned, results exact.
+    m.tuning_clear()
+    inplace = [serve.Job("inclusive_scan", d, d, op=m.op.Add) for d in dx[:64]]
+    serve.run_jobs(exec_, inplace)
+    assert m.batch_tuned_grid(0, 64, n) == -1
+    for i in (0, 63):
+        assert exec_.to_host(dx[i]).tobytes() == oracle.inclusive_scan(xs[i], "add").tobytes()
