@@ -23,8 +23,8 @@
 //    and finds each digit's global position by a decoupled look-back over the
 //    earlier tiles (4 status words per step); elements of 12 or 16 bytes go
 //    through LDS as keys and then values, and stay in separate key and value
-//    arrays between passes; 12-byte elements run 1024 x 4 tiles at 8 waves
-//    per SIMD (2^27 u64 keys + u32 values: uniform 10.28 ms, Zipf 1.1
+//    arrays between passes; 12- and 16-byte elements run 1024 x 4 tiles at
+//    8 waves per SIMD (2^27 u64 keys + u32 values: uniform 10.28 ms, Zipf 1.1
 //    10.10 ms, constant top byte 9.01 ms);
 //  * reduce-then-scan (11.86 / 10.84 / 10.43 ms there, 8.22 ms against 7.80
 //    for 2^28 u32 + u32; kept behind the option): pass 0's upsweep also records per
@@ -47,16 +47,17 @@ template <int EW>
 constexpr int radix_block() { return 512; }
 template <int EW>
 constexpr int radix_ipt() { return EW <= 2 ? 16 : 8; }
-// Onesweep passes over 12-byte elements (8-byte keys + 4-byte values) use
-// 1024 x 4 (4096-element tiles, as 512 x 8): 50 VGPRs and 52 KB of LDS give
-// two blocks and 8 waves per SIMD where 512 x 8 (78 VGPRs) holds 6. Measured
-// at 2^27 u64 + u32: uniform 10.41 -> 10.28 ms, Zipf 1.1 10.58 -> 10.10 ms,
-// top byte constant 9.13 -> 9.01 ms; 512 x 4 (13.99 ms) and 512 x 6 (64 VGPRs
-// with spills, 12.92 ms) were slower, 1024 x 6 no faster.
+// Onesweep passes over 12- and 16-byte elements (8-byte keys + 4- or 8-byte
+// values) use 1024 x 4 (4096-element tiles, as 512 x 8): 50 / 54 VGPRs and
+// 52 KB of LDS give two blocks and 8 waves per SIMD where 512 x 8 (78 VGPRs
+// for 12 bytes) holds 6. Measured at 2^27 u64 + u32: uniform 10.41 -> 10.28
+// ms, Zipf 1.1 10.58 -> 10.10 ms, top byte constant 9.13 -> 9.01 ms; 512 x 4
+// (13.99 ms) and 512 x 6 (64 VGPRs with spills, 12.92 ms) were slower, 1024 x
+// 6 no faster. 2^27 u64 + u64: 12.46 -> 11.14 ms.
 template <int EW>
-constexpr int radix_onesweep_block() { return EW == 3 ? 1024 : radix_block<EW>(); }
+constexpr int radix_onesweep_block() { return EW >= 3 ? 1024 : radix_block<EW>(); }
 template <int EW>
-constexpr int radix_onesweep_ipt() { return EW == 3 ? 4 : radix_ipt<EW>(); }
+constexpr int radix_onesweep_ipt() { return EW >= 3 ? 4 : radix_ipt<EW>(); }
 
 template <typename K, int MODE, int VS>
 static msv_status radix_sort_impl(Ctx& c, const void* keys, const void* vals, uint32_t n,
