@@ -23,8 +23,8 @@
 //    and finds each digit's global position by a decoupled look-back over the
 //    earlier tiles (4 status words per step); elements of 12 or 16 bytes go
 //    through LDS as keys and then values, and stay in separate key and value
-//    arrays between passes; 12- and 16-byte elements run 1024 x 4 tiles at
-//    8 waves per SIMD (2^27 u64 keys + u32 values: uniform 10.28 ms, Zipf 1.1
+//    arrays between passes; every pass runs 1024-thread tiles at 8 waves
+//    per SIMD (2^27 u64 keys + u32 values: uniform 10.28 ms, Zipf 1.1
 //    10.10 ms, constant top byte 9.01 ms);
 //  * reduce-then-scan (11.86 / 10.84 / 10.43 ms there, 8.22 ms against 7.80
 //    for 2^28 u32 + u32; kept behind the option): pass 0's upsweep also records per
@@ -39,25 +39,29 @@
 namespace msv {
 
 // ---- host -------------------------------------------------------------------
-// Tile shape per element width, from a sweep of (threads, keys per thread) at
-// 2^28 u32 and 2^27 u64 keys on MI355X: 512 x 16 for 4- and 8-byte elements
+// Tile shape of the reduce-then-scan passes per element width, from a sweep
+// of (threads, keys per thread) at 2^28 u32 and 2^27 u64 keys on MI355X:
+// 512 x 16 for 4- and 8-byte elements
 // (8192-key tiles, 31-39 Gkeys/s), 512 x 8 for 12- and 16-byte elements (LDS
 // holds the whole tile, so wider elements take fewer keys per thread).
 template <int EW>
 constexpr int radix_block() { return 512; }
 template <int EW>
 constexpr int radix_ipt() { return EW <= 2 ? 16 : 8; }
-// Onesweep passes over 12- and 16-byte elements (8-byte keys + 4- or 8-byte
-// values) use 1024 x 4 (4096-element tiles, as 512 x 8): 50 / 54 VGPRs and
-// 52 KB of LDS give two blocks and 8 waves per SIMD where 512 x 8 (78 VGPRs
-// for 12 bytes) holds 6. Measured at 2^27 u64 + u32: uniform 10.41 -> 10.28
-// ms, Zipf 1.1 10.58 -> 10.10 ms, top byte constant 9.13 -> 9.01 ms; 512 x 4
-// (13.99 ms) and 512 x 6 (64 VGPRs with spills, 12.92 ms) were slower, 1024 x
-// 6 no faster. 2^27 u64 + u64: 12.46 -> 11.14 ms.
+// Onesweep passes run 1024-thread blocks: 1024 x 8 for 4-byte elements,
+// 1024 x 6 for 8-byte and 1024 x 4 for 12- and 16-byte elements. Each shape
+// fits two blocks' LDS per CU (52-68 KB each) at 46-61 VGPRs, so 8 waves per
+// SIMD, where the 512-thread tiles (512 x 16, and 512 x 8 at 78 VGPRs for 12
+// bytes) held 4-6. Measured against those: 2^28 u32 keys 6.53 -> 5.58 ms,
+// 2^28 u32 + u32 7.80 -> 7.35 ms, 2^27 u64 keys 7.61 -> 6.96 ms, 2^27 u64 +
+// u32 uniform 10.41 -> 10.28 ms (Zipf 1.1 10.58 -> 10.10 ms, constant top
+// byte 9.13 -> 9.01 ms), 2^27 u64 + u64 12.46 -> 11.14 ms. Slower or no
+// faster: 1024 x 4 for 8-byte elements (9.05 ms for u32 + u32), 512 x 4 and
+// 512 x 6 (spills) and 1024 x 6 for 12-byte elements.
 template <int EW>
-constexpr int radix_onesweep_block() { return EW >= 3 ? 1024 : radix_block<EW>(); }
+constexpr int radix_onesweep_block() { return 1024; }
 template <int EW>
-constexpr int radix_onesweep_ipt() { return EW >= 3 ? 4 : radix_ipt<EW>(); }
+constexpr int radix_onesweep_ipt() { return EW >= 3 ? 4 : (EW == 2 ? 6 : 8); }
 
 template <typename K, int MODE, int VS>
 static msv_status radix_sort_impl(Ctx& c, const void* keys, const void* vals, uint32_t n,
