@@ -49,19 +49,20 @@ constexpr int radix_block() { return 512; }
 template <int EW>
 constexpr int radix_ipt() { return EW <= 2 ? 16 : 8; }
 // Onesweep passes run 1024-thread blocks: 1024 x 8 for 4-byte elements,
-// 1024 x 6 for 8-byte and 1024 x 4 for 12- and 16-byte elements. Each shape
-// fits two blocks' LDS per CU (52-68 KB each) at 46-61 VGPRs, so 8 waves per
+// 1024 x 6 for 8-byte and 1024 x 5 for 12- and 16-byte elements. Each shape
+// fits two blocks' LDS per CU (52-68 KB each) at 54-63 VGPRs, so 8 waves per
 // SIMD, where the 512-thread tiles (512 x 16, and 512 x 8 at 78 VGPRs for 12
 // bytes) held 4-6. Measured against those: 2^28 u32 keys 6.53 -> 5.58 ms,
 // 2^28 u32 + u32 7.80 -> 7.35 ms, 2^27 u64 keys 7.61 -> 6.96 ms, 2^27 u64 +
-// u32 uniform 10.41 -> 10.28 ms (Zipf 1.1 10.58 -> 10.10 ms, constant top
-// byte 9.13 -> 9.01 ms), 2^27 u64 + u64 12.46 -> 11.14 ms. Slower or no
-// faster: 1024 x 4 for 8-byte elements (9.05 ms for u32 + u32), 512 x 4 and
-// 512 x 6 (spills) and 1024 x 6 for 12-byte elements.
+// u32 10.18 -> 9.33 ms, 2^27 u64 + u64 12.46 -> 10.53 ms. Slower: any shape
+// above 64 VGPRs (one block per CU: 1024 x 10 or 12 for 4-byte, 1024 x 7 for
+// 8-byte elements), smaller tiles at the same occupancy (1024 x 4 for 8-byte
+// elements: 9.05 ms for u32 + u32; 1024 x 4 for 12-byte: 10.18 ms), and
+// 512 x 4 / 512 x 6 for 12-byte elements.
 template <int EW>
 constexpr int radix_onesweep_block() { return 1024; }
 template <int EW>
-constexpr int radix_onesweep_ipt() { return EW >= 3 ? 4 : (EW == 2 ? 6 : 8); }
+constexpr int radix_onesweep_ipt() { return EW >= 3 ? 5 : (EW == 2 ? 6 : 8); }
 
 template <typename K, int MODE, int VS>
 static msv_status radix_sort_impl(Ctx& c, const void* keys, const void* vals, uint32_t n,
