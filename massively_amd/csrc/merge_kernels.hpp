@@ -47,15 +47,17 @@
 
 namespace msv {
 
-// Merge: 512-thread blocks (4.5% faster than 256 at 2 x 2^27 u32); the set
-// kernel keeps 256 (512 was 30% slower there).
+// Merge: 512-thread blocks (4.5% faster than 256 at 2 x 2^27 u32).
 constexpr int kMergeBlock = 512;
-constexpr int kSetBlock = 256;
 constexpr int kMergeIpt = 16;
 constexpr uint32_t kMergeTile = kMergeBlock * kMergeIpt;
 // Set operations emit through a decoupled look-back, so they use larger
-// tiles (fewer tiles in the look-back chain).
-constexpr int kSetIpt = 32;
+// tiles (fewer tiles in the look-back chain): 512 x 16 = 8192 elements, at
+// 60 VGPRs and 34 KB of LDS (8 waves per SIMD). 256 x 32 (the same tile at
+// 86-90 VGPRs, 4 waves per SIMD) was 4-5% slower at 2 x 2^27 u32, 256 x 16
+// (half the tile) 2-8% slower, 512 x 32 30% slower.
+constexpr int kSetBlock = 512;
+constexpr int kSetIpt = 16;
 constexpr uint32_t kSetTile = kSetBlock * kSetIpt;
 // Tile diagonals are kSetSpan apart: moving a split to a pair-aligned point
 // lowers its diagonal by at most one, so a tile holds at most kSetTile inputs.
