@@ -48,16 +48,17 @@ static msv_status select_launch(Ctx& c, const void* in, const void* st, const ms
   }
   const Pred<S> pred = make_pred<S>(p);
   const bool large = n >= kSelLargeMin;
-  const uint32_t tile = (uint32_t)kSelBlock * (large ? sel_rows_large<T>() : kSelRowsSmall) * 4;
+  constexpr int BL = sel_block_large<T, FORM>(), RL = sel_rows_large_of<T, FORM>();
+  const uint32_t tile = large ? (uint32_t)BL * RL * 4 : (uint32_t)kSelBlock * kSelRowsSmall * 4;
   const uint32_t ntiles = div_up(n, tile);
   uint64_t *counter, *status;
   uint32_t gen;
   MSV_TRY(c.lookback(ntiles, &counter, &status, &gen));
   const bool aligned = (((uintptr_t)in | (uintptr_t)st) & 15u) == 0;
   if (large)
-    hipLaunchKernelGGL((select_kernel<T, S, FORM, sel_rows_large<T>()>), dim3(ntiles), dim3(kSelBlock), 0,
-                       c.stream(), (const T*)in, (const S*)st, pred, invert, n, aligned, (T*)out,
-                       out_len, status, gen, ntiles, (const uint32_t*)nullptr);
+    hipLaunchKernelGGL((select_kernel<T, S, FORM, RL, false, Pred<S>, const T* __restrict__, BL>),
+                       dim3(ntiles), dim3(BL), 0, c.stream(), (const T*)in, (const S*)st, pred, invert,
+                       n, aligned, (T*)out, out_len, status, gen, ntiles, (const uint32_t*)nullptr);
   else
     hipLaunchKernelGGL((select_kernel<T, S, FORM, kSelRowsSmall>), dim3(ntiles), dim3(kSelBlock), 0,
                        c.stream(), (const T*)in, (const S*)st, pred, invert, n, aligned, (T*)out,

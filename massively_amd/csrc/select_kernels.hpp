@@ -20,6 +20,19 @@ constexpr int kSelRowsLarge = 4;
 template <typename T>
 constexpr int sel_rows_large() { return sizeof(T) == 4 ? 8 : kSelRowsLarge; }
 constexpr uint32_t kSelLargeMin = 1u << 22;
+// Long inputs of 4-byte elements under the stencil and run-head forms (FORM 0
+// and 3): 1024 threads x 4 rows, the same 16384-element tiles at 51-63 VGPRs
+// and 8 waves per SIMD instead of 4 (124 VGPRs at 512 x 8); 2^28 u32 stencil
+// 10% / 50% 0.520 / 0.602 -> 0.493 / 0.553 ms, unique_by_key 0.513 -> 0.482
+// ms. The predicate forms keep 512 x 8 (10-15% slower at 1024 x 4).
+template <typename T, int FORM>
+constexpr int sel_block_large() {
+  return sizeof(T) == 4 && (FORM == 0 || FORM == 3) ? 1024 : kSelBlock;
+}
+template <typename T, int FORM>
+constexpr int sel_rows_large_of() {
+  return sel_block_large<T, FORM>() == 1024 ? 4 : sel_rows_large<T>();
+}
 
 // FORM 0: u32 stencil array (flag = stencil != 0).
 // FORM 1: predicate on a separate stencil column of type S.
@@ -33,8 +46,8 @@ constexpr uint32_t kSelLargeMin = 1u << 22;
 // functor compiled at run time). Src: the input source (a pointer, or a lazy
 // source compiled at run time).
 template <typename T, typename S, int FORM, int ROWS, bool PART = false, typename PredT = Pred<S>,
-          typename Src = const T* __restrict__>
-__global__ __launch_bounds__(kSelBlock) void select_kernel(Src in,
+          typename Src = const T* __restrict__, int BLOCK = kSelBlock>
+__global__ __launch_bounds__(BLOCK) void select_kernel(Src in,
                                                            const S* __restrict__ stencil,
                                                            PredT pred, bool invert,
                                                            uint32_t n, bool aligned,
@@ -43,8 +56,8 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(Src in,
                                                            uint64_t* status, uint32_t gen,
                                                            uint32_t ntiles,
                                                            const uint32_t* __restrict__ part_total) {
-  constexpr int NW = kSelBlock / kWave;
-  constexpr uint32_t kSelTile = (uint32_t)kSelBlock * ROWS * 4;
+  constexpr int NW = BLOCK / kWave;
+  constexpr uint32_t kSelTile = (uint32_t)BLOCK * ROWS * 4;
   __shared__ T s_stage[kSelTile];
   __shared__ uint32_t s_wave[ROWS][NW];
   __shared__ uint32_t s_prefix;
@@ -58,7 +71,7 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(Src in,
   uint32_t f[ROWS][4];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
-    const size_t i = base + ((size_t)r * kSelBlock + tid) * 4;
+    const size_t i = base + ((size_t)r * BLOCK + tid) * 4;
     load4<T, FORM == 2>(in, i, n, full, x[r]);  // non-temporal when the input is the only stream
     if constexpr (FORM == 0) {
       S s[4];
@@ -144,7 +157,7 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(Src in,
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
     uint32_t pos = pre[r] + shift_up1<uint32_t>(ws[r], 0u);
-    const uint32_t j0 = ((uint32_t)r * kSelBlock + tid) * 4;  // in-tile index of the group
+    const uint32_t j0 = ((uint32_t)r * BLOCK + tid) * 4;  // in-tile index of the group
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (f[r][k]) s_stage[pos] = x[r][k];
@@ -156,11 +169,11 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(Src in,
   }
   __syncthreads();
   const size_t o = s_prefix;
-  for (uint32_t i = tid; i < agg; i += kSelBlock) out[o + i] = s_stage[i];
+  for (uint32_t i = tid; i < agg; i += BLOCK) out[o + i] = s_stage[i];
   if constexpr (PART) {
     const uint32_t cnt = (uint32_t)min((size_t)kSelTile, (size_t)n - base);
     const size_t fo = (size_t)*part_total + (base - o);  // failing elements before the tile
-    for (uint32_t i = agg + tid; i < cnt; i += kSelBlock)
+    for (uint32_t i = agg + tid; i < cnt; i += BLOCK)
       if (fo + (i - agg) < n) out[fo + (i - agg)] = s_stage[i];  // always true when counts agree
   }
 }
