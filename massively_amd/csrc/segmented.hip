@@ -34,7 +34,10 @@ enum SegOut : int { SEG_INCL = 0, SEG_EXCL = 1, SEG_REDUCE = 2, SEG_RUNS = 3 };
 // Rows of 16-byte vectors per thread (512-thread blocks): 4 (8192 u32 / 4096
 // u64 per tile); 2 for SEG_RUNS, whose LDS staging would otherwise limit
 // occupancy. 512-thread blocks ran the seg reduce and reduce_by_key 3-7%
-// faster than 256-thread blocks with the same tiles.
+// faster than 256-thread blocks with the same tiles. 1024-thread blocks with
+// half the rows (the same tiles at 38-52 VGPRs, 8 waves per SIMD instead of
+// 4-5) were slower: seg scan 0.153 -> 0.182 ms, reduce_by_key 0.997 -> 1.532
+// ms at 2^26 / 2^28 u32.
 __host__ __device__ constexpr int seg_rows(int out) { return out == SEG_RUNS ? 2 : 4; }
 enum SegMode : int { SEG_LOOKBACK = 0, SEG_AGGREGATE = 1, SEG_PREFIXED = 2 };
 
